@@ -1,0 +1,253 @@
+"""Benchmark: cell-substeps/s of the full simulation substep at 128 x 192 x 128 on one MI355X.
+
+A step is one substep of the buoyant smoke puff (config 2 of the benchmark plan): emit (while active),
+buoyancy, vorticity confinement, MacCormack advection of u, v, w, projection with multigrid V(2,2) cycles
+to a 1e-5 relative residual (FP64 norm on the device), MacCormack advection of smoke (with the smoke
+budget) and temperature. Inputs live in HBM before the timed region; W warmup substeps (the 12 emitting
+frames by default) run first, then exactly K substeps are timed between device synchronisations.
+
+One JSON line on stdout. Besides the contract's keys it carries:
+  roofline       the dominant kernel (fused red-black sweep at 128 x 192 x 128): algorithmic bytes per
+                 launch / average launch time from HIP events, against 8 TB/s HBM;
+  substep_roofline  B_alg = 4 N (36 + 18.43 C) bytes per substep (C = V-cycles run) x substeps/s / 8 TB/s;
+  raymarch       256 x 256 px, 192 samples per ray, light volume with 64 steps per cell;
+  cpu_baseline   the FP32 CPU oracle (a scalar C port of the same operators, OpenMP) on a bounded sample.
+"""
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "cell-substeps/s at 128×192×128 (+% HBM roofline) vs plain GPU and CPU"
+GRID = (128, 192, 128)
+CELL_M = 0.01
+
+
+def scene_table(cells=GRID, frames=96, radius_cells=12, tolerance=1e-5, cycles=2):
+    """Config 2 mapped onto recipe keys: 1 cm cells, floor closed, normalised temperature (ambient 1,
+    emitter 2) so buoyancy is beta (T - 1) - alpha s with alpha = 0.1, beta = 1; confinement 0.3."""
+    size = [n * CELL_M for n in cells]
+    return {
+        "seed": 7,
+        "domain": {"size_m": size, "cell_m": CELL_M, "ambient_k": 1.0},
+        "time": {"duration_s": frames / 24.0, "frames": frames, "substeps": 4},
+        "emitter": [{"name": "puff", "center_m": [0.0, 0.10, 0.0], "radius_m": radius_cells * CELL_M,
+                     "edge_softness": 0.6, "end_s": 0.5, "smoke_per_s": 2.0, "temperature_k": 2.0}],
+        "forces": {"gravity_mps2": 9.81, "heat_buoyancy": 1.0 / 9.81, "smoke_weight": 0.1 / 9.81,
+                   "vorticity_confinement": 0.3},
+        "solver": {"multigrid_cycles": cycles, "multigrid_sweeps": 2, "multigrid_tolerance": tolerance,
+                   "multigrid_max_cycles": 24},
+        "render": {"glow_start_k": 1.0, "opacity_per_density": 20.0, "light_dir": [0.5, 0.7, 0.5]},
+    }
+
+
+def substep_bytes(cells: int, cycles: float) -> float:
+    """B_alg = 4 N (36 + 18.43 C): the fixed 36 floats per cell plus 16.43 (multigrid) + 2 (FP64 norm) per
+    V-cycle."""
+    return 4.0 * cells * (36.0 + 18.43 * cycles)
+
+
+def sweep_bytes(cells: int) -> float:
+    """One fused red-black sweep: read q and rhs, write q (3 floats per cell)."""
+    return 12.0 * cells
+
+
+def lib_timing(family: str):
+    from plume_amd._lib import LIB
+
+    return LIB, family.encode()
+
+
+def time_kernel_family(run, family: str) -> tuple[float, int]:
+    """Runs `run()` with HIP-event timing of every launch of `family`; returns (avg ms, launches)."""
+    import torch
+
+    lib, name = lib_timing(family)
+    lib.plm_timing_enable(name)
+    try:
+        run()
+        torch.cuda.synchronize()
+        total = ctypes.c_double()
+        count = ctypes.c_int64()
+        rc = lib.plm_timing_read(ctypes.byref(total), ctypes.byref(count))
+        if rc != 0:
+            raise RuntimeError(lib.plm_last_error().decode())
+    finally:
+        lib.plm_timing_disable()
+    return (total.value / count.value if count.value else float("nan")), count.value
+
+
+def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
+    """The FP32 oracle (scalar C port, OpenMP) on the same warm state for `steps` substeps."""
+    import numpy as np
+
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import Oracle
+    from oracle.sim import Stepper, make_grid
+
+    o = Oracle("f32")
+    grid = make_grid(o, derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k)
+    for name in ("u", "v", "w", "smoke", "temperature"):
+        getattr(grid, name).data[...] = getattr(gpu_grid, name).data.cpu().numpy()
+    grid.pressure[...] = gpu_grid.pressure.cpu().numpy()
+    stepper = Stepper(grid, recipe)
+    start = time.perf_counter()
+    for index in range(steps):
+        stepper.step(time_s + index * derived.dt_s, derived.dt_s)
+    elapsed = time.perf_counter() - start
+    assert np.isfinite(grid.smoke.data).all()
+    return elapsed
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=48, help="timed substeps (K)")
+    parser.add_argument("--warmup", type=int, default=48, help="untimed substeps first (W); 48 = 12 frames")
+    parser.add_argument("--cpu-steps", type=int, default=2, help="substeps of the CPU baseline sample (0 = skip)")
+    parser.add_argument("--no-render", action="store_true")
+    parser.add_argument("--tolerance", type=float, default=1e-5)
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from plume_amd.recipe.derived import derive
+    from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
+    from plume_amd.render.camera import make_camera
+    from plume_amd.render.frame_renderer import FrameRenderer, domain_view_params
+    from plume_amd.sim.device import init_device
+    from plume_amd.sim.grid import make_grid
+    from plume_amd.sim.stepper import Stepper
+
+    device = init_device("cuda")
+    recipe = fill_dependent_defaults(parse_recipe(scene_table(tolerance=args.tolerance)), "bench")
+    derived = derive(recipe)
+    assert derived.cells == GRID, derived.cells
+    cells = GRID[0] * GRID[1] * GRID[2]
+    grid = make_grid(derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k, device)
+    stepper = Stepper(grid, recipe)
+    solve = stepper.pressure_solve
+    dt = derived.dt_s
+    print(f"[bench] grid {GRID}, dt {dt:.6f} s, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)
+
+    step_index = 0
+
+    def run(n):
+        nonlocal step_index
+        for _ in range(n):
+            stepper.step(step_index * dt, dt)
+            step_index += 1
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    cycles_before = solve.total_cycles() if solve.tolerance > 0 else 0
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    cycles_after = solve.total_cycles() if solve.tolerance > 0 else 0
+    mean_cycles = (cycles_after - cycles_before) / args.steps if solve.tolerance > 0 else float(solve.cycles)
+    value = cells * args.steps / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+    residual = solve.last_relative_residual() if solve.tolerance > 0 else None
+    from plume_amd.sim.step_cap import StepCap
+
+    cfl_cells = StepCap(recipe).grid_speed(grid) * dt / CELL_M
+    print(f"[bench] timed: {ms_per_step:.3f} ms/substep, mean cycles {mean_cycles:.2f}", file=sys.stderr,
+          flush=True)
+
+    # dominant kernel: the fused red-black sweep on the top level, timed per launch with HIP events over a
+    # repeat of the timed region (same substeps, same state progression)
+    sweep_ms, sweep_launches = time_kernel_family(lambda: run(min(args.steps, 12)), "rb_sweep_fused_top")
+    achieved = sweep_bytes(cells) / (sweep_ms * 1e-3) / 1e9 if sweep_launches else float("nan")
+    substep_gbps = substep_bytes(cells, mean_cycles) * args.steps / elapsed / 1e9
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "cell-substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded smoke-puff scene, emitter on for frames 0-11)",
+        "config": {"workload": "buoyant smoke puff substep, 128x192x128 FP32 MAC grid (benchmark config 2)",
+                   "grid": list(GRID), "cell_m": CELL_M, "dt_s": dt,
+                   "pressure": f"multigrid V(2,2), relative residual <= {args.tolerance:g} (FP64 norm), "
+                               f"mean {mean_cycles:.2f} cycles",
+                   "confinement": 0.3, "buoyancy": "alpha 0.1, beta 1.0", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_rb_sweep_fused (128x192x128 level, 12 B/cell)",
+                     "avg_launch_us": sweep_ms * 1e3, "launches": sweep_launches},
+        "substep_roofline": {"bytes_per_substep": substep_bytes(cells, mean_cycles), "achieved": substep_gbps,
+                             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": substep_gbps / HBM_PEAK_GBPS,
+                             "mean_cycles": mean_cycles},
+        "final_relative_residual": residual,
+        "cfl_cells_per_step": cfl_cells,
+    }
+
+    if not args.no_render:
+        settings = recipe.render
+        view = type("View", (), {})()
+        camera = make_camera(recipe.camera.elevation_deg, recipe.camera.azimuth_deg)
+        # 256 x 256 px over the domain's 1.28 x 1.92 m projection, 192 samples per ray, 64-step light march
+        width = height = 256
+        span_x, span_y = GRID[0] * CELL_M, GRID[1] * CELL_M
+        pixel_m = max(span_x, span_y) / 256
+        view.camera, view.left_m, view.top_m, view.pixel_m = camera, -span_x / 2 - (256 * pixel_m - span_x) / 2, \
+            span_y + (256 * pixel_m - span_y) / 2, pixel_m
+        view.width_px, view.height_px = width, height
+        renderer = FrameRenderer(view, settings, GRID, recipe.output.channels, device, max_steps=192, light_steps=64)
+        for _ in range(2):
+            renderer.prepare(grid.smoke.data, grid.info.cell_m)
+            renderer.march(grid.smoke.data, grid.temperature.data, grid.info, renderer.view_params, renderer.image)
+        torch.cuda.synchronize()
+        frames = 10
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            renderer.prepare(grid.smoke.data, grid.info.cell_m)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(frames):
+            renderer.march(grid.smoke.data, grid.temperature.data, grid.info, renderer.view_params, renderer.image)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        light_s, march_s = (t1 - t0) / frames, (t2 - t1) / frames
+        result["raymarch"] = {
+            "mpixel_samples_per_s": width * height * 192 / march_s / 1e6,
+            "ms_per_frame_march": march_s * 1e3, "image": [height, width], "samples_per_ray": 192,
+            "light_volume_ms": light_s * 1e3, "light_cell_samples_per_s": cells * 64 / light_s,
+            "compulsory_hbm_frac": (12.0 * cells + 20.0 * width * height) / march_s / 1e9 / HBM_PEAK_GBPS,
+        }
+
+    if args.cpu_steps > 0:
+        import dataclasses
+
+        threads = min(os.cpu_count() or 1, 16)
+        cpu_cycles = max(1, round(mean_cycles))  # the oracle runs a fixed count, as the reference does
+        cpu_recipe = dataclasses.replace(recipe, solver=dataclasses.replace(recipe.solver,
+                                                                            multigrid_cycles=cpu_cycles))
+        cpu_s = cpu_baseline(grid, cpu_recipe, derived, step_index * dt, args.cpu_steps, threads)
+        result["cpu_baseline"] = {"value": cells * args.cpu_steps / cpu_s, "unit": "cell-substeps/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"{args.cpu_steps} substeps of the same 128x192x128 scene from the "
+                                            f"GPU's warm state, FP32 C oracle with OpenMP, {cpu_cycles} V-cycles",
+                                  "seconds": cpu_s}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

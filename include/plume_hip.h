@@ -1,0 +1,278 @@
+/*
+ * libplume_hip: MI355X (gfx950) kernels for the Plume simulation substep and volume ray march, behind a
+ * plain C ABI.
+ *
+ * The reference (chadmv/plume) exposes this path as a Python operator API over Warp arrays; it has no
+ * FFI. Each entry point below replaces one reference operator or kernel launch, cited as file:line
+ * relative to the reference repository (src/plume/...). The Python package `plume_amd` binds these with
+ * ctypes, passing device pointers of contiguous float32 torch tensors and torch's current HIP stream.
+ *
+ * Conventions
+ *   - Arrays are C-order [i][j][k] with k contiguous. Cells are dims = (nx, ny, nz); faces are
+ *     u (nx+1, ny, nz), v (nx, ny+1, nz), w (nx, ny, nz+1). vec3 arrays are AoS, 3 floats per cell.
+ *   - `stream` is a hipStream_t (NULL = the null stream). Every call is asynchronous on it, except where
+ *     noted; no call synchronises the device.
+ *   - Every call returns 0 on success, PLM_EINVAL for a bad argument (nothing launched), or the
+ *     hipError_t of a failed launch. plm_last_error() gives this thread's last message.
+ *   - The library allocates no device memory behind the caller's back: every buffer, scratch included,
+ *     comes in as an argument.
+ */
+#ifndef PLUME_HIP_H
+#define PLUME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLM_EINVAL (-1)
+
+/* Per-level grid geometry, passed by value to kernels. Same meaning as GridInfo, sim/grid.py:21-33:
+   open_lo/open_hi are 0/1 per axis (lo = left/bottom/back, hi = right/top/front, sim/grid.py:78-79);
+   fill_hi is the share of the last cell on each axis inside the domain and ghost_gap the distance of the
+   open-face pressure ghost, both (1,1,1)/0.5 on the simulation grid (sim/grid.py:80-81). */
+typedef struct plm_grid_info {
+    int32_t dims[3];
+    float cell_m;
+    float origin_m[3];
+    int32_t open_lo[3];
+    int32_t open_hi[3];
+    float fill_hi[3];
+    float ghost_gap;
+} plm_grid_info;
+
+/* One emitter, EmitterParams, sim/emitters.py:23-38 (shape 0 sphere, 1 box). */
+typedef struct plm_emitter {
+    int32_t shape;
+    float center_m[3];
+    float radius_m;
+    float half_size_m[3];
+    float edge_softness;
+    float start_s;
+    float end_s;
+    float smoke_per_s;
+    float temperature_k;
+    float velocity_mps[3];
+    int32_t outward;
+    float noise_scale_m;
+    float noise_strength;
+    int32_t stream;
+} plm_emitter;
+
+/* ViewParams, render/raymarch.py:30-42. */
+typedef struct plm_view {
+    float right[3];
+    float up[3];
+    float toward[3];
+    float left_m;
+    float top_m;
+    float pixel_m;
+    int32_t perspective;
+    float eye[3];
+} plm_view;
+
+/* LookParams, render/raymarch.py:45-52. */
+typedef struct plm_look {
+    float opacity_per_density;
+    float glow_start_k;
+    float smoke_hides_heat;
+    float normal_fade_thickness;
+    int32_t shade_layout;
+    float shade_fill;
+} plm_look;
+
+/* ------------------------------------------------------------------ library */
+
+/* Human-readable message of this thread's last failed call ("" if none). */
+const char *plm_last_error(void);
+/* ABI version of this header (bumped on any signature change). */
+int plm_abi_version(void);
+
+/* ------------------------------------------------------------------ advection: sim/advect.py */
+
+/* semi_lagrangian_kernel, sim/advect.py:24-31 (advect_semi_lagrangian :53-56, _launch_trace :71-76):
+   dst = sample_field(src, trace(index + offset, dt_over_cell)). `shape` is src's shape. */
+int plm_advect_sl(const plm_grid_info *info, const float *src, float *dst, const int32_t shape[3],
+                  const float offset[3], float outside, int32_t uses_outside, const float *u, const float *v,
+                  const float *w, float dt_over_cell, void *stream);
+
+/* maccormack_correct_kernel, sim/advect.py:34-50. */
+int plm_maccormack_correct(const plm_grid_info *info, const float *original, const float *hat, const float *back,
+                           float *dst, const int32_t shape[3], const float offset[3], float outside,
+                           int32_t uses_outside, const float *u, const float *v, const float *w,
+                           float dt_over_cell, void *stream);
+
+/* advect_maccormack, sim/advect.py:59-68: data -> hat (SL +dt), hat -> back (SL -dt), correct -> next.
+   With fuse_back != 0 the backward pass is evaluated inside the correction (same arithmetic, `back` is
+   neither written nor read, one launch fewer). */
+int plm_advect_maccormack(const plm_grid_info *info, const float *data, float *hat, float *back, float *next,
+                          const int32_t shape[3], const float offset[3], float outside, int32_t uses_outside,
+                          const float *u, const float *v, const float *w, float dt_over_cell, int32_t fuse_back,
+                          void *stream);
+
+/* Stepper._advect_velocity, sim/stepper.py:46-52: MacCormack of u, v and w by the same (old) velocity,
+   in one launch per pass over all three face arrays. *_next receive the results (callers swap). */
+int plm_advect_velocity(const plm_grid_info *info, const float *u, const float *v, const float *w, float *u_hat,
+                        float *v_hat, float *w_hat, float *u_next, float *v_next, float *w_next,
+                        float dt_over_cell, void *stream);
+
+/* ------------------------------------------------------------------ forces: sim/forces.py */
+
+/* buoyancy_kernel, sim/forces.py:13-31, on v. */
+int plm_buoyancy(const plm_grid_info *info, float *v, const float *smoke, const float *temperature, float gravity,
+                 float heat_buoyancy, float smoke_weight, float ambient_k, float dt, void *stream);
+
+/* drag_kernel on u, v, w, sim/forces.py:34-48 (keep = exp(-drag*dt) formed by the caller in double). */
+int plm_drag(const plm_grid_info *info, float *u, float *v, float *w, const float wind[3], float keep,
+             void *stream);
+
+/* ------------------------------------------------------------------ vorticity: sim/vorticity.py */
+
+/* center_velocity_kernel, sim/vorticity.py:17-22 (center: vec3 per cell). */
+int plm_center_velocity(const plm_grid_info *info, const float *u, const float *v, const float *w, float *center,
+                        void *stream);
+/* curl_kernel, sim/vorticity.py:38-45. */
+int plm_curl(const plm_grid_info *info, const float *center, float *curl, void *stream);
+/* confinement_push_kernel, sim/vorticity.py:53-71. */
+int plm_confinement_push(const plm_grid_info *info, const float *curl, float *push, float strength,
+                         void *stream);
+/* add_face_push_kernel on u, v, w, sim/vorticity.py:74-84. */
+int plm_add_face_push(const plm_grid_info *info, float *u, float *v, float *w, const float *push, float dt,
+                      void *stream);
+/* VorticityConfinement.apply, sim/vorticity.py:95-106: the four passes; nothing when strength is 0.
+   With fuse != 0 the centre-velocity pass is folded into the curl pass (center is not written). */
+int plm_confinement(const plm_grid_info *info, float *u, float *v, float *w, float *center, float *curl,
+                    float *push, float strength, float dt, int32_t fuse, void *stream);
+
+/* ------------------------------------------------------------------ emitters, noise, turbulence */
+
+/* emit, sim/emitters.py:147-153: emit_scalars_kernel (:90-102) and emit_velocity_kernel (:105-121) on all
+   three face arrays. `emitters` is a device array of n_emitters structs. */
+int plm_emit(const plm_grid_info *info, float *smoke, float *temperature, float *u, float *v, float *w,
+             const plm_emitter *emitters, int32_t n_emitters, int32_t seed, float time_s, float dt, void *stream);
+
+/* value_noise, sim/noise.py:40-57, at `count` points (xyz AoS) -> out; lattice_value (:24-32) when
+   points is NULL and `lattice` holds (i,j,k) int triples. For parity tests of the integer hash. */
+int plm_noise_probe(int32_t seed, int32_t stream_id, const float *points, const int32_t *lattice, int32_t count,
+                    float *out, void *stream);
+
+/* apply_turbulence, sim/turbulence.py:59-67 (turbulence_kernel :35-53) on u, v, w;
+   pattern_time = time_s * speed / scale formed by the caller in double. */
+int plm_turbulence(const plm_grid_info *info, float *u, float *v, float *w, int32_t seed, float scale_m,
+                   float strength, float pattern_time, float dt, void *stream);
+
+/* dissipate_kernel, sim/dissipate.py:10-24 (keeps formed by the caller in double). */
+int plm_dissipate(int64_t count, float *smoke, float *temperature, float smoke_keep, float heat_keep,
+                  float ambient_k, void *stream);
+
+/* ------------------------------------------------------------------ pressure: sim/pressure.py */
+
+/* zero_closed_faces_kernel on u, v, w, sim/pressure.py:21-28 (only boundary planes are touched). */
+int plm_zero_closed_faces(const plm_grid_info *info, float *u, float *v, float *w, void *stream);
+/* divergence_kernel, sim/pressure.py:31-36. */
+int plm_divergence(const plm_grid_info *info, const float *u, const float *v, const float *w, float *divergence,
+                   void *stream);
+/* The per-cell boundary flags behind neighbor_term, sim/pressure.py:39-69: for neighbours x-,x+,y-,y+,z-,z+
+   two bits each (0 in-domain, 1 open-face ghost, 2 closed), bits 12..14 = last cell on x, y, z. */
+int plm_build_flags(const plm_grid_info *info, uint16_t *flags, void *stream);
+/* red_black_sweep_kernel, sim/pressure.py:80-92: one colour, in place on q. */
+int plm_rb_sweep(const plm_grid_info *info, float *q, const float *rhs, int32_t color, float relaxation,
+                 void *stream);
+/* red_black_sweeps, sim/pressure.py:135-141: `sweeps` red+black sweeps, in place on q. With `scratch` (a
+   level-sized float buffer, or NULL) levels of 4096+ cells run both colours of a sweep in one LDS-tiled
+   launch that ping-pongs q -> scratch -> q; the result is bit-identical to the colour-by-colour launches
+   (each colour reads only the other). */
+int plm_rb_sweeps(const plm_grid_info *info, float *q, const float *rhs, float *scratch, int32_t sweeps,
+                  float relaxation, void *stream);
+/* subtract_gradient_kernel on u, v, w, sim/pressure.py:95-116. */
+int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *w, const float *q, void *stream);
+
+/* ------------------------------------------------------------------ multigrid: sim/multigrid.py */
+
+/* residual_kernel, sim/multigrid.py:36-43. */
+int plm_residual(const plm_grid_info *info, const float *q, const float *rhs, float *residual, void *stream);
+/* restrict_kernel, sim/multigrid.py:46-60. */
+int plm_restrict(const int32_t fine_shape[3], const float *fine, const int32_t coarse_shape[3], float *coarse,
+                 void *stream);
+/* prolong_add_kernel, sim/multigrid.py:63-68. */
+int plm_prolong_add(const int32_t coarse_shape[3], const float *coarse, const int32_t fine_shape[3], float *fine,
+                    void *stream);
+/* Fills `count` floats with 0 by a kernel (coarse.q.zero_(), sim/multigrid.py:143; graph-capture safe). */
+int plm_zero(int64_t count, float *f, void *stream);
+
+/* Multigrid.__call__ / _v_cycle, sim/multigrid.py:124-147, over caller-owned level arrays: level 0 is the
+   simulation grid (q[0] = pressure, rhs[0] = divergence), level l+1 = coarse_cells(level l). Runs
+   `cycles` V(sweeps, sweeps) cycles; the coarsest level gets max(20, 4*longest) sweeps at 1.5.
+   Levels with at most PLM_LDS_CELLS cells run as one workgroup in LDS (one launch for that whole tail).
+   If tolerance > 0 and norm_scratch is given, the cycle count becomes a cap: after each cycle the
+   FP64 relative residual |rhs - Aq| / |rhs| of level 0 is reduced on the device, and once it is at or
+   below `tolerance` the remaining cycles' kernels exit at once (no host round trip; graph-safe).
+   norm_scratch: device doubles, at least plm_norm_scratch_doubles(info[0]); cycles_run (device int32[2],
+   may be NULL): [0] receives the number of cycles this solve ran, [1] is a running total across solves
+   that the library only increments; norm_scratch[0] receives the last relative residual. */
+#define PLM_LDS_CELLS 8192
+int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, float *const *rhs,
+               float *const *residual, int32_t cycles, int32_t sweeps, double tolerance, double *norm_scratch,
+               int32_t *cycles_run, void *stream);
+int64_t plm_norm_scratch_doubles(const plm_grid_info *info);
+
+/* |rhs - A q|^2 and |rhs|^2 of one level in FP64, deterministic (fixed-order tree), into out[0..1]
+   (device). For the convergence tests of sim/multigrid.py's V-cycle. */
+int plm_residual_norms(const plm_grid_info *info, const float *q, const float *rhs, double *scratch, double *out,
+                       void *stream);
+
+/* ------------------------------------------------------------------ budget: sim/reduce.py, conserve.py */
+
+/* line_sums_kernel, sim/reduce.py:12-25: lines[3][ny][nz] = per-(j,k) sums over i, sequential FP32. */
+int plm_line_sums(const int32_t shape[3], const float *a, const float *b, const float *c, float *lines,
+                  void *stream);
+/* SmokeBudget.hold, sim/conserve.py:45-52, on the device with no host round trip: line sums, their FP64
+   totals (fixed-order tree, in place of the host NumPy sum at sim/reduce.py:37), held_total (:33-35),
+   factor = target / maccormack (1 if maccormack <= 0) and, when factor != 1, next *= (float)factor.
+   lines: 3*ny*nz floats; totals (device doubles, 4): before, semi-Lagrangian, MacCormack, factor. */
+int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat, float *next, float *lines,
+                     double *totals, void *stream);
+/* scale_kernel, sim/conserve.py:27-30. */
+int plm_scale(int64_t count, float *f, float factor, void *stream);
+
+/* ------------------------------------------------------------------ CFL cap: sim/step_cap.py */
+
+/* StepCap.grid_speed, sim/step_cap.py:50-55: max |f| over u, v, w into *out (device float). */
+int plm_max_abs3(const plm_grid_info *info, const float *u, const float *v, const float *w, float *out,
+                 void *stream);
+
+/* ------------------------------------------------------------------ rendering: render/ */
+
+/* light_volume_kernel, render/light_volume.py:18-33 (toward_light: unit, cell units). max_steps > 0 fixes
+   the march at that many steps per cell (a fixed-count bench mode: step = length / max_steps); 0 keeps the
+   reference's half-cell steps. */
+int plm_light_volume(const int32_t shape[3], const float *smoke, const float toward_light[3], float cell_m,
+                     float opacity_per_density, int32_t max_steps, float *light, void *stream);
+/* blur_axis_kernel, render/normal_field.py:12-22 (weights: device, 2*radius+1 floats). */
+int plm_blur_axis(const int32_t shape[3], const float *src, float *dst, const float *weights, int32_t radius,
+                  int32_t axis, void *stream);
+/* gradient_kernel, render/normal_field.py:25-38 (gradient: vec3 per cell). */
+int plm_gradient(const int32_t shape[3], const float *src, float *gradient, void *stream);
+/* raymarch_kernel, render/raymarch.py:86-140: image (height, width, 5). gradient / light are the layout's
+   volume (the unused one may be a 1-cell stand-in, render/frame_renderer.py:29-35). max_steps > 0 fixes
+   the march at that many samples per ray (bench mode); 0 keeps half-cell steps. */
+int plm_raymarch(const plm_grid_info *info, const float *smoke, const float *temperature, const float *gradient,
+                 const int32_t gradient_shape[3], const float *light, const int32_t light_shape[3],
+                 const plm_view *view, const plm_look *look, float *image, int32_t height, int32_t width,
+                 int32_t max_steps, void *stream);
+
+/* ------------------------------------------------------------------ timing */
+
+/* Kernel timing for the benchmark: while enabled, the library brackets every launch of the named kernel
+   family ("rb_sweeps", "advect", ...) with HIP events on the launch stream and accumulates their elapsed
+   time. plm_timing_read synchronises those events and returns total milliseconds and launch count. */
+int plm_timing_enable(const char *family);
+int plm_timing_disable(void);
+int plm_timing_read(double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLUME_HIP_H */

@@ -1,0 +1,194 @@
+// Shared device helpers and C-ABI plumbing for libplume_hip (gfx950).
+//
+// Arithmetic follows the reference's Warp kernels operation by operation (the library is built with
+// -ffp-contract=off and IEEE division/sqrt), so results match the reference's float32 CPU path bit for
+// bit wherever no transcendental function is involved.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/plume_hip.h"
+
+namespace plm {
+
+// ------------------------------------------------------------------ errors and launch plumbing
+
+int fail(int code, const char *fmt, ...);
+int check_launch(const char *what);
+
+struct TimingScope {
+    TimingScope(const char *family, hipStream_t stream);
+    ~TimingScope();
+    bool active;
+    hipStream_t stream;
+    int slot;
+};
+
+#define PLM_REQUIRE(cond, ...)                                                                     \
+    do {                                                                                           \
+        if (!(cond)) return ::plm::fail(PLM_EINVAL, __VA_ARGS__);                                  \
+    } while (0)
+
+#define PLM_TRY(expr)                                                                              \
+    do {                                                                                           \
+        int plm_rc_ = (expr);                                                                      \
+        if (plm_rc_ != 0) return plm_rc_;                                                          \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct Dims {
+    int n0, n1, n2;
+    __host__ __device__ int count() const { return n0 * n1 * n2; }
+    __host__ __device__ int at(int i, int j, int k) const { return (i * n1 + j) * n2 + k; }
+};
+
+__host__ __device__ inline Dims dims_of(const int32_t *s) { return Dims{s[0], s[1], s[2]}; }
+__host__ __device__ inline Dims cell_dims(const plm_grid_info &g) { return Dims{g.dims[0], g.dims[1], g.dims[2]}; }
+__host__ __device__ inline Dims face_dims(const plm_grid_info &g, int axis) {
+    Dims d = cell_dims(g);
+    if (axis == 0) d.n0 += 1;
+    if (axis == 1) d.n1 += 1;
+    if (axis == 2) d.n2 += 1;
+    return d;
+}
+
+int validate_info(const plm_grid_info *info);
+int validate_dims(const int32_t *shape, const char *what);
+
+// 3-d launches: 64 threads along k (one wavefront on the contiguous axis), 4 rows of j, grid.z over i.
+constexpr int BX = 64, BY = 4;
+inline dim3 block3() { return dim3(BX, BY, 1); }
+inline dim3 grid3(Dims d) { return dim3((d.n2 + BX - 1) / BX, (d.n1 + BY - 1) / BY, d.n0); }
+// u, v and w stacked along grid.z: [0, nx] u, [nx+1, 2nx] v, [2nx+1, 3nx] w.
+inline dim3 grid_faces(const plm_grid_info &g) {
+    return dim3((g.dims[2] + 1 + BX - 1) / BX, (g.dims[1] + 1 + BY - 1) / BY, 3 * g.dims[0] + 1);
+}
+
+__device__ __forceinline__ bool thread_cell(Dims d, int &i, int &j, int &k) {
+    k = blockIdx.x * BX + threadIdx.x;
+    j = blockIdx.y * BY + threadIdx.y;
+    i = blockIdx.z;
+    return k < d.n2 && j < d.n1 && i < d.n0;
+}
+
+// Which face array this block works on (grid_faces), and the thread's face index in it.
+__device__ __forceinline__ bool thread_face(const plm_grid_info &g, int &axis, Dims &d, int &i, int &j, int &k) {
+    const int nx = g.dims[0];
+    int z = blockIdx.z;
+    if (z <= nx) {
+        axis = 0;
+        i = z;
+    } else if (z <= 2 * nx) {
+        axis = 1;
+        i = z - (nx + 1);
+    } else {
+        axis = 2;
+        i = z - (2 * nx + 1);
+    }
+    d = Dims{g.dims[0] + (axis == 0), g.dims[1] + (axis == 1), g.dims[2] + (axis == 2)};
+    k = blockIdx.x * BX + threadIdx.x;
+    j = blockIdx.y * BY + threadIdx.y;
+    return k < d.n2 && j < d.n1;
+}
+
+template <typename T> struct Tri {
+    T *p[3];
+};
+
+// ------------------------------------------------------------------ float helpers (Warp semantics)
+
+__device__ __forceinline__ float fmin_(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmax_(float a, float b) { return a > b ? a : b; }
+// wp.clamp(x, lo, hi) = min(max(x, lo), hi)
+__device__ __forceinline__ float clamp_(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
+// wp.lerp(a, b, t) = a * (1 - t) + b * t
+__device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float length3(float x, float y, float z) { return __fsqrt_rn(x * x + y * y + z * z); }
+
+// sample_index_space, sim/sample.py:10-29
+__device__ __forceinline__ float sample_index(const float *__restrict__ f, Dims n, float gx, float gy, float gz) {
+    float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
+    float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
+    float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
+    int i0 = (int)x, j0 = (int)y, k0 = (int)z;
+    int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
+    float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
+    float c00 = lerp_(f[n.at(i0, j0, k0)], f[n.at(i1, j0, k0)], tx);
+    float c10 = lerp_(f[n.at(i0, j1, k0)], f[n.at(i1, j1, k0)], tx);
+    float c01 = lerp_(f[n.at(i0, j0, k1)], f[n.at(i1, j0, k1)], tx);
+    float c11 = lerp_(f[n.at(i0, j1, k1)], f[n.at(i1, j1, k1)], tx);
+    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+}
+
+// range_index_space, sim/sample.py:32-50
+__device__ __forceinline__ void range_index(const float *__restrict__ f, Dims n, float gx, float gy, float gz,
+                                            float &lo, float &hi) {
+    float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
+    float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
+    float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
+    int i0 = (int)x, j0 = (int)y, k0 = (int)z;
+    int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
+    float a = f[n.at(i0, j0, k0)];
+    lo = a;
+    hi = a;
+    // the reference visits (di, dj, dk) in this order; min/max are exact, so only the set matters
+    const float v[8] = {f[n.at(i0, j0, k0)], f[n.at(i0, j0, k1)], f[n.at(i0, j1, k0)], f[n.at(i0, j1, k1)],
+                        f[n.at(i1, j0, k0)], f[n.at(i1, j0, k1)], f[n.at(i1, j1, k0)], f[n.at(i1, j1, k1)]};
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        lo = fmin_(lo, v[t]);
+        hi = fmax_(hi, v[t]);
+    }
+}
+
+// is_outside_open_face, sim/sample.py:53-61
+__device__ __forceinline__ bool outside_open_face(const plm_grid_info &g, float px, float py, float pz) {
+    if (px < 0.0f && g.open_lo[0] == 1) return true;
+    if (px > (float)g.dims[0] && g.open_hi[0] == 1) return true;
+    if (py < 0.0f && g.open_lo[1] == 1) return true;
+    if (py > (float)g.dims[1] && g.open_hi[1] == 1) return true;
+    if (pz < 0.0f && g.open_lo[2] == 1) return true;
+    if (pz > (float)g.dims[2] && g.open_hi[2] == 1) return true;
+    return false;
+}
+
+// velocity_at, sim/sample.py:74-82
+__device__ __forceinline__ void velocity_at(const plm_grid_info &g, const float *__restrict__ u,
+                                            const float *__restrict__ v, const float *__restrict__ w, float px,
+                                            float py, float pz, float &vx, float &vy, float &vz) {
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    vx = sample_index(u, Dims{nx + 1, ny, nz}, px - 0.0f, py - 0.5f, pz - 0.5f);
+    vy = sample_index(v, Dims{nx, ny + 1, nz}, px - 0.5f, py - 0.0f, pz - 0.5f);
+    vz = sample_index(w, Dims{nx, ny, nz + 1}, px - 0.5f, py - 0.5f, pz - 0.0f);
+}
+
+// trace, sim/advect.py:16-21
+__device__ __forceinline__ void trace(const plm_grid_info &g, const float *__restrict__ u,
+                                      const float *__restrict__ v, const float *__restrict__ w, float px, float py,
+                                      float pz, float dto, float &sx, float &sy, float &sz) {
+    float vx, vy, vz;
+    velocity_at(g, u, v, w, px, py, pz, vx, vy, vz);
+    const float half = 0.5f * dto;
+    float mx = px - half * vx, my = py - half * vy, mz = pz - half * vz;
+    velocity_at(g, u, v, w, mx, my, mz, vx, vy, vz);
+    sx = px - dto * vx;
+    sy = py - dto * vy;
+    sz = pz - dto * vz;
+}
+
+// sample_field, sim/sample.py:64-71
+__device__ __forceinline__ float sample_field(const float *__restrict__ f, Dims n, const plm_grid_info &g, float px,
+                                              float py, float pz, float ox, float oy, float oz, float outside,
+                                              int uses_outside) {
+    if (uses_outside == 1 && outside_open_face(g, px, py, pz)) return outside;
+    return sample_index(f, n, px - ox, py - oy, pz - oz);
+}
+
+// cell_fill, sim/pressure.py:39-44
+__device__ __forceinline__ float cell_fill(const plm_grid_info &g, int axis, int index) {
+    return index == g.dims[axis] - 1 ? g.fill_hi[axis] : 1.0f;
+}
+
+}  // namespace plm

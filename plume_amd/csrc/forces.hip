@@ -1,0 +1,496 @@
+// Body forces, vorticity confinement, emitters, noise, turbulence and dissipation:
+// sim/forces.py, sim/vorticity.py, sim/emitters.py, sim/noise.py, sim/turbulence.py, sim/dissipate.py.
+#include "common.h"
+
+namespace plm {
+namespace {
+
+// ------------------------------------------------------------------ buoyancy, drag
+
+// buoyancy_kernel, sim/forces.py:13-23
+__global__ __launch_bounds__(256) void k_buoyancy(plm_grid_info g, float *__restrict__ v,
+                                                  const float *__restrict__ smoke,
+                                                  const float *__restrict__ temperature, float gravity,
+                                                  float heat_buoyancy, float smoke_weight, float ambient_k,
+                                                  float dt) {
+    const Dims nv = face_dims(g, 1), nc = cell_dims(g);
+    int i, j, k;
+    if (!thread_cell(nv, i, j, k)) return;
+    const int below = max(j - 1, 0), above = min(j, g.dims[1] - 1);
+    const float heat = 0.5f * (temperature[nc.at(i, below, k)] + temperature[nc.at(i, above, k)]);
+    const float density = 0.5f * (smoke[nc.at(i, below, k)] + smoke[nc.at(i, above, k)]);
+    const float lift = heat_buoyancy * (heat - ambient_k) / ambient_k - smoke_weight * density;
+    const int c = nv.at(i, j, k);
+    v[c] = v[c] + gravity * lift * dt;
+}
+
+// drag_kernel, sim/forces.py:34-37, on all three face arrays
+__global__ __launch_bounds__(256) void k_drag(plm_grid_info g, Tri<float> f, float wx, float wy, float wz,
+                                              float keep) {
+    int axis, i, j, k;
+    Dims d;
+    if (!thread_face(g, axis, d, i, j, k)) return;
+    const float wind = axis == 0 ? wx : (axis == 1 ? wy : wz);
+    float *p = f.p[axis] + d.at(i, j, k);
+    *p = wind + (*p - wind) * keep;
+}
+
+// ------------------------------------------------------------------ vorticity confinement
+
+// center_velocity_kernel, sim/vorticity.py:17-22
+__device__ __forceinline__ void center_velocity(const plm_grid_info &g, const float *__restrict__ u,
+                                                const float *__restrict__ v, const float *__restrict__ w, int i,
+                                                int j, int k, float &cx, float &cy, float &cz) {
+    const Dims nu = face_dims(g, 0), nv = face_dims(g, 1), nw = face_dims(g, 2);
+    cx = 0.5f * (u[nu.at(i, j, k)] + u[nu.at(i + 1, j, k)]);
+    cy = 0.5f * (v[nv.at(i, j, k)] + v[nv.at(i, j + 1, k)]);
+    cz = 0.5f * (w[nw.at(i, j, k)] + w[nw.at(i, j, k + 1)]);
+}
+
+__global__ __launch_bounds__(256) void k_center_velocity(plm_grid_info g, const float *__restrict__ u,
+                                                         const float *__restrict__ v, const float *__restrict__ w,
+                                                         float *__restrict__ center) {
+    const Dims nc = cell_dims(g);
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    float cx, cy, cz;
+    center_velocity(g, u, v, w, i, j, k, cx, cy, cz);
+    const int c = nc.at(i, j, k);
+    center[3 * c] = cx;
+    center[3 * c + 1] = cy;
+    center[3 * c + 2] = cz;
+}
+
+// curl_kernel with central_difference, sim/vorticity.py:25-45. CENTER_FROM_FACES evaluates each centre
+// velocity from the faces (same arithmetic as center_velocity_kernel) instead of reading `center`.
+template <bool CENTER_FROM_FACES>
+__global__ __launch_bounds__(256) void k_curl(plm_grid_info g, const float *__restrict__ center,
+                                              const float *__restrict__ u, const float *__restrict__ v,
+                                              const float *__restrict__ w, float *__restrict__ curl) {
+    const Dims nc = cell_dims(g);
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    const int cell[3] = {i, j, k};
+    float d[3][3];  // d[axis][component]
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+        lo[a] = max(cell[a] - 1, 0);
+        hi[a] = min(cell[a] + 1, g.dims[a] - 1);
+        const float span = (float)(hi[a] - lo[a]);
+        if (span == 0.0f) {
+            d[a][0] = d[a][1] = d[a][2] = 0.0f;
+            continue;
+        }
+        float h[3], l[3];
+        if (CENTER_FROM_FACES) {
+            center_velocity(g, u, v, w, hi[0], hi[1], hi[2], h[0], h[1], h[2]);
+            center_velocity(g, u, v, w, lo[0], lo[1], lo[2], l[0], l[1], l[2]);
+        } else {
+            const int ch = nc.at(hi[0], hi[1], hi[2]), cl = nc.at(lo[0], lo[1], lo[2]);
+            for (int q = 0; q < 3; q++) {
+                h[q] = center[3 * ch + q];
+                l[q] = center[3 * cl + q];
+            }
+        }
+        const float denom = span * g.cell_m;
+        for (int q = 0; q < 3; q++) d[a][q] = (h[q] - l[q]) / denom;
+    }
+    const int c = nc.at(i, j, k);
+    curl[3 * c] = d[1][2] - d[2][1];
+    curl[3 * c + 1] = d[2][0] - d[0][2];
+    curl[3 * c + 2] = d[0][1] - d[1][0];
+}
+
+__device__ __forceinline__ float spin_at(const float *__restrict__ curl, int c) {
+    return length3(curl[3 * c], curl[3 * c + 1], curl[3 * c + 2]);
+}
+
+// confinement_push_kernel, sim/vorticity.py:53-71
+__global__ __launch_bounds__(256) void k_confinement_push(plm_grid_info g, const float *__restrict__ curl,
+                                                          float *__restrict__ push, float strength) {
+    const Dims nc = cell_dims(g);
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    const int cell[3] = {i, j, k};
+    float toward[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+        lo[a] = max(cell[a] - 1, 0);
+        hi[a] = min(cell[a] + 1, g.dims[a] - 1);
+        const float span = (float)(hi[a] - lo[a]);
+        if (span > 0.0f)
+            toward[a] = (spin_at(curl, nc.at(hi[0], hi[1], hi[2])) - spin_at(curl, nc.at(lo[0], lo[1], lo[2]))) /
+                        (span * g.cell_m);
+    }
+    const int c = nc.at(i, j, k);
+    const float length = length3(toward[0], toward[1], toward[2]);
+    if (length < 1.0e-12f) {
+        push[3 * c] = push[3 * c + 1] = push[3 * c + 2] = 0.0f;
+        return;
+    }
+    const float nx = toward[0] / length, ny = toward[1] / length, nz = toward[2] / length;
+    const float wx = curl[3 * c], wy = curl[3 * c + 1], wz = curl[3 * c + 2];
+    const float scale = strength * g.cell_m;
+    push[3 * c] = scale * (ny * wz - nz * wy);
+    push[3 * c + 1] = scale * (nz * wx - nx * wz);
+    push[3 * c + 2] = scale * (nx * wy - ny * wx);
+}
+
+// add_face_push_kernel, sim/vorticity.py:74-84, on all three face arrays
+__global__ __launch_bounds__(256) void k_add_face_push(plm_grid_info g, Tri<float> f,
+                                                       const float *__restrict__ push, float dt) {
+    int axis, i, j, k;
+    Dims d;
+    if (!thread_face(g, axis, d, i, j, k)) return;
+    const Dims nc = cell_dims(g);
+    int below[3] = {i, j, k}, above[3] = {i, j, k};
+    const int face[3] = {i, j, k};
+    below[axis] = max(face[axis] - 1, 0);
+    above[axis] = min(face[axis], g.dims[axis] - 1);
+    const float average =
+        0.5f * (push[3 * nc.at(below[0], below[1], below[2]) + axis] + push[3 * nc.at(above[0], above[1], above[2]) + axis]);
+    float *p = f.p[axis] + d.at(i, j, k);
+    *p = *p + average * dt;
+}
+
+// ------------------------------------------------------------------ noise, sim/noise.py
+
+// hash_u32, sim/noise.py:13-21 (lowbias32)
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+constexpr int32_t LATTICE_OFFSET = 1048576;
+
+// lattice_value, sim/noise.py:24-32
+__device__ __forceinline__ float lattice_value(int32_t seed, int32_t stream, int32_t i, int32_t j, int32_t k) {
+    uint32_t h = hash_u32((uint32_t)seed + (uint32_t)LATTICE_OFFSET);
+    h = hash_u32(h ^ (uint32_t)(stream + LATTICE_OFFSET));
+    h = hash_u32(h ^ (uint32_t)(i + LATTICE_OFFSET));
+    h = hash_u32(h ^ (uint32_t)(j + LATTICE_OFFSET));
+    h = hash_u32(h ^ (uint32_t)(k + LATTICE_OFFSET));
+    return (float)(h >> 8) * (2.0f / 16777215.0f) - 1.0f;
+}
+
+__device__ __forceinline__ float fade(float t) { return t * t * (3.0f - 2.0f * t); }
+
+// value_noise, sim/noise.py:40-57. The seed's and stream's hash prefix is shared by all 8 corners.
+__device__ __forceinline__ float value_noise(int32_t seed, int32_t stream, float px, float py, float pz) {
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    const int i = (int)fx, j = (int)fy, k = (int)fz;
+    const float tx = fade(px - fx), ty = fade(py - fy), tz = fade(pz - fz);
+    const uint32_t hs = hash_u32(hash_u32((uint32_t)seed + (uint32_t)LATTICE_OFFSET) ^ (uint32_t)(stream + LATTICE_OFFSET));
+    float corner[2][2][2];
+#pragma unroll
+    for (int di = 0; di < 2; di++) {
+        const uint32_t hi = hash_u32(hs ^ (uint32_t)(i + di + LATTICE_OFFSET));
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++) {
+            const uint32_t hj = hash_u32(hi ^ (uint32_t)(j + dj + LATTICE_OFFSET));
+#pragma unroll
+            for (int dk = 0; dk < 2; dk++) {
+                const uint32_t h = hash_u32(hj ^ (uint32_t)(k + dk + LATTICE_OFFSET));
+                corner[di][dj][dk] = (float)(h >> 8) * (2.0f / 16777215.0f) - 1.0f;
+            }
+        }
+    }
+    const float c00 = lerp_(corner[0][0][0], corner[1][0][0], tx);
+    const float c10 = lerp_(corner[0][1][0], corner[1][1][0], tx);
+    const float c01 = lerp_(corner[0][0][1], corner[1][0][1], tx);
+    const float c11 = lerp_(corner[0][1][1], corner[1][1][1], tx);
+    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+}
+
+__global__ void k_noise_probe(int32_t seed, int32_t stream, const float *__restrict__ points,
+                              const int32_t *__restrict__ lattice, int count, float *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    if (points != nullptr)
+        out[t] = value_noise(seed, stream, points[3 * t], points[3 * t + 1], points[3 * t + 2]);
+    else
+        out[t] = lattice_value(seed, stream, lattice[3 * t], lattice[3 * t + 1], lattice[3 * t + 2]);
+}
+
+// ------------------------------------------------------------------ emitters, sim/emitters.py
+
+// soft_edge, sim/emitters.py:41-50
+__device__ __forceinline__ float soft_edge(float distance, float extent, float softness) {
+    const float inner = extent * (1.0f - softness);
+    if (distance <= inner) return 1.0f;
+    if (distance >= extent) return 0.0f;
+    const float t = (distance - inner) / (extent - inner);
+    return 1.0f - t * t * (3.0f - 2.0f * t);
+}
+
+// coverage, sim/emitters.py:53-67
+__device__ __forceinline__ float coverage(const plm_emitter &e, float x, float y, float z, int32_t seed) {
+    const float ox = x - e.center_m[0], oy = y - e.center_m[1], oz = z - e.center_m[2];
+    float amount;
+    if (e.shape == 0)
+        amount = soft_edge(length3(ox, oy, oz), e.radius_m, e.edge_softness);
+    else
+        amount = soft_edge(fabsf(ox), e.half_size_m[0], e.edge_softness) *
+                 soft_edge(fabsf(oy), e.half_size_m[1], e.edge_softness) *
+                 soft_edge(fabsf(oz), e.half_size_m[2], e.edge_softness);
+    if (amount > 0.0f && e.noise_strength > 0.0f) {
+        const float breakup = 1.0f + e.noise_strength * value_noise(seed, e.stream, x / e.noise_scale_m,
+                                                                    y / e.noise_scale_m, z / e.noise_scale_m);
+        amount = amount * fmax_(breakup, 0.0f);
+    }
+    return fmin_(amount, 1.0f);
+}
+
+// blend_factor, sim/emitters.py:70-77 (BLEND_REFERENCE_S = 1/80 s, :20)
+__device__ __forceinline__ float blend_factor(float amount, float dt) {
+    return 1.0f - powf(1.0f - amount, dt / 0.0125f);
+}
+
+__device__ __forceinline__ bool is_active(const plm_emitter &e, float time_s) {
+    return time_s >= e.start_s && time_s < e.end_s;
+}
+
+// emit_scalars_kernel, sim/emitters.py:90-102
+__global__ __launch_bounds__(256) void k_emit_scalars(plm_grid_info g, float *__restrict__ smoke,
+                                                      float *__restrict__ temperature,
+                                                      const plm_emitter *__restrict__ emitters, int n_emitters,
+                                                      int32_t seed, float time_s, float dt) {
+    const Dims nc = cell_dims(g);
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    const float x = g.origin_m[0] + ((float)i + 0.5f) * g.cell_m;
+    const float y = g.origin_m[1] + ((float)j + 0.5f) * g.cell_m;
+    const float z = g.origin_m[2] + ((float)k + 0.5f) * g.cell_m;
+    const int c = nc.at(i, j, k);
+    for (int n = 0; n < n_emitters; n++) {
+        const plm_emitter e = emitters[n];
+        if (!is_active(e, time_s)) continue;
+        const float amount = coverage(e, x, y, z, seed);
+        if (amount > 0.0f) {
+            smoke[c] = smoke[c] + e.smoke_per_s * amount * dt;
+            temperature[c] = lerp_(temperature[c], e.temperature_k, blend_factor(amount, dt));
+        }
+    }
+}
+
+// emit_velocity_kernel, sim/emitters.py:105-121, on all three face arrays
+__global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, Tri<float> f,
+                                                       const plm_emitter *__restrict__ emitters, int n_emitters,
+                                                       int32_t seed, float time_s, float dt) {
+    int axis, i, j, k;
+    Dims d;
+    if (!thread_face(g, axis, d, i, j, k)) return;
+    const float x = g.origin_m[0] + ((float)i + (axis == 0 ? 0.0f : 0.5f)) * g.cell_m;
+    const float y = g.origin_m[1] + ((float)j + (axis == 1 ? 0.0f : 0.5f)) * g.cell_m;
+    const float z = g.origin_m[2] + ((float)k + (axis == 2 ? 0.0f : 0.5f)) * g.cell_m;
+    float *p = f.p[axis] + d.at(i, j, k);
+    for (int n = 0; n < n_emitters; n++) {
+        const plm_emitter e = emitters[n];
+        if (!is_active(e, time_s)) continue;
+        const float amount = coverage(e, x, y, z, seed);
+        if (amount > 0.0f) {
+            float target[3] = {e.velocity_mps[0], e.velocity_mps[1], e.velocity_mps[2]};
+            if (e.outward == 1) {
+                const float ax = x - e.center_m[0], ay = y - e.center_m[1], az = z - e.center_m[2];
+                const float la = length3(ax, ay, az);
+                if (la > 0.0f) {
+                    const float speed = length3(e.velocity_mps[0], e.velocity_mps[1], e.velocity_mps[2]);
+                    target[0] = (ax / la) * speed;
+                    target[1] = (ay / la) * speed;
+                    target[2] = (az / la) * speed;
+                }
+            }
+            *p = lerp_(*p, target[axis], blend_factor(amount, dt));
+        }
+    }
+}
+
+// ------------------------------------------------------------------ turbulence, sim/turbulence.py
+
+// potential, sim/turbulence.py:26-32 (TURBULENCE_STREAM = 1 << 24, :23)
+__device__ __forceinline__ float potential(int32_t seed, int component, float px, float py, float pz,
+                                           float pattern_time) {
+    const float slice_lo = floorf(pattern_time);
+    const float blend = fade(pattern_time - slice_lo);
+    const int32_t first = (1 << 24) + 3 * (int32_t)slice_lo + component;
+    return lerp_(value_noise(seed, first, px, py, pz), value_noise(seed, first + 3, px, py, pz), blend);
+}
+
+// turbulence_kernel, sim/turbulence.py:35-53, on all three face arrays
+__global__ __launch_bounds__(256) void k_turbulence(plm_grid_info g, Tri<float> f, int32_t seed, float scale_m,
+                                                    float strength, float pattern_time, float dt) {
+    int axis, i, j, k;
+    Dims d;
+    if (!thread_face(g, axis, d, i, j, k)) return;
+    const float face[3] = {(float)i + (axis == 0 ? 0.0f : 0.5f), (float)j + (axis == 1 ? 0.0f : 0.5f),
+                           (float)k + (axis == 2 ? 0.0f : 0.5f)};
+    const int b = (axis + 1) % 3, c = (axis + 2) % 3;
+    float hb[3] = {0.0f, 0.0f, 0.0f}, hc[3] = {0.0f, 0.0f, 0.0f};
+    hb[b] = 0.5f;
+    hc[c] = 0.5f;
+    const float to_lattice = g.cell_m / scale_m;
+    const float c_plus = potential(seed, c, (face[0] + hb[0]) * to_lattice, (face[1] + hb[1]) * to_lattice,
+                                   (face[2] + hb[2]) * to_lattice, pattern_time);
+    const float c_minus = potential(seed, c, (face[0] - hb[0]) * to_lattice, (face[1] - hb[1]) * to_lattice,
+                                    (face[2] - hb[2]) * to_lattice, pattern_time);
+    const float b_plus = potential(seed, b, (face[0] + hc[0]) * to_lattice, (face[1] + hc[1]) * to_lattice,
+                                   (face[2] + hc[2]) * to_lattice, pattern_time);
+    const float b_minus = potential(seed, b, (face[0] - hc[0]) * to_lattice, (face[1] - hc[1]) * to_lattice,
+                                    (face[2] - hc[2]) * to_lattice, pattern_time);
+    const float curl = ((c_plus - c_minus) - (b_plus - b_minus)) / to_lattice;
+    float *p = f.p[axis] + d.at(i, j, k);
+    *p = *p + strength * curl * dt;
+}
+
+// ------------------------------------------------------------------ elementwise
+
+// dissipate_kernel, sim/dissipate.py:10-15
+__global__ __launch_bounds__(256) void k_dissipate(int64_t count, float *__restrict__ smoke,
+                                                   float *__restrict__ temperature, float smoke_keep,
+                                                   float heat_keep, float ambient_k) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x) {
+        smoke[c] = smoke[c] * smoke_keep;
+        temperature[c] = ambient_k + (temperature[c] - ambient_k) * heat_keep;
+    }
+}
+
+inline dim3 grid1(int64_t count) {
+    int64_t blocks = (count + 255) / 256;
+    return dim3((unsigned)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536));
+}
+
+}  // namespace
+}  // namespace plm
+
+using namespace plm;
+
+extern "C" {
+
+int plm_buoyancy(const plm_grid_info *info, float *v, const float *smoke, const float *temperature, float gravity,
+                 float heat_buoyancy, float smoke_weight, float ambient_k, float dt, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(v && smoke && temperature, "plm_buoyancy: NULL array");
+    hipStream_t s = as_stream(stream);
+    TimingScope t("forces", s);
+    k_buoyancy<<<grid3(face_dims(*info, 1)), block3(), 0, s>>>(*info, v, smoke, temperature, gravity, heat_buoyancy,
+                                                              smoke_weight, ambient_k, dt);
+    return check_launch("plm_buoyancy");
+}
+
+int plm_drag(const plm_grid_info *info, float *u, float *v, float *w, const float wind[3], float keep,
+             void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && wind, "plm_drag: NULL argument");
+    hipStream_t s = as_stream(stream);
+    k_drag<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, wind[0], wind[1], wind[2], keep);
+    return check_launch("plm_drag");
+}
+
+int plm_center_velocity(const plm_grid_info *info, const float *u, const float *v, const float *w, float *center,
+                        void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && center, "plm_center_velocity: NULL array");
+    k_center_velocity<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, u, v, w, center);
+    return check_launch("plm_center_velocity");
+}
+
+int plm_curl(const plm_grid_info *info, const float *center, float *curl, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(center && curl, "plm_curl: NULL array");
+    k_curl<false><<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, center, nullptr, nullptr,
+                                                                              nullptr, curl);
+    return check_launch("plm_curl");
+}
+
+int plm_confinement_push(const plm_grid_info *info, const float *curl, float *push, float strength,
+                         void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(curl && push, "plm_confinement_push: NULL array");
+    k_confinement_push<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, curl, push, strength);
+    return check_launch("plm_confinement_push");
+}
+
+int plm_add_face_push(const plm_grid_info *info, float *u, float *v, float *w, const float *push, float dt,
+                      void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && push, "plm_add_face_push: NULL array");
+    k_add_face_push<<<grid_faces(*info), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    return check_launch("plm_add_face_push");
+}
+
+int plm_confinement(const plm_grid_info *info, float *u, float *v, float *w, float *center, float *curl,
+                    float *push, float strength, float dt, int32_t fuse, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && curl && push, "plm_confinement: NULL array");
+    PLM_REQUIRE(fuse || center, "plm_confinement: center is NULL");
+    if (strength == 0.0f) return 0;
+    hipStream_t s = as_stream(stream);
+    const Dims nc = cell_dims(*info);
+    TimingScope t("confinement", s);
+    if (fuse) {
+        k_curl<true><<<grid3(nc), block3(), 0, s>>>(*info, nullptr, u, v, w, curl);
+    } else {
+        k_center_velocity<<<grid3(nc), block3(), 0, s>>>(*info, u, v, w, center);
+        PLM_TRY(check_launch("plm_confinement center"));
+        k_curl<false><<<grid3(nc), block3(), 0, s>>>(*info, center, nullptr, nullptr, nullptr, curl);
+    }
+    PLM_TRY(check_launch("plm_confinement curl"));
+    k_confinement_push<<<grid3(nc), block3(), 0, s>>>(*info, curl, push, strength);
+    PLM_TRY(check_launch("plm_confinement push"));
+    k_add_face_push<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    return check_launch("plm_confinement faces");
+}
+
+int plm_emit(const plm_grid_info *info, float *smoke, float *temperature, float *u, float *v, float *w,
+             const plm_emitter *emitters, int32_t n_emitters, int32_t seed, float time_s, float dt, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(smoke && temperature && u && v && w, "plm_emit: NULL array");
+    PLM_REQUIRE(n_emitters >= 0 && (n_emitters == 0 || emitters), "plm_emit: bad emitter array");
+    if (n_emitters == 0) return 0;
+    hipStream_t s = as_stream(stream);
+    TimingScope t("emit", s);
+    k_emit_scalars<<<grid3(cell_dims(*info)), block3(), 0, s>>>(*info, smoke, temperature, emitters, n_emitters,
+                                                               seed, time_s, dt);
+    PLM_TRY(check_launch("plm_emit scalars"));
+    k_emit_velocity<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, emitters, n_emitters, seed,
+                                                          time_s, dt);
+    return check_launch("plm_emit velocity");
+}
+
+int plm_noise_probe(int32_t seed, int32_t stream_id, const float *points, const int32_t *lattice, int32_t count,
+                    float *out, void *stream) {
+    PLM_REQUIRE(count >= 0 && out && (points || lattice), "plm_noise_probe: bad arguments");
+    if (count == 0) return 0;
+    k_noise_probe<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(seed, stream_id, points, lattice, count, out);
+    return check_launch("plm_noise_probe");
+}
+
+int plm_turbulence(const plm_grid_info *info, float *u, float *v, float *w, int32_t seed, float scale_m,
+                   float strength, float pattern_time, float dt, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w, "plm_turbulence: NULL array");
+    PLM_REQUIRE(scale_m > 0.0f, "plm_turbulence: scale_m must be positive");
+    if (strength == 0.0f) return 0;
+    hipStream_t s = as_stream(stream);
+    TimingScope t("turbulence", s);
+    k_turbulence<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
+                                                       pattern_time, dt);
+    return check_launch("plm_turbulence");
+}
+
+int plm_dissipate(int64_t count, float *smoke, float *temperature, float smoke_keep, float heat_keep,
+                  float ambient_k, void *stream) {
+    PLM_REQUIRE(count >= 0 && smoke && temperature, "plm_dissipate: bad arguments");
+    if (count == 0) return 0;
+    k_dissipate<<<grid1(count), 256, 0, as_stream(stream)>>>(count, smoke, temperature, smoke_keep, heat_keep,
+                                                            ambient_k);
+    return check_launch("plm_dissipate");
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+// Deterministic reductions: the smoke budget's line sums and totals (sim/reduce.py, sim/conserve.py) and the
+// CFL cap's fastest face velocity (sim/step_cap.py). No float atomics: sums run in a fixed order, and the
+// maximum (order-free) uses an integer atomic on the non-negative float's bits.
+#include "common.h"
+
+namespace plm {
+namespace {
+
+// line_sums_kernel, sim/reduce.py:12-25: one thread per (j, k) line, sequential over i in FP32
+__global__ __launch_bounds__(256) void k_line_sums(Dims n, const float *__restrict__ a, const float *__restrict__ b,
+                                                   const float *__restrict__ c, float *__restrict__ lines) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int plane = n.n1 * n.n2;
+    if (t >= plane) return;
+    float ta = 0.0f, tb = 0.0f, tc = 0.0f;
+    for (int i = 0; i < n.n0; i++) {
+        const int x = i * plane + t;
+        ta = ta + a[x];
+        tb = tb + b[x];
+        tc = tc + c[x];
+    }
+    lines[t] = ta;
+    lines[plane + t] = tb;
+    lines[2 * plane + t] = tc;
+}
+
+constexpr int BUDGET_THREADS = 1024;
+
+// TripleSum's host float64 totals (sim/reduce.py:37) by a fixed-order tree, then SmokeBudget.hold's factor
+// (sim/conserve.py:48-50): totals = before, semi-Lagrangian, MacCormack, factor.
+__global__ __launch_bounds__(BUDGET_THREADS) void k_budget_totals(int plane, const float *__restrict__ lines,
+                                                                  double *__restrict__ totals) {
+    __shared__ double s[3][BUDGET_THREADS];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < plane; t += BUDGET_THREADS)
+        for (int q = 0; q < 3; q++) acc[q] += (double)lines[q * plane + t];
+    for (int q = 0; q < 3; q++) s[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int half = BUDGET_THREADS / 2; half > 0; half /= 2) {
+        if ((int)threadIdx.x < half)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] += s[q][threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double before = s[0][0], semi_lagrangian = s[1][0], maccormack = s[2][0];
+    // held_total, sim/conserve.py:33-35
+    const double target = fmax(semi_lagrangian, fmin(maccormack, before));
+    totals[0] = before;
+    totals[1] = semi_lagrangian;
+    totals[2] = maccormack;
+    totals[3] = maccormack > 0.0 ? target / maccormack : 1.0;
+}
+
+// scale_kernel, sim/conserve.py:27-30, applied only when the factor is not 1 (sim/conserve.py:51-52)
+__global__ __launch_bounds__(256) void k_scale_by_total(int64_t count, float *__restrict__ f,
+                                                        const double *__restrict__ totals) {
+    const double factor = totals[3];
+    if (factor == 1.0) return;
+    const float scale = (float)factor;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
+        f[c] = f[c] * scale;
+}
+
+__global__ __launch_bounds__(256) void k_scale(int64_t count, float *__restrict__ f, float factor) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
+        f[c] = f[c] * factor;
+}
+
+__global__ void k_clear_u32(unsigned *p) { *p = 0u; }
+
+// max |f| over the three face arrays; |f| >= 0, so its float bits order like unsigned integers
+__global__ __launch_bounds__(256) void k_max_abs3(Tri<const float> f, int64_t nu, int64_t nv, int64_t nw,
+                                                  unsigned *__restrict__ out) {
+    float best = 0.0f;
+    const int64_t total = nu + nv + nw;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total; c += (int64_t)gridDim.x * blockDim.x) {
+        float x;
+        if (c < nu)
+            x = f.p[0][c];
+        else if (c < nu + nv)
+            x = f.p[1][c - nu];
+        else
+            x = f.p[2][c - nu - nv];
+        best = fmaxf(best, fabsf(x));
+    }
+    __shared__ float s[256];
+    s[threadIdx.x] = best;
+    __syncthreads();
+    for (int half = 128; half > 0; half /= 2) {
+        if ((int)threadIdx.x < half) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + half]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));
+}
+
+inline dim3 grid1(int64_t count, int64_t cap = 16384) {
+    int64_t blocks = (count + 255) / 256;
+    return dim3((unsigned)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap));
+}
+
+}  // namespace
+}  // namespace plm
+
+using namespace plm;
+
+extern "C" {
+
+int plm_line_sums(const int32_t shape[3], const float *a, const float *b, const float *c, float *lines,
+                  void *stream) {
+    PLM_TRY(validate_dims(shape, "plm_line_sums"));
+    PLM_REQUIRE(a && b && c && lines, "plm_line_sums: NULL array");
+    const Dims n = dims_of(shape);
+    k_line_sums<<<(n.n1 * n.n2 + 255) / 256, 256, 0, as_stream(stream)>>>(n, a, b, c, lines);
+    return check_launch("plm_line_sums");
+}
+
+int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat, float *next, float *lines,
+                     double *totals, void *stream) {
+    PLM_TRY(validate_dims(shape, "plm_smoke_budget"));
+    PLM_REQUIRE(data && hat && next && lines && totals, "plm_smoke_budget: NULL array");
+    const Dims n = dims_of(shape);
+    hipStream_t s = as_stream(stream);
+    TimingScope t("budget", s);
+    k_line_sums<<<(n.n1 * n.n2 + 255) / 256, 256, 0, s>>>(n, data, hat, next, lines);
+    PLM_TRY(check_launch("plm_smoke_budget line sums"));
+    k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, totals);
+    PLM_TRY(check_launch("plm_smoke_budget totals"));
+    k_scale_by_total<<<grid1(n.count()), 256, 0, s>>>(n.count(), next, totals);
+    return check_launch("plm_smoke_budget scale");
+}
+
+int plm_scale(int64_t count, float *f, float factor, void *stream) {
+    PLM_REQUIRE(count >= 0 && (count == 0 || f), "plm_scale: bad arguments");
+    if (count == 0) return 0;
+    k_scale<<<grid1(count), 256, 0, as_stream(stream)>>>(count, f, factor);
+    return check_launch("plm_scale");
+}
+
+int plm_max_abs3(const plm_grid_info *info, const float *u, const float *v, const float *w, float *out,
+                 void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && out, "plm_max_abs3: NULL array");
+    hipStream_t s = as_stream(stream);
+    const int64_t nu = face_dims(*info, 0).count(), nv = face_dims(*info, 1).count(), nw = face_dims(*info, 2).count();
+    k_clear_u32<<<1, 1, 0, s>>>(reinterpret_cast<unsigned *>(out));
+    PLM_TRY(check_launch("plm_max_abs3 clear"));
+    k_max_abs3<<<grid1(nu + nv + nw, 2048), 256, 0, s>>>(Tri<const float>{{u, v, w}}, nu, nv, nw,
+                                                         reinterpret_cast<unsigned *>(out));
+    return check_launch("plm_max_abs3");
+}
+
+}  // extern "C"

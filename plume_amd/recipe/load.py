@@ -1,0 +1,234 @@
+"""Building and loading recipes: strict table -> dataclass build (recipe/build.py:16-113 of the reference),
+the value rules the simulation relies on (recipe/rules.py:21-112), --preview (recipe/load.py:46-69) and the
+dependent defaults (recipe/load.py:72-87)."""
+
+import dataclasses
+import difflib
+import math
+import typing
+from pathlib import Path
+from typing import Any
+
+import tomli
+
+from plume_amd.recipe.errors import RecipeError
+from plume_amd.recipe.schema import SEED_MAX, Recipe
+
+IGNORED_TOP_LEVEL = ("resolved",)
+
+
+# ---------------------------------------------------------------------- build (strict, typed)
+
+def build(cls: type, table: dict[str, Any], where: str = "") -> Any:
+    """Builds `cls` from `table`, filling defaults; raises RecipeError naming the offending key."""
+    fields = {f.name: f for f in dataclasses.fields(cls)}
+    hints = typing.get_type_hints(cls)
+    ignored = IGNORED_TOP_LEVEL if where == "" else ()
+    for key in table:
+        if key not in fields and key not in ignored:
+            close = difflib.get_close_matches(key, list(fields), n=1, cutoff=0.6)
+            hint = f"; did you mean {close[0]}?" if close else f"; valid keys are {', '.join(fields)}"
+            raise RecipeError(f"unknown key {_join(where, key)}{hint}")
+    values = {}
+    for name, field in fields.items():
+        key_path = _join(where, name)
+        if field.metadata.get("section"):
+            values[name] = build(hints[name], _table(table.get(name, {}), key_path), key_path)
+        elif name in table:
+            values[name] = _convert(table[name], hints[name], key_path, field)
+        elif field.default is dataclasses.MISSING:
+            raise RecipeError(f"{key_path} is required")
+        else:
+            values[name] = field.default
+    return cls(**values)
+
+
+def _convert(value: Any, annotation: Any, key_path: str, field: dataclasses.Field) -> Any:
+    if annotation is float:
+        if isinstance(value, bool) or not isinstance(value, (int, float)):
+            raise RecipeError(f"{key_path} must be a number, got {type(value).__name__} {value!r}")
+        try:
+            number = float(value)
+        except OverflowError:
+            number = math.inf
+        if not math.isfinite(number):
+            raise RecipeError(f"{key_path} must be a finite number, got {number}")
+        return number
+    if annotation is int:
+        if isinstance(value, bool) or not isinstance(value, int):
+            raise RecipeError(f"{key_path} must be a whole number, got {type(value).__name__} {value!r}")
+        return value
+    if annotation is str:
+        if not isinstance(value, str):
+            raise RecipeError(f"{key_path} must be text, got {type(value).__name__} {value!r}")
+        choices = field.metadata.get("choices", ())
+        if choices and value not in choices:
+            raise RecipeError(f"{key_path} must be one of {', '.join(choices)}; got {value!r}")
+        return value
+    if typing.get_origin(annotation) is tuple:
+        args = typing.get_args(annotation)
+        if not isinstance(value, list):
+            raise RecipeError(f"{key_path} must be a list, got {type(value).__name__} {value!r}")
+        if len(args) == 2 and args[1] is Ellipsis:
+            item = args[0]
+            if dataclasses.is_dataclass(item):
+                return tuple(build(item, _table(v, f"{key_path}[{i}]"), f"{key_path}[{i}]") for i, v in enumerate(value))
+            return tuple(_convert(v, item, f"{key_path}[{i}]", field) for i, v in enumerate(value))
+        if len(value) != len(args):
+            raise RecipeError(f"{key_path} must have {len(args)} values, got {len(value)}")
+        return tuple(_convert(v, t, f"{key_path}[{i}]", field) for i, (v, t) in enumerate(zip(value, args)))
+    raise TypeError(f"schema type {annotation!r} of {key_path} has no converter")
+
+
+def _table(value: Any, key_path: str) -> dict:
+    if not isinstance(value, dict):
+        raise RecipeError(f"{key_path} must be a table, got {type(value).__name__} {value!r}")
+    return value
+
+
+def _join(where: str, name: str) -> str:
+    return f"{where}.{name}" if where else name
+
+
+# ---------------------------------------------------------------------- value rules
+
+def _positive(key, value):
+    if not value > 0:
+        raise RecipeError(f"{key} must be greater than 0, got {value}")
+
+
+def _at_least(key, value, minimum):
+    if value < minimum:
+        raise RecipeError(f"{key} must be at least {minimum}, got {value}")
+
+
+def _between(key, value, low, high):
+    if not low <= value <= high:
+        raise RecipeError(f"{key} must be between {low} and {high}, got {value}")
+
+
+def check_rules(recipe: Recipe) -> None:
+    """The rules of recipe/rules.py:21-135 that the simulation and the march depend on."""
+    _between("seed", recipe.seed, 0, SEED_MAX)
+    domain = recipe.domain
+    _positive("domain.cell_m", domain.cell_m)
+    for axis, size in zip("xyz", domain.size_m):
+        _positive(f"domain.size_m ({axis})", size)
+    _positive("domain.ambient_k", domain.ambient_k)
+    time, solver = recipe.time, recipe.solver
+    _positive("time.duration_s", time.duration_s)
+    if not math.isfinite(time.frames * time.substeps / time.duration_s):
+        raise RecipeError(f"time.duration_s is too small for {time.frames} frames, got {time.duration_s}")
+    _at_least("time.frames", time.frames, 1)
+    _at_least("time.substeps", time.substeps, 1)
+    _at_least("time.warmup_s", time.warmup_s, 0.0)
+    _at_least("solver.pressure_iterations", solver.pressure_iterations, 1)
+    _at_least("solver.multigrid_cycles", solver.multigrid_cycles, 1)
+    _at_least("solver.multigrid_sweeps", solver.multigrid_sweeps, 2)
+    _at_least("solver.multigrid_tolerance", solver.multigrid_tolerance, 0.0)
+    _at_least("solver.multigrid_max_cycles", solver.multigrid_max_cycles, 1)
+    _positive("solver.max_cells_per_step", solver.max_cells_per_step)
+    _at_least("solver.max_substeps", solver.max_substeps, 1)
+    if not recipe.emitter:
+        raise RecipeError("emitter needs at least one [[emitter]]")
+    names = set()
+    for index, emitter in enumerate(recipe.emitter):
+        where = f"emitter[{index}]"
+        if emitter.name in names:
+            raise RecipeError(f"{where}.name {emitter.name!r} is used twice")
+        if not emitter.name or any(mark in emitter.name for mark in ".[]"):
+            raise RecipeError(f"{where}.name must be a name without '.', '[' or ']'; got {emitter.name!r}")
+        names.add(emitter.name)
+        if emitter.shape == "sphere":
+            _positive(f"{where}.radius_m", emitter.radius_m)
+        else:
+            for axis, size in zip("xyz", emitter.size_m):
+                _positive(f"{where}.size_m ({axis})", size)
+        _between(f"{where}.edge_softness", emitter.edge_softness, 0.0, 1.0)
+        if emitter.end_s <= emitter.start_s:
+            raise RecipeError(f"{where}.end_s must be after start_s")
+        _at_least(f"{where}.smoke_per_s", emitter.smoke_per_s, 0.0)
+        if emitter.temperature_k != -1.0:
+            _positive(f"{where}.temperature_k", emitter.temperature_k)
+        _positive(f"{where}.noise.scale_m", emitter.noise.scale_m)
+        _at_least(f"{where}.noise.strength", emitter.noise.strength, 0.0)
+    forces = recipe.forces
+    _at_least("forces.drag_per_s", forces.drag_per_s, 0.0)
+    _at_least("forces.vorticity_confinement", forces.vorticity_confinement, 0.0)
+    _positive("forces.turbulence.scale_m", forces.turbulence.scale_m)
+    _at_least("forces.turbulence.strength_mps2", forces.turbulence.strength_mps2, 0.0)
+    _at_least("forces.turbulence.speed_mps", forces.turbulence.speed_mps, 0.0)
+    if any(forces.wind_mps) and forces.drag_per_s == 0.0:
+        raise RecipeError("forces.wind_mps needs forces.drag_per_s above 0: drag is what pulls the gas toward the "
+                          "wind")
+    _between("camera.elevation_deg", recipe.camera.elevation_deg, -90.0, 90.0)
+    render = recipe.render
+    _at_least("render.px_per_cell", render.px_per_cell, 1.0)
+    _positive("render.opacity_per_density", render.opacity_per_density)
+    _between("render.smoke_hides_heat", render.smoke_hides_heat, 0.0, 1.0)
+    _at_least("render.normal_blur_cells", render.normal_blur_cells, 0.0)
+    _positive("render.normal_fade_thickness", render.normal_fade_thickness)
+    if not any(render.light_dir):
+        raise RecipeError("render.light_dir must point somewhere; got (0, 0, 0)")
+    _between("render.shade_fill", render.shade_fill, 0.0, 1.0)
+    _at_least("preview.longest_axis_cells", recipe.preview.longest_axis_cells, 4)
+    _at_least("preview.frames", recipe.preview.frames, 1)
+
+
+# ---------------------------------------------------------------------- load
+
+def parse_recipe(table: dict) -> Recipe:
+    """Builds and checks a recipe from a parsed TOML table, with schema defaults only."""
+    recipe = build(Recipe, table)
+    check_rules(recipe)
+    return recipe
+
+
+def preview_substeps(frames: int, substeps: int, preview_frames: int) -> int:
+    """Substeps per preview frame that keep the full run's step length (recipe/load.py:66-72)."""
+    return max(1, math.ceil(frames * substeps / preview_frames))
+
+
+def apply_preview(recipe: Recipe) -> Recipe:
+    """Coarsens the grid to preview.longest_axis_cells and uses the preview frames (recipe/load.py:46-63)."""
+    preview = recipe.preview
+    cell_m = max(recipe.domain.size_m) / preview.longest_axis_cells
+    substeps = preview_substeps(recipe.time.frames, recipe.time.substeps, preview.frames)
+    return dataclasses.replace(
+        recipe,
+        domain=dataclasses.replace(recipe.domain, cell_m=cell_m),
+        time=dataclasses.replace(recipe.time, frames=preview.frames, substeps=substeps),
+        output=dataclasses.replace(recipe.output, frame_px=preview.frame_px, columns=0),
+    )
+
+
+def fill_dependent_defaults(recipe: Recipe, recipe_stem: str) -> Recipe:
+    """Replaces the 'empty means derived' defaults with their values (recipe/load.py:72-87)."""
+    output = recipe.output
+    columns = output.columns or math.ceil(math.sqrt(recipe.time.frames))
+    ambient = recipe.domain.ambient_k
+    emitters = tuple(dataclasses.replace(e, temperature_k=ambient) if e.temperature_k == -1.0 else e
+                     for e in recipe.emitter)
+    return dataclasses.replace(
+        recipe, emitter=emitters,
+        camera=dataclasses.replace(recipe.camera, seat=recipe.camera.seat or emitters[0].name),
+        output=dataclasses.replace(output, name=output.name or recipe_stem, columns=columns))
+
+
+def resolve_table(table: dict, recipe_stem: str, preview: bool = False) -> Recipe:
+    recipe = parse_recipe(table)
+    if preview:
+        recipe = apply_preview(recipe)
+    return fill_dependent_defaults(recipe, recipe_stem)
+
+
+def load_recipe(path: Path, preview: bool = False) -> Recipe:
+    """Reads the TOML recipe at `path` and returns it fully resolved."""
+    path = Path(path)
+    try:
+        table = tomli.loads(path.read_text())
+    except tomli.TOMLDecodeError as error:
+        raise RecipeError(f"{path.name} is not valid TOML: {error}") from error
+    if table.get("extends"):
+        raise RecipeError("extends is not supported by this loader; resolve the parent recipe first")
+    return resolve_table(table, path.stem, preview)

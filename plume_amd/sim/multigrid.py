@@ -1,0 +1,127 @@
+"""Geometric multigrid for the pressure solve (sim/multigrid.py of the reference).
+
+Same hierarchy as the reference: each level halves the cell count per axis (rounding up) and doubles the
+cell size, down to 2 cells on the shortest axis or 8 levels; a coarse level whose axis overhangs the domain
+weighs its last cell by the share inside (`coarse_geometry`). One V(sweeps, sweeps) cycle: pre-smooth,
+residual, restrict, coarse q = 0, recurse, prolong-add, post-smooth; the coarsest level gets
+max(20, 4 * longest) sweeps at 1.5. Warm start from `grid.pressure`.
+
+The whole solve is one call into libplume_hip (plm_vcycle): fused two-colour sweeps, a fused
+residual + restrict, and every level of at most a few thousand cells solved by one workgroup in LDS. With
+`tolerance` > 0 the solve stops once |rhs - A q| / |rhs| (FP64, reduced on the device) is at most
+`tolerance`, capped at `max_cycles` cycles, with no host round trip.
+"""
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from plume_amd._lib import LIB, GridInfo, call, ptr, stream
+from plume_amd.sim.grid import grid_info
+
+MIN_LEVEL_CELLS = 2
+MAX_LEVELS = 8
+COARSEST_SWEEPS = 20
+COARSEST_SWEEPS_PER_CELL = 4
+COARSEST_RELAXATION = 1.5
+SMOOTHING_RELAXATION = 1.0
+
+
+@dataclass
+class Level:
+    info: GridInfo
+    cells: tuple[int, int, int]
+    q: torch.Tensor
+    rhs: torch.Tensor
+    residual: torch.Tensor
+
+
+def coarse_cells(cells) -> tuple[int, int, int]:
+    return tuple((n + 1) // 2 for n in cells)
+
+
+def coarse_geometry(top_cells, cells, level: int) -> tuple[tuple[float, float, float], float]:
+    """fill_hi and ghost_gap of a level `level` steps down (sim/multigrid.py:84-94)."""
+    scale = 2.0 ** level
+    fill = tuple(top_n / scale - (n - 1) for top_n, n in zip(top_cells, cells))
+    return fill, 0.5 / scale
+
+
+def build_levels(grid, max_levels: int = MAX_LEVELS) -> list[Level]:
+    """The level list of Multigrid._build_levels (sim/multigrid.py:106-122); level 0 aliases the grid's
+    pressure and divergence."""
+    device = grid.device
+    top = Level(grid.info, tuple(grid.cells), grid.pressure, grid.divergence,
+                torch.zeros(grid.cells, dtype=torch.float32, device=device))
+    levels = [top]
+    while min(levels[-1].cells) > MIN_LEVEL_CELLS and len(levels) < max_levels:
+        above = levels[-1]
+        cells = coarse_cells(above.cells)
+        fill, ghost_gap = coarse_geometry(grid.cells, cells, len(levels))
+        info = grid_info(cells, above.info.cell_m * 2.0, tuple(above.info.origin_m), tuple(above.info.open_lo),
+                         tuple(above.info.open_hi), fill, ghost_gap)
+        levels.append(Level(info, cells, *(torch.zeros(cells, dtype=torch.float32, device=device) for _ in range(3))))
+    return levels
+
+
+class Multigrid:
+    """`cycles` V-cycles per solve with `sweeps` smoothing sweeps a side (sim/multigrid.py:97-104).
+    tolerance > 0: stop at that FP64 relative residual, at most `max_cycles` cycles."""
+
+    def __init__(self, grid, cycles: int, sweeps: int, tolerance: float = 0.0, max_cycles: int | None = None,
+                 max_levels: int = MAX_LEVELS):
+        self.cycles = cycles
+        self.sweeps = sweeps
+        self.tolerance = float(tolerance)
+        self.max_cycles = int(max_cycles if max_cycles is not None else cycles)
+        self.device = grid.device
+        self.levels = build_levels(grid, max_levels)
+        n = len(self.levels)
+        self._infos = (GridInfo * n)(*(level.info for level in self.levels))
+        doubles = LIB.plm_norm_scratch_doubles(ctypes.byref(self.levels[0].info))
+        self.norm_scratch = torch.zeros(doubles, dtype=torch.float64, device=self.device)
+        # [0] cycles of the last solve, [1] running total (tolerance mode)
+        self.cycles_run = torch.zeros(2, dtype=torch.int32, device=self.device)
+
+    def _pointers(self):
+        array = ctypes.c_void_p * len(self.levels)
+        return (array(*(ptr(level.q) for level in self.levels)), array(*(ptr(level.rhs) for level in self.levels)),
+                array(*(ptr(level.residual) for level in self.levels)))
+
+    def __call__(self, grid) -> None:
+        self.levels[0].q = grid.pressure
+        self.levels[0].rhs = grid.divergence
+        q, rhs, residual = self._pointers()
+        cycles = self.max_cycles if self.tolerance > 0.0 else self.cycles
+        call("plm_vcycle", len(self.levels), self._infos, q, rhs, residual, cycles, self.sweeps, self.tolerance,
+             ptr(self.norm_scratch, torch.float64) if self.tolerance > 0.0 else None,
+             ptr(self.cycles_run, torch.int32) if self.tolerance > 0.0 else None, stream())
+
+    def last_relative_residual(self) -> float:
+        """The FP64 |rhs - A q| / |rhs| after the last solve in tolerance mode (synchronises)."""
+        return float(self.norm_scratch[0].item())
+
+    def last_cycles(self) -> int:
+        """V-cycles the last tolerance-mode solve ran (synchronises)."""
+        return int(self.cycles_run[0].item())
+
+    def total_cycles(self) -> int:
+        """V-cycles run by every tolerance-mode solve so far (synchronises)."""
+        return int(self.cycles_run[1].item())
+
+
+def residual(level: Level) -> torch.Tensor:
+    """residual_kernel (sim/multigrid.py:36-43) into `level.residual`."""
+    call("plm_residual", ctypes.byref(level.info), ptr(level.q), ptr(level.rhs), ptr(level.residual), stream())
+    return level.residual
+
+
+def relative_residual(info, q: torch.Tensor, rhs: torch.Tensor) -> float:
+    """|rhs - A q| / |rhs| in FP64 on the device (synchronises)."""
+    scratch = torch.zeros(LIB.plm_norm_scratch_doubles(ctypes.byref(info)), dtype=torch.float64, device=q.device)
+    out = torch.zeros(2, dtype=torch.float64, device=q.device)
+    call("plm_residual_norms", ctypes.byref(info), ptr(q), ptr(rhs), ptr(scratch, torch.float64),
+         ptr(out, torch.float64), stream())
+    r2, b2 = out.tolist()
+    return (r2 ** 0.5) / (b2 ** 0.5) if b2 > 0 else float("inf") if r2 > 0 else 0.0

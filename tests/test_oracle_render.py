@@ -1,0 +1,200 @@
+"""Pins the CPU oracle's light volume and ray march with the reference's render tests
+(tests/render/test_raymarch.py, test_shade.py, test_view_cameras.py of chadmv/plume) and closed forms."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle.render import render_oracle, view_params
+from plume_amd.recipe.schema import NORMAL_LAYOUT, SHADE_LAYOUT, RenderSettings
+from plume_amd.render.camera import make_camera, make_domain_view
+from plume_amd.render.raw_frame import HEAT, NORMAL_X, NORMAL_Y, NORMAL_Z, SHADE, THICKNESS
+from support import ALL_CLOSED, oracle_grid
+
+CELL_M = 0.1
+CELLS = (16, 16, 16)
+LEFT = (-1.0, 0.0, 0.0)
+
+
+def grid(o):
+    return oracle_grid(o, CELLS, CELL_M, ALL_CLOSED)
+
+
+def blob(shape, center, radius):
+    idx = np.indices(shape).astype(np.float64) + 0.5
+    return np.exp(-sum((idx[a] - center[a]) ** 2 for a in range(3)) / (2 * radius**2))
+
+
+def render(o, g, elevation_deg=0.0, channels=NORMAL_LAYOUT, **settings):
+    view = make_domain_view(make_camera(elevation_deg, 0.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    return view, render_oracle(o, g, view, RenderSettings(**settings), channels)
+
+
+def test_domain_view_covers_the_box_at_two_pixels_per_cell():
+    view = make_domain_view(make_camera(0.0, 0.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    assert (view.width_px, view.height_px) == (32, 32)
+    assert view.to_pixels(np.array([-0.8, 1.6, 0.0])) == pytest.approx((0.0, 0.0))
+
+
+def test_uniform_slab_thickness_follows_beer_lambert(oracle32):
+    g = grid(oracle32)
+    g.smoke.data[...] = 0.5
+    _, image = render(oracle32, g, opacity_per_density=2.0)
+    assert image[16, 16, THICKNESS] == pytest.approx(1.0 - math.exp(-2.0 * 0.5 * 1.6), rel=0.01)
+
+
+def test_uniform_slab_thickness_is_exact_in_fp64(oracle64):
+    # closed form: a uniform density d over a path of length L gives 1 - exp(-opd d L) exactly
+    g = grid(oracle64)
+    g.smoke.data[...] = 0.5
+    _, image = render(oracle64, g, opacity_per_density=2.0)
+    assert image[16, 16, THICKNESS] == pytest.approx(1.0 - math.exp(-2.0 * 0.5 * 1.6), rel=1e-9)
+
+
+def test_sphere_normals_point_outward_and_face_the_camera_at_the_center(oracle32):
+    g = grid(oracle32)
+    g.smoke.data[...] = 3.0 * (blob(CELLS, (8.0, 8.0, 8.0), 3.0) > 0.3)
+    _, image = render(oracle32, g)
+    assert image[16, 16, NORMAL_Z] > 0.95
+    assert image[16, 7, NORMAL_X] < -0.5
+    assert image[7, 16, NORMAL_Y] > 0.5
+
+
+def test_empty_space_faces_the_camera(oracle32):
+    _, image = render(oracle32, grid(oracle32))
+    assert np.allclose(image[..., NORMAL_Z], 1.0)
+    assert np.all(image[..., THICKNESS] == 0.0)
+
+
+def hot_cell_behind_smoke(o):
+    g = grid(o)
+    temperature = np.full(CELLS, 293.0)
+    temperature[7:9, 7:9, 2:4] = 1200.0
+    smoke = np.zeros(CELLS)
+    smoke[:, :, 10:14] = 20.0
+    g.temperature.data[...] = temperature
+    g.smoke.data[...] = smoke
+    return g
+
+
+def test_smoke_in_front_hides_heat(oracle32):
+    _, hidden = render(oracle32, hot_cell_behind_smoke(oracle32), smoke_hides_heat=1.0)
+    _, shown = render(oracle32, hot_cell_behind_smoke(oracle32), smoke_hides_heat=0.0)
+    assert shown[16, 16, HEAT] > 10.0
+    assert hidden[16, 16, HEAT] < 0.01 * shown[16, 16, HEAT]
+
+
+def test_looking_straight_down_shows_the_back_at_the_top(oracle32):
+    g = grid(oracle32)
+    smoke = np.zeros(CELLS)
+    smoke[6:10, 6:10, 1:4] = 2.0
+    g.smoke.data[...] = smoke
+    view, image = render(oracle32, g, elevation_deg=90.0)
+    rows = np.nonzero(image[..., THICKNESS].max(axis=1) > 0.1)[0]
+    assert (view.width_px, view.height_px) == (32, 32)
+    assert rows.max() < 16
+
+
+# ---------------------------------------------------------------------- shade (tests/render/test_shade.py)
+
+def smoke_sphere(o):
+    g = grid(o)
+    g.smoke.data[...] = 3.0 * (blob(CELLS, (8.0, 8.0, 8.0), 3.0) > 0.3)
+    return g
+
+
+def half_means(image):
+    covered = image[..., THICKNESS] > 0.5
+    shade = np.where(covered, image[..., SHADE] / np.maximum(image[..., THICKNESS], 0.5), np.nan)
+    return float(np.nanmean(shade[:, :16])), float(np.nanmean(shade[:, 16:]))
+
+
+def test_light_falls_through_a_uniform_slab_as_beer_lambert(oracle32):
+    smoke = np.full(CELLS, 0.5, dtype=np.float32)
+    light = np.zeros(CELLS, dtype=np.float32)
+    oracle32.light_volume(smoke, (0.0, 0.0, 1.0), CELL_M, 2.0, light)
+    for k in (15, 8, 0):
+        depth_m = (CELLS[2] - k - 0.5) * CELL_M
+        assert light[8, 8, k] == pytest.approx(math.exp(-2.0 * 0.5 * depth_m), rel=0.01), k
+
+
+def test_a_sphere_lit_from_the_left_is_darker_on_its_right(oracle32):
+    _, image = render(oracle32, smoke_sphere(oracle32), channels=SHADE_LAYOUT, light_dir=LEFT, shade_fill=0.25)
+    left, right = half_means(image)
+    assert left > right + 0.2
+
+
+def test_a_sphere_lit_from_above_is_darker_below(oracle32):
+    _, image = render(oracle32, smoke_sphere(oracle32), elevation_deg=30.0, channels=SHADE_LAYOUT,
+                      light_dir=(0.0, 1.0, 0.0))
+    covered = image[..., THICKNESS] > 0.5
+    shade = np.where(covered, image[..., SHADE] / np.maximum(image[..., THICKNESS], 0.5), np.nan)
+    assert float(np.nanmean(shade[:16])) > float(np.nanmean(shade[16:])) + 0.2
+
+
+def test_a_sphere_lit_from_the_camera_side_shows_more_light(oracle32):
+    def mean_shade(light_dir):
+        _, image = render(oracle32, smoke_sphere(oracle32), elevation_deg=30.0, channels=SHADE_LAYOUT,
+                          light_dir=light_dir)
+        covered = image[..., THICKNESS] > 0.5
+        return float(np.mean(image[covered, SHADE] / image[covered, THICKNESS]))
+
+    assert mean_shade((0.0, 0.0, 1.0)) > mean_shade((0.0, 0.0, -1.0)) + 0.2
+
+
+def test_full_fill_casts_no_shadow(oracle32):
+    _, image = render(oracle32, smoke_sphere(oracle32), channels=SHADE_LAYOUT, light_dir=LEFT, shade_fill=1.0)
+    covered = image[..., THICKNESS] > 0.0
+    assert np.allclose(image[covered, SHADE], image[covered, THICKNESS], atol=1e-5)
+    left, right = half_means(image)
+    assert left == pytest.approx(right, abs=1e-5)
+
+
+def test_empty_space_has_no_shade(oracle32):
+    _, image = render(oracle32, grid(oracle32), channels=SHADE_LAYOUT)
+    assert np.all(image[..., THICKNESS] == 0.0)
+    assert np.all(image[..., SHADE] == 0.0)
+
+
+# ---------------------------------------------------------------------- perspective (tests/render/test_view_cameras.py)
+
+def perspective(o, view, distance_m):
+    camera = view.camera
+    center_x = view.left_m + view.width_px * view.pixel_m / 2
+    center_y = view.top_m - view.height_px * view.pixel_m / 2
+    eye = camera.right * center_x + camera.up * center_y + camera.toward * distance_m
+    return o.view(camera.right, camera.up, camera.toward, (view.left_m - center_x) / distance_m,
+                  (view.top_m - center_y) / distance_m, view.pixel_m / distance_m, 1, eye)
+
+
+def test_a_far_narrow_perspective_camera_matches_the_orthographic_march(oracle32):
+    g = grid(oracle32)
+    g.smoke.data[...] = 3.0 * blob(CELLS, (8.0, 7.0, 9.0), 3.0)
+    g.temperature.data[...] = 293.0 + 900.0 * blob(CELLS, (8.0, 7.0, 9.0), 2.0)
+    view = make_domain_view(make_camera(25.0, 40.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    settings = RenderSettings()
+    ortho = render_oracle(oracle32, g, view, settings, SHADE_LAYOUT)
+    persp = render_oracle(oracle32, g, view, settings, SHADE_LAYOUT, params=perspective(oracle32, view, 200.0))
+    assert np.abs(persp[..., THICKNESS] - ortho[..., THICKNESS]).max() < 0.01
+    assert np.abs(persp[..., SHADE] - ortho[..., SHADE]).max() < 0.01
+    assert np.abs(persp[..., HEAT] - ortho[..., HEAT]).max() < 0.01 * ortho[..., HEAT].max()
+    assert ortho[..., THICKNESS].max() > 0.9
+
+
+def test_an_eye_inside_the_smoke_sees_only_the_smoke_in_front_of_it(oracle32):
+    g = grid(oracle32)
+    g.smoke.data[...] = 0.5
+    view = make_domain_view(make_camera(0.0, 0.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    camera = view.camera
+    params = oracle32.view(camera.right, camera.up, camera.toward, -0.5, 0.5, 1.0 / 9.0, 1, (0.0, 0.8, 0.3))
+    image = render_oracle(oracle32, g, view, RenderSettings(opacity_per_density=2.0), SHADE_LAYOUT, params=params,
+                          height=9, width=9)
+    assert image[4, 4, THICKNESS] == pytest.approx(1.0 - math.exp(-2.0 * 0.5 * 1.1), rel=1e-4)
+
+
+def test_view_params_keep_the_camera_basis(oracle32):
+    view = make_domain_view(make_camera(25.0, 40.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    params = view_params(oracle32, view)
+    assert tuple(params.right) == pytest.approx(tuple(view.camera.right), rel=1e-6)
+    assert params.perspective == 0
