@@ -144,6 +144,8 @@ def main() -> int:
     print(f"[bench] grid {GRID}, dt {dt:.6f} s, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)
 
+    tuned = solve.autotune() if hasattr(solve, "autotune") else {}
+    print(f"[bench] sweep tiles: {tuned}", file=sys.stderr, flush=True)
     step_index = 0
 
     def run(n):
@@ -174,7 +176,7 @@ def main() -> int:
 
     # dominant kernel: the fused red-black sweep on the top level, timed per launch with HIP events over a
     # repeat of the timed region (same substeps, same state progression)
-    sweep_ms, sweep_launches = time_kernel_family(lambda: run(min(args.steps, 12)), "rb_sweep_fused_top")
+    sweep_ms, sweep_launches = time_kernel_family(lambda: run(min(args.steps, 12)), "rb_sweep_tile_top")
     achieved = sweep_bytes(cells) / (sweep_ms * 1e-3) / 1e9 if sweep_launches else float("nan")
     substep_gbps = substep_bytes(cells, mean_cycles) * args.steps / elapsed / 1e9
 
@@ -189,12 +191,13 @@ def main() -> int:
                    "confinement": 0.3, "buoyancy": "alpha 0.1, beta 1.0", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_rb_sweep_fused (128x192x128 level, 12 B/cell)",
+                     "kernel": "k_rb_sweep_tile (128x192x128 level, 12 B/cell)",
                      "avg_launch_us": sweep_ms * 1e3, "launches": sweep_launches},
         "substep_roofline": {"bytes_per_substep": substep_bytes(cells, mean_cycles), "achieved": substep_gbps,
                              "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": substep_gbps / HBM_PEAK_GBPS,
                              "mean_cycles": mean_cycles},
         "final_relative_residual": residual,
+        "sweep_tiles": {"x".join(map(str, k)): v[0] for k, v in tuned.items()},
         "cfl_cells_per_step": cfl_cells,
     }
 

@@ -117,6 +117,25 @@ int plm_advect_velocity(const plm_grid_info *info, const float *u, const float *
                         float *v_hat, float *w_hat, float *u_next, float *v_next, float *w_next,
                         float dt_over_cell, void *stream);
 
+/* The same velocity advection with the clamp range carried from the forward pass: the forward pass also
+   writes the min/max of the 8 taps it blended into *_bounds (2 x the field's size: lo, then hi), and the
+   correction reads them instead of retracing (sim/advect.py:43-49 takes the range at that same source).
+   The trace's first, grid-aligned velocity sample skips its zero-weight taps. Bit-identical to
+   plm_advect_velocity for finite fields. */
+int plm_advect_velocity_bounded(const plm_grid_info *info, const float *u, const float *v, const float *w,
+                                float *u_hat, float *v_hat, float *w_hat, float *u_bounds, float *v_bounds,
+                                float *w_bounds, float *u_next, float *v_next, float *w_next, float dt_over_cell,
+                                void *stream);
+
+/* Stepper._advect_scalars, sim/stepper.py:54-60: MacCormack of two cell-centred fields (smoke and
+   temperature) that move with the same velocity, sharing one trace per cell; each field reads its
+   `outside` value beyond open faces. The second field's pointers may be NULL (one field). Bounds as in
+   plm_advect_velocity_bounded. The smoke budget (plm_smoke_budget) runs after this on the first field. */
+int plm_advect_scalars(const plm_grid_info *info, const float *a_data, float *a_hat, float *a_bounds, float *a_next,
+                       float a_outside, const float *b_data, float *b_hat, float *b_bounds, float *b_next,
+                       float b_outside, const float *u, const float *v, const float *w, float dt_over_cell,
+                       void *stream);
+
 /* ------------------------------------------------------------------ forces: sim/forces.py */
 
 /* buoyancy_kernel, sim/forces.py:13-31, on v. */
@@ -188,6 +207,18 @@ int plm_rb_sweeps(const plm_grid_info *info, float *q, const float *rhs, float *
 /* subtract_gradient_kernel on u, v, w, sim/pressure.py:95-116. */
 int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *w, const float *q, void *stream);
 
+/* Tile shapes of the two-colour sweep (i x j x 64 cells per block): fills up to `capacity` entries of each
+   (NULL skips an array) and returns how many shapes there are. */
+int32_t plm_sweep_tile_shapes(int32_t *ti, int32_t *tj, int32_t *threads, int32_t capacity);
+/* The project's autotuner for the sweep tile (no reference counterpart; the sweep is bit-identical under
+   every shape): times each shape on this level (2 * repeats sweeps ping-ponging q <-> scratch, which are
+   overwritten), writes the mean microseconds per sweep into times_us (may be NULL), pins the fastest shape
+   for this level size for the rest of the process and returns it in *chosen. Synchronises the stream. */
+int plm_autotune_sweep(const plm_grid_info *info, float *q, const float *rhs, float *scratch, int32_t repeats,
+                       float *times_us, int32_t *chosen, void *stream);
+/* Pins a tile shape for a level size (-1 restores the default). */
+int plm_set_sweep_tile(const int32_t dims[3], int32_t shape);
+
 /* ------------------------------------------------------------------ multigrid: sim/multigrid.py */
 
 /* residual_kernel, sim/multigrid.py:36-43. */
@@ -210,11 +241,13 @@ int plm_zero(int64_t count, float *f, void *stream);
    below `tolerance` the remaining cycles' kernels exit at once (no host round trip; graph-safe).
    norm_scratch: device doubles, at least plm_norm_scratch_doubles(info[0]); cycles_run (device int32[2],
    may be NULL): [0] receives the number of cycles this solve ran, [1] is a running total across solves
-   that the library only increments; norm_scratch[0] receives the last relative residual. */
+   that the library only increments; norm_scratch[0] receives the last relative residual.
+   resume != 0 continues the previous tolerance-mode solve (no initial norm; the gate and the cycle
+   count carry on), so a caller can run a predicted number of cycles, read the gate, and add more. */
 #define PLM_LDS_CELLS 8192
 int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, float *const *rhs,
-               float *const *residual, int32_t cycles, int32_t sweeps, double tolerance, double *norm_scratch,
-               int32_t *cycles_run, void *stream);
+               float *const *residual, int32_t cycles, int32_t sweeps, double tolerance, int32_t resume,
+               double *norm_scratch, int32_t *cycles_run, void *stream);
 int64_t plm_norm_scratch_doubles(const plm_grid_info *info);
 
 /* |rhs - A q|^2 and |rhs|^2 of one level in FP64, deterministic (fixed-order tree), into out[0..1]

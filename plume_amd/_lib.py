@@ -75,6 +75,8 @@ SIGNATURES = {
     "plm_maccormack_correct": [_INFO, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _F, _P],
     "plm_advect_maccormack": [_INFO, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _F, _I, _P],
     "plm_advect_velocity": [_INFO, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P],
+    "plm_advect_velocity_bounded": [_INFO, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _P],
+    "plm_advect_scalars": [_INFO, _P, _P, _P, _P, _F, _P, _P, _P, _P, _F, _P, _P, _P, _F, _P],
     "plm_buoyancy": [_INFO, _P, _P, _P, _F, _F, _F, _F, _F, _P],
     "plm_drag": [_INFO, _P, _P, _P, _P, _F, _P],
     "plm_center_velocity": [_INFO, _P, _P, _P, _P, _P],
@@ -92,11 +94,13 @@ SIGNATURES = {
     "plm_rb_sweep": [_INFO, _P, _P, _I, _F, _P],
     "plm_rb_sweeps": [_INFO, _P, _P, _P, _I, _F, _P],
     "plm_subtract_gradient": [_INFO, _P, _P, _P, _P, _P],
+    "plm_autotune_sweep": [_INFO, _P, _P, _P, _I, _P, _P, _P],
+    "plm_set_sweep_tile": [_P, _I],
     "plm_residual": [_INFO, _P, _P, _P, _P],
     "plm_restrict": [_P, _P, _P, _P, _P],
     "plm_prolong_add": [_P, _P, _P, _P, _P],
     "plm_zero": [_L, _P, _P],
-    "plm_vcycle": [_I, _P, _P, _P, _P, _I, _I, _D, _P, _P, _P],
+    "plm_vcycle": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _P, _P, _P],
     "plm_residual_norms": [_INFO, _P, _P, _P, _P, _P],
     "plm_line_sums": [_P, _P, _P, _P, _P, _P],
     "plm_smoke_budget": [_P, _P, _P, _P, _P, _P, _P],
@@ -122,6 +126,8 @@ def _load() -> ctypes.CDLL:
     lib.plm_abi_version.restype = ctypes.c_int
     lib.plm_norm_scratch_doubles.restype = ctypes.c_int64
     lib.plm_norm_scratch_doubles.argtypes = [_INFO]
+    lib.plm_sweep_tile_shapes.restype = ctypes.c_int32
+    lib.plm_sweep_tile_shapes.argtypes = [_P, _P, _P, _I]
     for name, argtypes in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = ctypes.c_int
@@ -130,7 +136,7 @@ def _load() -> ctypes.CDLL:
 
 
 LIB = _load()
-ABI_VERSION = 1
+ABI_VERSION = 2
 if LIB.plm_abi_version() != ABI_VERSION:
     raise ImportError(f"{LIBRARY} has ABI {LIB.plm_abi_version()}, expected {ABI_VERSION}: rebuild it")
 

@@ -1,16 +1,22 @@
 // Pressure projection and geometric multigrid: sim/pressure.py and sim/multigrid.py.
 //
-// Kernels that restate the reference one launch per kernel (colour sweeps, residual, restrict, prolong) are
-// kept for parity tests and small grids. The solver path uses three MI355X-shaped forms with the same
-// arithmetic, so results are bit-identical:
-//   - k_rb_sweep_fused: both colours of one sweep in one launch. A 64(k) x 8(j) tile streams along i with
-//     a 2-cell halo in LDS; red is updated on the tile plus a 1-cell ring, then black on the tile. It
-//     reads q_in and writes q_out (ping-pong), since a neighbouring tile's halo must see old values.
-//   - k_residual_restrict: the coarse right-hand side straight from q and rhs (residual never stored).
+// Every form below has the reference's arithmetic, so results are bit-identical to the colour-by-colour,
+// kernel-by-kernel restatement:
+//   - Couplings. neighbor_term's coupling area / (0.5 (own + neighbour fill)) takes at most a few distinct
+//     values per level (fills are 1 except the last cell per axis), so the host forms them once per level
+//     in float32, in the same operations (`Coupling`), and the kernels index them instead of dividing.
+//   - k_rb_sweep_tile: both colours of one sweep in one launch. A 64 (k) x 4 (j) x 4 (i) tile plus a 2-cell
+//     halo sits in LDS; red is updated in place on the tile and a 1-cell ring (red reads only black), then
+//     black on the tile. It reads q_in and writes q_out (ping-pong), since a neighbouring tile's halo must
+//     see old values. Its first sweep of a level can start from q = 0 (the coarse q the reference zeroes)
+//     or from q + prolongation of the coarser correction (prolong_add fused into the load).
+//   - k_residual_restrict: the coarse right-hand side straight from q and rhs (the residual is never stored).
 //   - k_mg_tail: every level from the first with few enough cells down to the coarsest, solved by one
-//     1024-thread workgroup in LDS, replacing ~90 launches per cycle at 128 x 192 x 128.
+//     1024-thread workgroup in LDS, instead of ~90 launches per cycle at 128 x 192 x 128.
 // Multigrid kernels take an optional device gate (tolerance mode): when gate[0] == 0 they return at once.
-#include <vector>
+#include <array>
+#include <map>
+#include <mutex>
 
 #include "common.h"
 
@@ -75,43 +81,103 @@ __global__ __launch_bounds__(256) void k_build_flags(plm_grid_info g, uint16_t *
 
 // ------------------------------------------------------------------ the finite-volume Poisson operator
 
-// neighbor_term / neighbor_sum, sim/pressure.py:52-77. `q(i, j, k)` fetches a pressure. UNIT: the level
-// has fill_hi = (1,1,1) and ghost_gap = 0.5 (the simulation grid), where every coupling is exactly 1, so
-// coupling * q = q and the divisions are skipped; the sums are formed in the reference's order.
+// The values neighbor_term (sim/pressure.py:52-69) and cell_volume (:47-49) can take on one level. With
+// La = "cell is the last on axis a" and the other two axes b < c, area = (1 * F_b) * F_c, F = fill_hi if last
+// else 1, indexed Lb + 2 Lc. An in-domain neighbour's coupling is area / (0.5 (own + its fill)), where
+// own + fill is 1 + 1 or 1 + fill_hi (index 0 / 1); an open-face ghost's is area / (0.5 own + ghost_gap),
+// own 1 or fill_hi (index 0 / 1). vol = (F_0 F_1) F_2, indexed L0 + 2 L1 + 4 L2.
+struct Coupling {
+    float in[3][4][2];
+    float ghost[3][4][2];
+    float vol[8];
+};
+
+struct LevelArgs {
+    plm_grid_info g;
+    Coupling cp;
+    int unit;  // every coupling and volume is exactly 1 (the simulation grid)
+};
+
+Coupling make_coupling(const plm_grid_info &g) {
+    Coupling cp;
+    for (int a = 0; a < 3; a++) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        for (int idx = 0; idx < 4; idx++) {
+            const float Fb = (idx & 1) ? g.fill_hi[b] : 1.0f, Fc = (idx & 2) ? g.fill_hi[c] : 1.0f;
+            volatile float area = 1.0f * Fb;  // volatile: keep each float operation as written
+            area = area * Fc;
+            const float fa = g.fill_hi[a];
+            volatile float d0 = 1.0f + 1.0f, d1 = 1.0f + fa;
+            volatile float h0 = 0.5f * d0, h1 = 0.5f * d1;
+            cp.in[a][idx][0] = area / h0;
+            cp.in[a][idx][1] = area / h1;
+            volatile float g0 = 0.5f * 1.0f, g1 = 0.5f * fa;
+            g0 = g0 + g.ghost_gap;
+            g1 = g1 + g.ghost_gap;
+            cp.ghost[a][idx][0] = area / g0;
+            cp.ghost[a][idx][1] = area / g1;
+        }
+    }
+    for (int idx = 0; idx < 8; idx++) {
+        volatile float v = ((idx & 1) ? g.fill_hi[0] : 1.0f) * ((idx & 2) ? g.fill_hi[1] : 1.0f);
+        v = v * ((idx & 4) ? g.fill_hi[2] : 1.0f);
+        cp.vol[idx] = v;
+    }
+    return cp;
+}
+
+bool unit_level(const plm_grid_info &g) {
+    return g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f && g.ghost_gap == 0.5f;
+}
+
+LevelArgs level_args(const plm_grid_info &g) {
+    LevelArgs L;
+    L.g = g;
+    L.cp = make_coupling(g);
+    L.unit = unit_level(g) ? 1 : 0;
+    return L;
+}
+
+// neighbor_sum, sim/pressure.py:72-77: (sum of coupling * q over the six neighbours, sum of couplings), added
+// in the reference's order (x-, x+, y-, y+, z-, z+), ghosts and closed faces adding 0 to the first.
+// UNIT levels skip the multiplications by an exact 1.
 template <bool UNIT, typename Q>
-__device__ __forceinline__ void neighbor_sum(const plm_grid_info &g, int i, int j, int k, Q q, float &t0,
-                                             float &t1) {
+__device__ __forceinline__ void neighbor_sum(const LevelArgs &L, int i, int j, int k, Q q, float &t0, float &t1) {
+    const plm_grid_info &g = L.g;
     const int cell[3] = {i, j, k};
+    const int last[3] = {i == g.dims[0] - 1, j == g.dims[1] - 1, k == g.dims[2] - 1};
     t0 = 0.0f;
     t1 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int step = s == 0 ? -1 : 1;
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const int area = last[b] + 2 * last[c];
+        // step -1
+        {
             int nb[3] = {i, j, k};
-            nb[a] += step;
+            nb[a] -= 1;
             float v0 = 0.0f, v1 = 0.0f;
-            if (UNIT) {
-                if (nb[a] >= 0 && nb[a] < g.dims[a]) {
-                    v0 = q(nb[0], nb[1], nb[2]);
-                    v1 = 1.0f;
-                } else if ((step < 0 && g.open_lo[a] == 1) || (step > 0 && g.open_hi[a] == 1)) {
-                    v1 = 1.0f;
-                }
-            } else {
-                float area = 1.0f;
-#pragma unroll
-                for (int other = 0; other < 3; other++)
-                    if (other != a) area = area * cell_fill(g, other, cell[other]);
-                const float own = cell_fill(g, a, cell[a]);
-                if (nb[a] >= 0 && nb[a] < g.dims[a]) {
-                    const float coupling = area / (0.5f * (own + cell_fill(g, a, nb[a])));
-                    v0 = coupling * q(nb[0], nb[1], nb[2]);
-                    v1 = coupling;
-                } else if ((step < 0 && g.open_lo[a] == 1) || (step > 0 && g.open_hi[a] == 1)) {
-                    v1 = area / (0.5f * own + g.ghost_gap);
-                }
+            if (nb[a] >= 0) {
+                const float cp = UNIT ? 1.0f : L.cp.in[a][area][last[a]];
+                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : cp * q(nb[0], nb[1], nb[2]);
+                v1 = cp;
+            } else if (g.open_lo[a] == 1) {
+                v1 = UNIT ? 1.0f : L.cp.ghost[a][area][last[a]];
+            }
+            t0 = t0 + v0;
+            t1 = t1 + v1;
+        }
+        // step +1
+        {
+            int nb[3] = {i, j, k};
+            nb[a] += 1;
+            float v0 = 0.0f, v1 = 0.0f;
+            if (nb[a] < g.dims[a]) {
+                const float cp = UNIT ? 1.0f : L.cp.in[a][area][cell[a] + 1 == g.dims[a] - 1];
+                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : cp * q(nb[0], nb[1], nb[2]);
+                v1 = cp;
+            } else if (g.open_hi[a] == 1) {
+                v1 = UNIT ? 1.0f : L.cp.ghost[a][area][1];
             }
             t0 = t0 + v0;
             t1 = t1 + v1;
@@ -119,24 +185,24 @@ __device__ __forceinline__ void neighbor_sum(const plm_grid_info &g, int i, int 
     }
 }
 
-// cell_volume, sim/pressure.py:47-49
-__device__ __forceinline__ float cell_volume(const plm_grid_info &g, int i, int j, int k) {
-    return cell_fill(g, 0, i) * cell_fill(g, 1, j) * cell_fill(g, 2, k);
+__device__ __forceinline__ float cell_volume(const LevelArgs &L, int i, int j, int k) {
+    const plm_grid_info &g = L.g;
+    return L.cp.vol[(i == g.dims[0] - 1) + 2 * (j == g.dims[1] - 1) + 4 * (k == g.dims[2] - 1)];
 }
 
-// One red_black_sweep_kernel update (sim/pressure.py:86-92); returns q unchanged when no coupling.
-__device__ __forceinline__ float gs_update(const plm_grid_info &g, int i, int j, int k, float q_c, float rhs_c,
-                                          float t0, float t1, float relaxation) {
+// One red_black_sweep_kernel update (sim/pressure.py:86-92); q unchanged when nothing couples
+__device__ __forceinline__ float gs_update(const LevelArgs &L, int i, int j, int k, float q_c, float rhs_c, float t0,
+                                          float t1, float relaxation) {
     if (t1 == 0.0f) return q_c;
-    const float source = cell_volume(g, i, j, k) * g.cell_m * g.cell_m * rhs_c;
+    const float source = cell_volume(L, i, j, k) * L.g.cell_m * L.g.cell_m * rhs_c;
     const float target = (t0 - source) / t1;
     return q_c + relaxation * (target - q_c);
 }
 
 // residual_kernel's value, sim/multigrid.py:40-43
-__device__ __forceinline__ float residual_value(const plm_grid_info &g, int i, int j, int k, float q_c, float rhs_c,
+__device__ __forceinline__ float residual_value(const LevelArgs &L, int i, int j, int k, float q_c, float rhs_c,
                                                float t0, float t1) {
-    const float scale = cell_volume(g, i, j, k) * g.cell_m * g.cell_m;
+    const float scale = cell_volume(L, i, j, k) * L.g.cell_m * L.g.cell_m;
     return rhs_c - (t0 - t1 * q_c) / scale;
 }
 
@@ -146,119 +212,124 @@ struct GlobalQ {
     __device__ float operator()(int i, int j, int k) const { return q[n.at(i, j, k)]; }
 };
 
-inline bool unit_level(const plm_grid_info &g) {
-    return g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f && g.ghost_gap == 0.5f;
-}
-
 // red_black_sweep_kernel, sim/pressure.py:80-92 (one colour, in place)
 template <bool UNIT>
-__global__ __launch_bounds__(256) void k_rb_sweep(plm_grid_info g, float *__restrict__ q,
-                                                  const float *__restrict__ rhs, int color, float relaxation,
-                                                  const double *gate) {
+__global__ __launch_bounds__(256) void k_rb_sweep(LevelArgs L, float *__restrict__ q, const float *__restrict__ rhs,
+                                                  int color, float relaxation, const double *gate) {
     if (gated_off(gate)) return;
-    const Dims n = cell_dims(g);
+    const Dims n = cell_dims(L.g);
     int i, j, k;
     if (!thread_cell(n, i, j, k)) return;
     if ((i + j + k) % 2 != color) return;
     float t0, t1;
-    neighbor_sum<UNIT>(g, i, j, k, GlobalQ{q, n}, t0, t1);
-    const int c = n.at(i, j, k);
+    neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
     if (t1 == 0.0f) return;
-    q[c] = gs_update(g, i, j, k, q[c], rhs[c], t0, t1, relaxation);
+    const int c = n.at(i, j, k);
+    q[c] = gs_update(L, i, j, k, q[c], rhs[c], t0, t1, relaxation);
 }
 
 // ------------------------------------------------------------------ both colours of a sweep in one launch
 
-constexpr int FK = 64, FJ = 8, FH = 2;           // tile, halo
-constexpr int PK = FK + 2 * FH, PJ = FJ + 2 * FH;  // LDS plane: 68 x 12
-constexpr int FTHREADS = FK * FJ;                  // 512
-constexpr int FCHUNK = 16;                         // i-planes per block
+constexpr int TK = 64, TH = 2, LK = TK + 2 * TH;
 
-struct LdsPlane {
-    const float *base;  // one plane of (PJ, PK) floats; (jj, kk) in tile-local coords, origin at -FH
-    int j0, k0;         // global (j, k) of the tile's interior origin
-    __device__ float operator()(int j, int k) const { return base[(j - j0 + FH) * PK + (k - k0 + FH)]; }
-};
+enum SweepStart { FROM_Q = 0, FROM_ZERO = 1, FROM_PROLONG = 2 };
 
-template <bool UNIT>
-__global__ __launch_bounds__(FTHREADS) void k_rb_sweep_fused(plm_grid_info g, const float *__restrict__ q_in,
-                                                             float *__restrict__ q_out,
-                                                             const float *__restrict__ rhs, float relaxation,
-                                                             const double *gate) {
+// sample_index_space of the coarse correction at a fine cell (prolong_add_kernel, sim/multigrid.py:63-68)
+__device__ __forceinline__ float prolongation(const float *__restrict__ coarse, Dims nc, int i, int j, int k) {
+    return sample_index(coarse, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f, (float)k * 0.5f - 0.25f);
+}
+
+// One two-colour sweep over a TI x TJ x 64 tile (i, j, k) with THREADS threads; LDS holds the tile plus a
+// 2-cell halo, (TI + 4) x (TJ + 4) x 68 floats.
+template <int TI, int TJ, int THREADS, bool UNIT, int START>
+__global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const float *__restrict__ q_in,
+                                                           float *__restrict__ q_out, const float *__restrict__ rhs,
+                                                           const float *__restrict__ coarse, Dims nc,
+                                                           float relaxation, const double *gate) {
+    constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH;
     if (gated_off(gate)) return;
-    __shared__ float old_ring[4][PJ * PK];  // old q, planes p..p+2 (+1 spare), tile + 2-cell halo
-    __shared__ float red_ring[3][PJ * PK];  // red cells updated, planes p-1..p+1, tile + 1-cell ring
-    const Dims n = cell_dims(g);
-    const int k0 = blockIdx.x * FK, j0 = blockIdx.y * FJ;
-    const int i_begin = blockIdx.z * FCHUNK, i_end = min(i_begin + FCHUNK, n.n0);
-    const int tid = threadIdx.x;
-
-    auto load_old = [&](int plane) {
-        float *dst = old_ring[(plane + 4) & 3];
-        for (int t = tid; t < PJ * PK; t += FTHREADS) {
-            const int jj = t / PK, kk = t % PK;
-            const int j = j0 - FH + jj, k = k0 - FH + kk;
-            float value = 0.0f;
-            if (plane >= 0 && plane < n.n0 && j >= 0 && j < n.n1 && k >= 0 && k < n.n2) value = q_in[n.at(plane, j, k)];
-            dst[t] = value;
+    __shared__ float s[LI * LJ * LK];
+    const Dims n = cell_dims(L.g);
+    const int k0 = blockIdx.x * TK, j0 = blockIdx.y * TJ, i0 = blockIdx.z * TI;
+    for (int t = threadIdx.x; t < LI * LJ * LK; t += THREADS) {
+        const int kk = t % LK, r = t / LK, jj = r % LJ, ii = r / LJ;
+        const int i = i0 - TH + ii, j = j0 - TH + jj, k = k0 - TH + kk;
+        float value = 0.0f;
+        if (i >= 0 && i < n.n0 && j >= 0 && j < n.n1 && k >= 0 && k < n.n2) {
+            if (START == FROM_Q) value = q_in[n.at(i, j, k)];
+            if (START == FROM_PROLONG) value = q_in[n.at(i, j, k)] + prolongation(coarse, nc, i, j, k);
         }
-    };
-    // old q at (i, j, k) for a point inside the old ring
-    auto old_at = [&](int i, int j, int k) {
-        return old_ring[(i + 4) & 3][(j - j0 + FH) * PK + (k - k0 + FH)];
-    };
-    // red update of plane `plane` on the tile and its 1-cell ring
-    auto red_plane = [&](int plane) {
-        float *dst = red_ring[(plane + 3) % 3];
-        if (plane < 0 || plane >= n.n0) return;
-        for (int t = tid; t < (FJ + 2) * (FK + 2); t += FTHREADS) {
-            const int jj = t / (FK + 2), kk = t % (FK + 2);
-            const int j = j0 - 1 + jj, k = k0 - 1 + kk;
-            if (j < 0 || j >= n.n1 || k < 0 || k >= n.n2) continue;
-            if (((plane + j + k) & 1) != 0) continue;
-            float t0, t1;
-            neighbor_sum<UNIT>(g, plane, j, k, [&](int a, int b, int c) { return old_at(a, b, c); }, t0, t1);
-            const float qc = old_at(plane, j, k);
-            dst[(j - j0 + FH) * PK + (k - k0 + FH)] = gs_update(g, plane, j, k, qc, rhs[n.at(plane, j, k)], t0, t1,
-                                                                relaxation);
-        }
-    };
-
-    // prologue: old planes i_begin-2 .. i_begin+1, red planes i_begin-1 and i_begin
-    for (int p = i_begin - 2; p <= i_begin + 1; p++) load_old(p);
-    __syncthreads();
-    red_plane(i_begin - 1);
-    red_plane(i_begin);
-    __syncthreads();
-
-    const int jl = tid / FK, kl = tid % FK;
-    const int j = j0 + jl, k = k0 + kl;
-    const bool mine = j < n.n1 && k < n.n2;
-    for (int p = i_begin; p < i_end; p++) {
-        load_old(p + 2);  // overwrites plane p-2, whose last reader was red_plane(p-1)
-        __syncthreads();
-        red_plane(p + 1);
-        __syncthreads();
-        if (mine) {
-            const int c = n.at(p, j, k);
-            const int slot = (j - j0 + FH) * PK + (k - k0 + FH);
-            float result;
-            if (((p + j + k) & 1) == 0) {
-                result = red_ring[(p + 3) % 3][slot];
-            } else {
-                float t0, t1;
-                neighbor_sum<UNIT>(g, p, j, k,
-                                   [&](int a, int b, int cc) {
-                                       return red_ring[(a + 3) % 3][(b - j0 + FH) * PK + (cc - k0 + FH)];
-                                   },
-                                   t0, t1);
-                result = gs_update(g, p, j, k, old_at(p, j, k), rhs[c], t0, t1, relaxation);
-            }
-            q_out[c] = result;
-        }
-        // the next iteration's load_old(p+3) overwrites plane p-1, read above by red_plane(p+1) and black(p)
-        __syncthreads();
+        s[t] = value;
     }
+    __syncthreads();
+    auto at = [&](int i, int j, int k) -> float & {
+        return s[((i - i0 + TH) * LJ + (j - j0 + TH)) * LK + (k - k0 + TH)];
+    };
+    // red (i + j + k even) on the tile and a 1-cell ring: 33 red cells in each 66-cell row
+    constexpr int RI = TI + 2, RJ = TJ + 2, RROW = (TK + 2) / 2;
+    for (int t = threadIdx.x; t < RI * RJ * RROW; t += THREADS) {
+        const int kr = t % RROW, row = t / RROW, jj = row % RJ, ii = row / RJ;
+        const int i = i0 - 1 + ii, j = j0 - 1 + jj;
+        const int k = k0 - 1 + 2 * kr + ((i + j + k0 - 1) & 1);
+        if (i < 0 || i >= n.n0 || j < 0 || j >= n.n1 || k < 0 || k >= n.n2) continue;
+        float t0, t1;
+        neighbor_sum<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
+        float &cell = at(i, j, k);
+        cell = gs_update(L, i, j, k, cell, rhs[n.at(i, j, k)], t0, t1, relaxation);
+    }
+    __syncthreads();
+    // black on the tile; every tile cell is written
+    for (int t = threadIdx.x; t < TI * TJ * TK; t += THREADS) {
+        const int kk = t % TK, r = t / TK, jj = r % TJ, ii = r / TJ;
+        const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
+        if (i >= n.n0 || j >= n.n1 || k >= n.n2) continue;
+        float value = at(i, j, k);
+        const int c = n.at(i, j, k);
+        if (((i + j + k) & 1) == 1) {
+            float t0, t1;
+            neighbor_sum<UNIT>(L, i, j, k, [&](int a, int b, int cc) { return at(a, b, cc); }, t0, t1);
+            value = gs_update(L, i, j, k, value, rhs[c], t0, t1, relaxation);
+        }
+        q_out[c] = value;
+    }
+}
+
+// The tile shapes the autotuner chooses among: TI, TJ (k is 64 wide), threads per block.
+struct TileShape {
+    int ti, tj, threads;
+};
+constexpr TileShape TILE_SHAPES[] = {{4, 4, 256}, {2, 8, 256}, {8, 4, 256}, {4, 8, 512}, {8, 8, 512}, {2, 4, 256}};
+constexpr int N_TILE_SHAPES = 6;
+constexpr int DEFAULT_TILE_SHAPE = 0;
+
+template <bool UNIT, int START>
+void launch_tile_shape(int shape, dim3 grid, hipStream_t s, const LevelArgs &L, const float *src, float *dst,
+                       const float *rhs, const float *coarse, Dims nc, float relaxation, const double *gate) {
+    switch (shape) {
+    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    default: k_rb_sweep_tile<4, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    }
+}
+
+// The shape pinned for a level size (by plm_autotune_sweep or plm_set_sweep_tile), else the default.
+struct TileChoices {
+    std::mutex mutex;
+    std::map<std::array<int, 3>, int> shape;
+};
+TileChoices &tile_choices() {
+    static TileChoices c;
+    return c;
+}
+
+int tile_shape_for(const Dims &n) {
+    TileChoices &c = tile_choices();
+    std::lock_guard<std::mutex> lock(c.mutex);
+    auto it = c.shape.find({n.n0, n.n1, n.n2});
+    return it == c.shape.end() ? DEFAULT_TILE_SHAPE : it->second;
 }
 
 // ------------------------------------------------------------------ gradient subtraction
@@ -292,15 +363,15 @@ __global__ __launch_bounds__(256) void k_subtract_gradient(plm_grid_info g, Tri<
 // ------------------------------------------------------------------ multigrid transfers
 
 template <bool UNIT>
-__global__ __launch_bounds__(256) void k_residual(plm_grid_info g, const float *__restrict__ q,
+__global__ __launch_bounds__(256) void k_residual(LevelArgs L, const float *__restrict__ q,
                                                   const float *__restrict__ rhs, float *__restrict__ r) {
-    const Dims n = cell_dims(g);
+    const Dims n = cell_dims(L.g);
     int i, j, k;
     if (!thread_cell(n, i, j, k)) return;
     float t0, t1;
-    neighbor_sum<UNIT>(g, i, j, k, GlobalQ{q, n}, t0, t1);
+    neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
     const int c = n.at(i, j, k);
-    r[c] = residual_value(g, i, j, k, q[c], rhs[c], t0, t1);
+    r[c] = residual_value(L, i, j, k, q[c], rhs[c], t0, t1);
 }
 
 // restrict_kernel, sim/multigrid.py:46-60
@@ -322,13 +393,14 @@ __global__ __launch_bounds__(256) void k_restrict(const float *__restrict__ fine
 }
 
 // residual_kernel + restrict_kernel fused: each coarse cell forms its children's residuals in registers.
+// coarse_q (may be NULL) is set to 0, the reference's coarse.q.zero_() (sim/multigrid.py:143).
 template <bool UNIT>
-__global__ __launch_bounds__(256) void k_residual_restrict(plm_grid_info g, const float *__restrict__ q,
+__global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const float *__restrict__ q,
                                                            const float *__restrict__ rhs,
-                                                           float *__restrict__ coarse, Dims nc,
-                                                           const double *gate) {
+                                                           float *__restrict__ coarse_rhs, float *__restrict__ coarse_q,
+                                                           Dims nc, const double *gate) {
     if (gated_off(gate)) return;
-    const Dims nf = cell_dims(g);
+    const Dims nf = cell_dims(L.g);
     int i, j, k;
     if (!thread_cell(nc, i, j, k)) return;
     float total = 0.0f, count = 0.0f;
@@ -338,13 +410,15 @@ __global__ __launch_bounds__(256) void k_residual_restrict(plm_grid_info g, cons
                 const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
                 if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
                     float t0, t1;
-                    neighbor_sum<UNIT>(g, fi, fj, fk, GlobalQ{q, nf}, t0, t1);
+                    neighbor_sum<UNIT>(L, fi, fj, fk, GlobalQ{q, nf}, t0, t1);
                     const int c = nf.at(fi, fj, fk);
-                    total = total + residual_value(g, fi, fj, fk, q[c], rhs[c], t0, t1);
+                    total = total + residual_value(L, fi, fj, fk, q[c], rhs[c], t0, t1);
                     count = count + 1.0f;
                 }
             }
-    coarse[nc.at(i, j, k)] = total / count;
+    const int c = nc.at(i, j, k);
+    coarse_rhs[c] = total / count;
+    if (coarse_q != nullptr) coarse_q[c] = 0.0f;
 }
 
 // prolong_add_kernel, sim/multigrid.py:63-68
@@ -354,8 +428,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ c
     int i, j, k;
     if (!thread_cell(nf, i, j, k)) return;
     const int c = nf.at(i, j, k);
-    fine[c] = fine[c] + sample_index(coarse, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f,
-                                     (float)k * 0.5f - 0.25f);
+    fine[c] = fine[c] + prolongation(coarse, nc, i, j, k);
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t count, float *__restrict__ f, const double *gate) {
@@ -382,9 +455,10 @@ constexpr int TAIL_THREADS = 1024;
 constexpr int MAX_TAIL = 8;
 
 struct TailLevels {
-    int count;              // levels in the tail
-    plm_grid_info info[MAX_TAIL];
-    int offset[MAX_TAIL];   // float offset of the level's q in LDS; rhs follows at offset + cells
+    int count;  // levels in the tail
+    int zero_top;  // the top level's q starts at 0 (a coarse level the V-cycle just cleared)
+    LevelArgs level[MAX_TAIL];
+    int offset[MAX_TAIL];  // float offset of the level's q in LDS; its rhs follows at offset + cells
     int cells[MAX_TAIL];
 };
 
@@ -394,149 +468,186 @@ struct LdsQ {
     __device__ float operator()(int i, int j, int k) const { return q[n.at(i, j, k)]; }
 };
 
-__device__ void tail_sweeps(const plm_grid_info &g, float *q, const float *rhs, int sweeps, float relaxation) {
-    const Dims n = cell_dims(g);
+__device__ __forceinline__ void decode(int c, Dims n, int &i, int &j, int &k) {
+    k = c % n.n2;
+    const int r = c / n.n2;
+    j = r % n.n1;
+    i = r / n.n1;
+}
+
+template <bool UNIT>
+__device__ __forceinline__ void tail_sweeps_t(const LevelArgs &L, float *q, const float *rhs, int sweeps, float relaxation) {
+    const Dims n = cell_dims(L.g);
     const int count = n.count();
-    const bool unit = g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f && g.ghost_gap == 0.5f;
     for (int s = 0; s < sweeps; s++) {
         for (int color = 0; color < 2; color++) {
             for (int c = threadIdx.x; c < count; c += TAIL_THREADS) {
-                const int k = c % n.n2, j = (c / n.n2) % n.n1, i = c / (n.n1 * n.n2);
+                int i, j, k;
+                decode(c, n, i, j, k);
                 if (((i + j + k) & 1) != color) continue;
                 float t0, t1;
-                if (unit)
-                    neighbor_sum<true>(g, i, j, k, LdsQ{q, n}, t0, t1);
-                else
-                    neighbor_sum<false>(g, i, j, k, LdsQ{q, n}, t0, t1);
-                q[c] = gs_update(g, i, j, k, q[c], rhs[c], t0, t1, relaxation);
+                neighbor_sum<UNIT>(L, i, j, k, LdsQ{q, n}, t0, t1);
+                q[c] = gs_update(L, i, j, k, q[c], rhs[c], t0, t1, relaxation);
             }
             __syncthreads();
         }
     }
 }
 
-__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels t, float *__restrict__ q_top,
+__device__ __forceinline__ void tail_sweeps(const LevelArgs &L, float *q, const float *rhs, int sweeps, float relaxation) {
+    if (L.unit)
+        tail_sweeps_t<true>(L, q, rhs, sweeps, relaxation);
+    else
+        tail_sweeps_t<false>(L, q, rhs, sweeps, relaxation);
+}
+
+template <bool UNIT>
+__device__ __forceinline__ void tail_restrict_t(const LevelArgs &L, const float *q, const float *rhs, Dims nc, float *crhs, float *cq) {
+    const Dims nf = cell_dims(L.g);
+    for (int c = threadIdx.x; c < nc.count(); c += TAIL_THREADS) {
+        int i, j, k;
+        decode(c, nc, i, j, k);
+        float total = 0.0f, count = 0.0f;
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int dk = 0; dk < 2; dk++) {
+                    const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+                    if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
+                        float t0, t1;
+                        neighbor_sum<UNIT>(L, fi, fj, fk, LdsQ{q, nf}, t0, t1);
+                        const int f = nf.at(fi, fj, fk);
+                        total = total + residual_value(L, fi, fj, fk, q[f], rhs[f], t0, t1);
+                        count = count + 1.0f;
+                    }
+                }
+        crhs[c] = total / count;
+        cq[c] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
                                                           const float *__restrict__ rhs_top, int sweeps,
                                                           const double *gate) {
     if (gated_off(gate)) return;
     extern __shared__ float lds[];
-    // top of the tail: q and rhs from global memory
+    // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
+    // be copied to per-thread scratch, so it is staged in LDS once.
+    __shared__ TailLevels t;
+    {
+        static_assert(sizeof(TailLevels) % 4 == 0, "TailLevels is copied as 32-bit words");
+        const int words = sizeof(TailLevels) / 4;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(&args);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&t);
+        for (int w = threadIdx.x; w < words; w += TAIL_THREADS) dst[w] = src[w];
+    }
+    __syncthreads();
     for (int c = threadIdx.x; c < t.cells[0]; c += TAIL_THREADS) {
-        lds[t.offset[0] + c] = q_top[c];
+        lds[t.offset[0] + c] = t.zero_top ? 0.0f : q_top[c];
         lds[t.offset[0] + t.cells[0] + c] = rhs_top[c];
     }
     __syncthreads();
     const int last = t.count - 1;
     // down: smooth, residual + restrict, zero the coarse q (sim/multigrid.py:137-143)
     for (int l = 0; l < last; l++) {
-        const plm_grid_info &g = t.info[l];
+        const LevelArgs &L = t.level[l];
         float *q = lds + t.offset[l];
         float *rhs = q + t.cells[l];
-        tail_sweeps(g, q, rhs, sweeps, 1.0f);
-        const Dims nf = cell_dims(g), nc = cell_dims(t.info[l + 1]);
+        tail_sweeps(L, q, rhs, sweeps, 1.0f);
         float *cq = lds + t.offset[l + 1];
         float *crhs = cq + t.cells[l + 1];
-        const bool unit = g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f && g.ghost_gap == 0.5f;
-        for (int c = threadIdx.x; c < t.cells[l + 1]; c += TAIL_THREADS) {
-            const int k = c % nc.n2, j = (c / nc.n2) % nc.n1, i = c / (nc.n1 * nc.n2);
-            float total = 0.0f, count = 0.0f;
-            for (int di = 0; di < 2; di++)
-                for (int dj = 0; dj < 2; dj++)
-                    for (int dk = 0; dk < 2; dk++) {
-                        const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
-                        if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
-                            float t0, t1;
-                            if (unit)
-                                neighbor_sum<true>(g, fi, fj, fk, LdsQ{q, nf}, t0, t1);
-                            else
-                                neighbor_sum<false>(g, fi, fj, fk, LdsQ{q, nf}, t0, t1);
-                            const int f = nf.at(fi, fj, fk);
-                            total = total + residual_value(g, fi, fj, fk, q[f], rhs[f], t0, t1);
-                            count = count + 1.0f;
-                        }
-                    }
-            crhs[c] = total / count;
-            cq[c] = 0.0f;
-        }
+        if (L.unit)
+            tail_restrict_t<true>(L, q, rhs, cell_dims(t.level[l + 1].g), crhs, cq);
+        else
+            tail_restrict_t<false>(L, q, rhs, cell_dims(t.level[l + 1].g), crhs, cq);
         __syncthreads();
     }
-    // coarsest: max(COARSEST_SWEEPS, COARSEST_SWEEPS_PER_CELL * longest) sweeps at 1.5 (sim/multigrid.py:132-136)
+    // coarsest: max(20, 4 * longest) sweeps at 1.5 (sim/multigrid.py:30-33, 132-136)
     {
-        const plm_grid_info &g = t.info[last];
-        const int longest = max(g.dims[0], max(g.dims[1], g.dims[2]));
+        const LevelArgs &L = t.level[last];
+        const int longest = max(L.g.dims[0], max(L.g.dims[1], L.g.dims[2]));
         float *q = lds + t.offset[last];
-        tail_sweeps(g, q, q + t.cells[last], max(20, 4 * longest), 1.5f);
+        tail_sweeps(L, q, q + t.cells[last], max(20, 4 * longest), 1.5f);
     }
     // up: prolong-add, smooth (sim/multigrid.py:145-147)
     for (int l = last - 1; l >= 0; l--) {
-        const plm_grid_info &g = t.info[l];
-        const Dims nf = cell_dims(g), nc = cell_dims(t.info[l + 1]);
+        const LevelArgs &L = t.level[l];
+        const Dims nf = cell_dims(L.g), nc = cell_dims(t.level[l + 1].g);
         float *q = lds + t.offset[l];
         const float *cq = lds + t.offset[l + 1];
         for (int c = threadIdx.x; c < t.cells[l]; c += TAIL_THREADS) {
-            const int k = c % nf.n2, j = (c / nf.n2) % nf.n1, i = c / (nf.n1 * nf.n2);
-            q[c] = q[c] + sample_index(cq, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f, (float)k * 0.5f - 0.25f);
+            int i, j, k;
+            decode(c, nf, i, j, k);
+            q[c] = q[c] + prolongation(cq, nc, i, j, k);
         }
         __syncthreads();
-        tail_sweeps(g, q, q + t.cells[l], sweeps, 1.0f);
+        tail_sweeps(L, q, q + t.cells[l], sweeps, 1.0f);
     }
     for (int c = threadIdx.x; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
 }
 
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
 
-constexpr int NORM_BLOCKS = 512, NORM_THREADS = 256;
+constexpr int NORM_THREADS = 256;
+constexpr int FINISH_THREADS = 1024;
 
+inline int64_t norm_blocks(const plm_grid_info &g) {
+    const dim3 grid = grid3(cell_dims(g));
+    return (int64_t)grid.x * grid.y * grid.z;
+}
+
+// per block: sum of r^2 and rhs^2 over its 256 cells, in a fixed tree order
 template <bool UNIT>
-__global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(plm_grid_info g, const float *__restrict__ q,
+__global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
                                                                 const float *__restrict__ rhs,
                                                                 double *__restrict__ partials, const double *gate) {
     if (gated_off(gate)) return;
-    const Dims n = cell_dims(g);
-    const int count = n.count();
+    const Dims n = cell_dims(L.g);
+    int i, j, k;
     double rr = 0.0, bb = 0.0;
-    for (int c = blockIdx.x * NORM_THREADS + threadIdx.x; c < count; c += NORM_BLOCKS * NORM_THREADS) {
-        const int k = c % n.n2, j = (c / n.n2) % n.n1, i = c / (n.n1 * n.n2);
+    if (thread_cell(n, i, j, k)) {
         float t0, t1;
-        neighbor_sum<UNIT>(g, i, j, k, GlobalQ{q, n}, t0, t1);
-        const double r = (double)residual_value(g, i, j, k, q[c], rhs[c], t0, t1);
+        neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
+        const int c = n.at(i, j, k);
+        const double r = (double)residual_value(L, i, j, k, q[c], rhs[c], t0, t1);
         const double b = (double)rhs[c];
-        rr += r * r;
-        bb += b * b;
+        rr = r * r;
+        bb = b * b;
     }
     __shared__ double srr[NORM_THREADS], sbb[NORM_THREADS];
-    srr[threadIdx.x] = rr;
-    sbb[threadIdx.x] = bb;
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    srr[tid] = rr;
+    sbb[tid] = bb;
     __syncthreads();
     for (int half = NORM_THREADS / 2; half > 0; half /= 2) {
-        if ((int)threadIdx.x < half) {
-            srr[threadIdx.x] += srr[threadIdx.x + half];
-            sbb[threadIdx.x] += sbb[threadIdx.x + half];
+        if (tid < half) {
+            srr[tid] += srr[tid + half];
+            sbb[tid] += sbb[tid + half];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = srr[0];
-        partials[2 * blockIdx.x + 1] = sbb[0];
+    if (tid == 0) {
+        const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * block] = srr[0];
+        partials[2 * block + 1] = sbb[0];
     }
 }
 
 // scratch layout: [0] relative residual, [1] gate (1 active / 0 converged), [2] |r|^2, [3] |rhs|^2,
-// [4 ...] 2 * NORM_BLOCKS partials. `mode`: 0 plain norms (no gate change), 1 initial check, 2 after a cycle.
-__global__ __launch_bounds__(NORM_THREADS) void k_norm_finish(double *scratch, double tolerance, int mode,
-                                                              int32_t *cycles_run) {
+// [4 ...] 2 per block partials. `mode`: 0 plain norms (no gate change), 1 initial check, 2 after a cycle.
+__global__ __launch_bounds__(FINISH_THREADS) void k_norm_finish(double *scratch, int64_t blocks, double tolerance,
+                                                                int mode, int32_t *cycles_run) {
     if (mode == 2 && scratch[1] == 0.0) return;
     const double *partials = scratch + 4;
-    __shared__ double srr[NORM_THREADS], sbb[NORM_THREADS];
+    __shared__ double srr[FINISH_THREADS], sbb[FINISH_THREADS];
     double rr = 0.0, bb = 0.0;
-    for (int b = threadIdx.x; b < NORM_BLOCKS; b += NORM_THREADS) {
+    for (int64_t b = threadIdx.x; b < blocks; b += FINISH_THREADS) {
         rr += partials[2 * b];
         bb += partials[2 * b + 1];
     }
     srr[threadIdx.x] = rr;
     sbb[threadIdx.x] = bb;
     __syncthreads();
-    for (int half = NORM_THREADS / 2; half > 0; half /= 2) {
+    for (int half = FINISH_THREADS / 2; half > 0; half /= 2) {
         if ((int)threadIdx.x < half) {
             srr[threadIdx.x] += srr[threadIdx.x + half];
             sbb[threadIdx.x] += sbb[threadIdx.x + half];
@@ -560,52 +671,81 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_finish(double *scratch, d
 
 // ------------------------------------------------------------------ host orchestration
 
-int launch_norms(const plm_grid_info &g, const float *q, const float *rhs, double *scratch, const double *gate,
-                 hipStream_t s) {
-    if (unit_level(g))
-        k_norm_partials<true><<<NORM_BLOCKS, NORM_THREADS, 0, s>>>(g, q, rhs, scratch + 4, gate);
+int launch_norms(const LevelArgs &L, const float *q, const float *rhs, double *scratch, const double *gate,
+                 double tolerance, int mode, int32_t *cycles_run, hipStream_t s) {
+    const Dims n = cell_dims(L.g);
+    if (L.unit)
+        k_norm_partials<true><<<grid3(n), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
     else
-        k_norm_partials<false><<<NORM_BLOCKS, NORM_THREADS, 0, s>>>(g, q, rhs, scratch + 4, gate);
-    return check_launch("norm partials");
+        k_norm_partials<false><<<grid3(n), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
+    PLM_TRY(check_launch("norm partials"));
+    k_norm_finish<<<1, FINISH_THREADS, 0, s>>>(scratch, norm_blocks(L.g), tolerance, mode, cycles_run);
+    return check_launch("norm finish");
 }
 
-// `sweeps` sweeps on one level. q is updated in place; scratch (a level-sized buffer) enables the fused
-// two-colour form, which ping-pongs q -> scratch -> q (an odd count starts with one colour-by-colour sweep).
-int launch_sweeps(const plm_grid_info &g, float *q, const float *rhs, float *scratch, int sweeps, float relaxation,
-                  const double *gate, hipStream_t s) {
-    const Dims n = cell_dims(g);
-    const bool unit = unit_level(g);
-    int first_fused = 0;
-    if (scratch == nullptr || n.count() < 4096) first_fused = sweeps;
-    else if (sweeps % 2 == 1) first_fused = 1;
+template <bool UNIT>
+int launch_tile(const LevelArgs &L, int start, const float *src, float *dst, const float *rhs, const float *coarse,
+                Dims nc, float relaxation, const double *gate, hipStream_t s, int shape = -1) {
+    const Dims n = cell_dims(L.g);
+    if (shape < 0) shape = tile_shape_for(n);
+    const TileShape &ts = TILE_SHAPES[shape];
+    const dim3 grid((n.n2 + TK - 1) / TK, (n.n1 + ts.tj - 1) / ts.tj, (n.n0 + ts.ti - 1) / ts.ti);
+    TimingScope launch_timing(UNIT ? "rb_sweep_tile_top" : "rb_sweep_tile", s);
+    if (start == FROM_ZERO)
+        launch_tile_shape<UNIT, FROM_ZERO>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+    else if (start == FROM_PROLONG)
+        launch_tile_shape<UNIT, FROM_PROLONG>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+    else
+        launch_tile_shape<UNIT, FROM_Q>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+    return check_launch("rb sweep tile");
+}
+
+// `sweeps` sweeps on one level, the result in q. Before the first: FROM_Q uses q; FROM_ZERO treats q as 0
+// (q need not be cleared); FROM_PROLONG adds the prolongation of `coarse` to q. With a level-sized scratch the
+// sweeps ping-pong q -> scratch -> q in two-colour tiles (an odd count ends with a copy back); without one,
+// the start is applied by its own kernel and the sweeps run colour by colour in place.
+int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch, int sweeps, float relaxation,
+                  int start, const float *coarse, Dims nc, const double *gate, hipStream_t s) {
+    const Dims n = cell_dims(L.g);
     TimingScope t("rb_sweeps", s);
-    for (int sw = 0; sw < first_fused; sw++) {
-        for (int color = 0; color < 2; color++) {
-            if (unit)
-                k_rb_sweep<true><<<grid3(n), block3(), 0, s>>>(g, q, rhs, color, relaxation, gate);
-            else
-                k_rb_sweep<false><<<grid3(n), block3(), 0, s>>>(g, q, rhs, color, relaxation, gate);
-            PLM_TRY(check_launch("rb sweep"));
+    if (scratch == nullptr || sweeps == 0) {
+        if (start == FROM_ZERO) {
+            k_zero<<<grid1(n.count()), 256, 0, s>>>(n.count(), q, gate);
+            PLM_TRY(check_launch("zero level"));
+        } else if (start == FROM_PROLONG) {
+            k_prolong_add<<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
+            PLM_TRY(check_launch("prolong add"));
         }
+        for (int sw = 0; sw < sweeps; sw++)
+            for (int color = 0; color < 2; color++) {
+                if (L.unit)
+                    k_rb_sweep<true><<<grid3(n), block3(), 0, s>>>(L, q, rhs, color, relaxation, gate);
+                else
+                    k_rb_sweep<false><<<grid3(n), block3(), 0, s>>>(L, q, rhs, color, relaxation, gate);
+                PLM_TRY(check_launch("rb sweep"));
+            }
+        return 0;
     }
-    const dim3 grid((n.n2 + FK - 1) / FK, (n.n1 + FJ - 1) / FJ, (n.n0 + FCHUNK - 1) / FCHUNK);
-    for (int sw = first_fused; sw < sweeps; sw++) {
-        const bool to_scratch = (sw - first_fused) % 2 == 0;
+    for (int sw = 0; sw < sweeps; sw++) {
+        const bool to_scratch = sw % 2 == 0;
         const float *src = to_scratch ? q : scratch;
         float *dst = to_scratch ? scratch : q;
-        TimingScope launch_timing(unit ? "rb_sweep_fused_top" : "rb_sweep_fused", s);
-        if (unit)
-            k_rb_sweep_fused<true><<<grid, FTHREADS, 0, s>>>(g, src, dst, rhs, relaxation, gate);
+        const int first = sw == 0 ? start : FROM_Q;
+        if (L.unit)
+            PLM_TRY(launch_tile<true>(L, first, src, dst, rhs, coarse, nc, relaxation, gate, s));
         else
-            k_rb_sweep_fused<false><<<grid, FTHREADS, 0, s>>>(g, src, dst, rhs, relaxation, gate);
-        PLM_TRY(check_launch("fused rb sweep"));
+            PLM_TRY(launch_tile<false>(L, first, src, dst, rhs, coarse, nc, relaxation, gate, s));
+    }
+    if (sweeps % 2 == 1) {
+        k_copy<<<grid1(n.count()), 256, 0, s>>>(n.count(), scratch, q, gate);
+        PLM_TRY(check_launch("copy back"));
     }
     return 0;
 }
 
 struct Hierarchy {
     int n_levels;
-    const plm_grid_info *infos;
+    LevelArgs level[16];
     float *const *q;
     float *const *rhs;
     float *const *residual;
@@ -616,34 +756,37 @@ struct Hierarchy {
     const double *gate;
 };
 
-int v_cycle(const Hierarchy &h, int index, hipStream_t s) {
+// _v_cycle, sim/multigrid.py:130-147, from level `index`; `cleared`: this level's q is the reference's
+// freshly zeroed coarse q (the caller did not write it)
+int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     if (index == h.tail_start) {
+        h.tail.zero_top = cleared ? 1 : 0;
         k_mg_tail<<<1, TAIL_THREADS, h.tail_bytes, s>>>(h.tail, h.q[index], h.rhs[index], h.sweeps, h.gate);
         return check_launch("multigrid tail");
     }
-    const plm_grid_info &g = h.infos[index];
-    const Dims n = cell_dims(g);
+    const LevelArgs &L = h.level[index];
+    const Dims none{1, 1, 1};
+    const int start = cleared ? FROM_ZERO : FROM_Q;
     if (index == h.n_levels - 1) {  // coarsest level outside the tail (only when the tail is disabled)
-        const int longest = max(g.dims[0], max(g.dims[1], g.dims[2]));
-        return launch_sweeps(g, h.q[index], h.rhs[index], h.residual[index], max(20, 4 * longest), 1.5f, h.gate, s);
+        const int longest = max(L.g.dims[0], max(L.g.dims[1], L.g.dims[2]));
+        return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], max(20, 4 * longest), 1.5f, start,
+                             nullptr, none, h.gate, s);
     }
-    // residual[index] doubles as the ping-pong buffer of this level's fused sweeps
-    PLM_TRY(launch_sweeps(g, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, h.gate, s));
-    const plm_grid_info &cg = h.infos[index + 1];
-    const Dims nc = cell_dims(cg);
-    if (unit_level(g))
-        k_residual_restrict<true><<<grid3(nc), block3(), 0, s>>>(g, h.q[index], h.rhs[index], h.rhs[index + 1], nc,
-                                                                h.gate);
+    // residual[index] doubles as the ping-pong buffer of this level's sweeps
+    PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
+                          h.gate, s));
+    const Dims nc = cell_dims(h.level[index + 1].g);
+    // the coarse q is cleared by the next level's first sweep (or the tail), so it is not written here
+    if (L.unit)
+        k_residual_restrict<true><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+                                                                nullptr, nc, h.gate);
     else
-        k_residual_restrict<false><<<grid3(nc), block3(), 0, s>>>(g, h.q[index], h.rhs[index], h.rhs[index + 1], nc,
-                                                                 h.gate);
+        k_residual_restrict<false><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+                                                                 nullptr, nc, h.gate);
     PLM_TRY(check_launch("residual restrict"));
-    k_zero<<<grid1(nc.count()), 256, 0, s>>>(nc.count(), h.q[index + 1], h.gate);
-    PLM_TRY(check_launch("zero coarse"));
-    PLM_TRY(v_cycle(h, index + 1, s));
-    k_prolong_add<<<grid3(n), block3(), 0, s>>>(h.q[index + 1], nc, h.q[index], n, h.gate);
-    PLM_TRY(check_launch("prolong add"));
-    return launch_sweeps(g, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, h.gate, s);
+    PLM_TRY(v_cycle(h, index + 1, true, s));
+    return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, FROM_PROLONG,
+                         h.q[index + 1], nc, h.gate, s);
 }
 
 }  // namespace
@@ -682,11 +825,12 @@ int plm_rb_sweep(const plm_grid_info *info, float *q, const float *rhs, int32_t 
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(q && rhs, "plm_rb_sweep: NULL array");
     PLM_REQUIRE(color == 0 || color == 1, "plm_rb_sweep: color must be 0 or 1");
+    const LevelArgs L = level_args(*info);
     const Dims n = cell_dims(*info);
-    if (unit_level(*info))
-        k_rb_sweep<true><<<grid3(n), block3(), 0, as_stream(stream)>>>(*info, q, rhs, color, relaxation, nullptr);
+    if (L.unit)
+        k_rb_sweep<true><<<grid3(n), block3(), 0, as_stream(stream)>>>(L, q, rhs, color, relaxation, nullptr);
     else
-        k_rb_sweep<false><<<grid3(n), block3(), 0, as_stream(stream)>>>(*info, q, rhs, color, relaxation, nullptr);
+        k_rb_sweep<false><<<grid3(n), block3(), 0, as_stream(stream)>>>(L, q, rhs, color, relaxation, nullptr);
     return check_launch("plm_rb_sweep");
 }
 
@@ -695,7 +839,8 @@ int plm_rb_sweeps(const plm_grid_info *info, float *q, const float *rhs, float *
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(q && rhs, "plm_rb_sweeps: NULL array");
     PLM_REQUIRE(sweeps >= 0, "plm_rb_sweeps: sweeps must be >= 0");
-    return launch_sweeps(*info, q, rhs, scratch, sweeps, relaxation, nullptr, as_stream(stream));
+    return launch_sweeps(level_args(*info), q, rhs, scratch, sweeps, relaxation, FROM_Q, nullptr, Dims{1, 1, 1},
+                         nullptr, as_stream(stream));
 }
 
 int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *w, const float *q, void *stream) {
@@ -708,11 +853,12 @@ int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *
 int plm_residual(const plm_grid_info *info, const float *q, const float *rhs, float *residual, void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(q && rhs && residual, "plm_residual: NULL array");
+    const LevelArgs L = level_args(*info);
     const Dims n = cell_dims(*info);
-    if (unit_level(*info))
-        k_residual<true><<<grid3(n), block3(), 0, as_stream(stream)>>>(*info, q, rhs, residual);
+    if (L.unit)
+        k_residual<true><<<grid3(n), block3(), 0, as_stream(stream)>>>(L, q, rhs, residual);
     else
-        k_residual<false><<<grid3(n), block3(), 0, as_stream(stream)>>>(*info, q, rhs, residual);
+        k_residual<false><<<grid3(n), block3(), 0, as_stream(stream)>>>(L, q, rhs, residual);
     return check_launch("plm_residual");
 }
 
@@ -745,9 +891,74 @@ int plm_zero(int64_t count, float *f, void *stream) {
     return check_launch("plm_zero");
 }
 
+int32_t plm_sweep_tile_shapes(int32_t *ti, int32_t *tj, int32_t *threads, int32_t capacity) {
+    for (int v = 0; v < N_TILE_SHAPES && v < capacity; v++) {
+        if (ti) ti[v] = TILE_SHAPES[v].ti;
+        if (tj) tj[v] = TILE_SHAPES[v].tj;
+        if (threads) threads[v] = TILE_SHAPES[v].threads;
+    }
+    return N_TILE_SHAPES;
+}
+
+int plm_set_sweep_tile(const int32_t dims[3], int32_t shape) {
+    PLM_TRY(validate_dims(dims, "plm_set_sweep_tile"));
+    PLM_REQUIRE(shape >= -1 && shape < N_TILE_SHAPES, "plm_set_sweep_tile: shape %d out of range", shape);
+    TileChoices &c = tile_choices();
+    std::lock_guard<std::mutex> lock(c.mutex);
+    if (shape < 0)
+        c.shape.erase({dims[0], dims[1], dims[2]});
+    else
+        c.shape[{dims[0], dims[1], dims[2]}] = shape;
+    return 0;
+}
+
+int plm_autotune_sweep(const plm_grid_info *info, float *q, const float *rhs, float *scratch, int32_t repeats,
+                       float *times_us, int32_t *chosen, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(q && rhs && scratch && chosen, "plm_autotune_sweep: NULL argument");
+    PLM_REQUIRE(repeats >= 1, "plm_autotune_sweep: repeats must be >= 1");
+    hipStream_t s = as_stream(stream);
+    const LevelArgs L = level_args(*info);
+    const Dims n = cell_dims(*info), none{1, 1, 1};
+    hipEvent_t start, stop;
+    PLM_REQUIRE(hipEventCreate(&start) == hipSuccess && hipEventCreate(&stop) == hipSuccess,
+                "plm_autotune_sweep: event creation failed");
+    int best = DEFAULT_TILE_SHAPE;
+    float best_ms = 0.0f;
+    int rc = 0;
+    for (int v = 0; v < N_TILE_SHAPES && rc == 0; v++) {
+        auto sweep = [&](int r) {
+            const bool even = r % 2 == 0;
+            return L.unit ? launch_tile<true>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
+                                              1.0f, nullptr, s, v)
+                          : launch_tile<false>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
+                                               1.0f, nullptr, s, v);
+        };
+        rc = sweep(0);  // warm-up
+        if (rc == 0 && hipEventRecord(start, s) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: event record failed");
+        for (int r = 0; r < 2 * repeats && rc == 0; r++) rc = sweep(r + 1);
+        if (rc == 0 && hipEventRecord(stop, s) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: event record failed");
+        if (rc == 0 && hipEventSynchronize(stop) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: synchronise failed");
+        float ms = 0.0f;
+        if (rc == 0) (void)hipEventElapsedTime(&ms, start, stop);
+        ms /= (float)(2 * repeats);
+        if (times_us) times_us[v] = ms * 1000.0f;
+        if (rc == 0 && (v == 0 || ms < best_ms)) {
+            best = v;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    PLM_TRY(rc);
+    *chosen = best;
+    (void)n;
+    return plm_set_sweep_tile(info->dims, best);
+}
+
 int64_t plm_norm_scratch_doubles(const plm_grid_info *info) {
-    (void)info;
-    return 4 + 2 * NORM_BLOCKS;
+    if (validate_info(info) != 0) return -1;
+    return 4 + 2 * norm_blocks(*info);
 }
 
 int plm_residual_norms(const plm_grid_info *info, const float *q, const float *rhs, double *scratch, double *out,
@@ -755,17 +966,15 @@ int plm_residual_norms(const plm_grid_info *info, const float *q, const float *r
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(q && rhs && scratch && out, "plm_residual_norms: NULL array");
     hipStream_t s = as_stream(stream);
-    PLM_TRY(launch_norms(*info, q, rhs, scratch, nullptr, s));
-    k_norm_finish<<<1, NORM_THREADS, 0, s>>>(scratch, 0.0, 0, nullptr);
-    PLM_TRY(check_launch("norm finish"));
+    PLM_TRY(launch_norms(level_args(*info), q, rhs, scratch, nullptr, 0.0, 0, nullptr, s));
     PLM_REQUIRE(hipMemcpyAsync(out, scratch + 2, 2 * sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess,
                 "plm_residual_norms: copy failed");
     return 0;
 }
 
 int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, float *const *rhs,
-               float *const *residual, int32_t cycles, int32_t sweeps, double tolerance, double *norm_scratch,
-               int32_t *cycles_run, void *stream) {
+               float *const *residual, int32_t cycles, int32_t sweeps, double tolerance, int32_t resume,
+               double *norm_scratch, int32_t *cycles_run, void *stream) {
     PLM_REQUIRE(n_levels >= 1 && n_levels <= 16 && infos && q && rhs && residual, "plm_vcycle: bad level arrays");
     PLM_REQUIRE(cycles >= 0 && sweeps >= 0, "plm_vcycle: negative cycles or sweeps");
     for (int l = 0; l < n_levels; l++) {
@@ -782,7 +991,7 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
 
     Hierarchy h{};
     h.n_levels = n_levels;
-    h.infos = infos;
+    for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
     h.q = q;
     h.rhs = rhs;
     h.residual = residual;
@@ -801,8 +1010,8 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
         h.tail.count = n_levels - h.tail_start;
         int offset = 0;
         for (int m = 0; m < h.tail.count; m++) {
+            h.tail.level[m] = h.level[h.tail_start + m];
             const plm_grid_info &g = infos[h.tail_start + m];
-            h.tail.info[m] = g;
             h.tail.cells[m] = g.dims[0] * g.dims[1] * g.dims[2];
             h.tail.offset[m] = offset;
             offset += 2 * h.tail.cells[m];
@@ -811,18 +1020,12 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
     }
 
     TimingScope t("multigrid", s);
-    if (tolerance_mode) {
-        PLM_TRY(launch_norms(infos[0], q[0], rhs[0], norm_scratch, nullptr, s));
-        k_norm_finish<<<1, NORM_THREADS, 0, s>>>(norm_scratch, tolerance, 1, cycles_run);
-        PLM_TRY(check_launch("norm finish"));
-    }
+    if (tolerance_mode && !resume)
+        PLM_TRY(launch_norms(h.level[0], q[0], rhs[0], norm_scratch, nullptr, tolerance, 1, cycles_run, s));
     for (int c = 0; c < cycles; c++) {
-        PLM_TRY(v_cycle(h, 0, s));
-        if (tolerance_mode) {
-            PLM_TRY(launch_norms(infos[0], q[0], rhs[0], norm_scratch, h.gate, s));
-            k_norm_finish<<<1, NORM_THREADS, 0, s>>>(norm_scratch, tolerance, 2, cycles_run);
-            PLM_TRY(check_launch("norm finish"));
-        }
+        PLM_TRY(v_cycle(h, 0, false, s));
+        if (tolerance_mode)
+            PLM_TRY(launch_norms(h.level[0], q[0], rhs[0], norm_scratch, h.gate, tolerance, 2, cycles_run, s));
     }
     return 0;
 }

@@ -95,7 +95,7 @@ extern "C" {
 
 const char *plm_last_error(void) { return plm::g_last_error; }
 
-int plm_abi_version(void) { return 1; }
+int plm_abi_version(void) { return 2; }
 
 int plm_timing_enable(const char *family) {
     PLM_REQUIRE(family != nullptr, "timing family is NULL");

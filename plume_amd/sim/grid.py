@@ -46,6 +46,8 @@ class AdvectedField:
         self.hat = torch.zeros(shape, dtype=torch.float32, device=device)
         self.back = torch.zeros(shape, dtype=torch.float32, device=device)
         self.next = torch.zeros(shape, dtype=torch.float32, device=device)
+        # the clamp range the forward pass carries to the correction: lo, then hi
+        self.bounds = torch.zeros((2,) + tuple(shape), dtype=torch.float32, device=device)
 
     @property
     def shape(self):

@@ -22,6 +22,7 @@ from plume_amd.sim.grid import grid_info
 
 MIN_LEVEL_CELLS = 2
 MAX_LEVELS = 8
+PLM_LDS_CELLS = 8192  # include/plume_hip.h: levels this small (together) are solved in one workgroup's LDS
 COARSEST_SWEEPS = 20
 COARSEST_SWEEPS_PER_CELL = 4
 COARSEST_RELAXATION = 1.5
@@ -83,6 +84,7 @@ class Multigrid:
         self.norm_scratch = torch.zeros(doubles, dtype=torch.float64, device=self.device)
         # [0] cycles of the last solve, [1] running total (tolerance mode)
         self.cycles_run = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._predicted = 1
 
     def _pointers(self):
         array = ctypes.c_void_p * len(self.levels)
@@ -92,11 +94,47 @@ class Multigrid:
     def __call__(self, grid) -> None:
         self.levels[0].q = grid.pressure
         self.levels[0].rhs = grid.divergence
+        if self.tolerance <= 0.0:
+            self._cycles(self.cycles, resume=False)
+            return
+        # Tolerance mode: run the cycle count the last solve needed (cycles past convergence exit at once
+        # on the device gate), then add single cycles while the gate says the target is not met. One host
+        # read per solve in the usual case, instead of a launch per cycle up to the cap.
+        predicted = max(1, min(self._predicted, self.max_cycles))
+        self._cycles(predicted, resume=False)
+        ran = predicted
+        while ran < self.max_cycles and float(self.norm_scratch[1].item()) != 0.0:
+            self._cycles(1, resume=True)
+            ran += 1
+        self._predicted = max(1, int(self.cycles_run[0].item()))
+
+    def _cycles(self, count: int, resume: bool) -> None:
         q, rhs, residual = self._pointers()
-        cycles = self.max_cycles if self.tolerance > 0.0 else self.cycles
-        call("plm_vcycle", len(self.levels), self._infos, q, rhs, residual, cycles, self.sweeps, self.tolerance,
-             ptr(self.norm_scratch, torch.float64) if self.tolerance > 0.0 else None,
-             ptr(self.cycles_run, torch.int32) if self.tolerance > 0.0 else None, stream())
+        tolerance_mode = self.tolerance > 0.0
+        call("plm_vcycle", len(self.levels), self._infos, q, rhs, residual, int(count), self.sweeps, self.tolerance,
+             1 if resume else 0, ptr(self.norm_scratch, torch.float64) if tolerance_mode else None,
+             ptr(self.cycles_run, torch.int32) if tolerance_mode else None, stream())
+
+    def autotune(self, repeats: int = 5) -> dict:
+        """Times every tile shape of the two-colour sweep on each level that runs it (those above the LDS
+        tail) and pins the fastest per level size, process-wide. Uses scratch tensors, not the solve's
+        state; synchronises. Returns {level cells: (chosen shape, microseconds per shape)}."""
+        shapes = int(LIB.plm_sweep_tile_shapes(None, None, None, 0))
+        tail_cells = 0
+        tuned = {}
+        for level in reversed(self.levels):
+            tail_cells += int(level.q.numel())
+            if tail_cells <= PLM_LDS_CELLS:
+                continue
+            q = torch.zeros_like(level.q)
+            rhs = torch.randn(level.cells, generator=torch.Generator().manual_seed(1)).to(self.device)
+            scratch = torch.empty_like(level.q)
+            times = (ctypes.c_float * shapes)()
+            chosen = ctypes.c_int32()
+            call("plm_autotune_sweep", ctypes.byref(level.info), ptr(q), ptr(rhs), ptr(scratch), int(repeats), times,
+                 ctypes.byref(chosen), stream())
+            tuned[level.cells] = (chosen.value, [round(t, 2) for t in times])
+        return tuned
 
     def last_relative_residual(self) -> float:
         """The FP64 |rhs - A q| / |rhs| after the last solve in tolerance mode (synchronises)."""

@@ -2,7 +2,9 @@
 drag, advect velocity (all three from the old velocity, then swap), project, advect smoke and hold its
 budget, advect temperature, dissipate."""
 
-from plume_amd.sim.advect import advect_maccormack, advect_velocity
+import numpy as np
+
+from plume_amd.sim.advect import advect_scalars, advect_velocity
 from plume_amd.sim.conserve import SmokeBudget
 from plume_amd.sim.dissipate import dissipate
 from plume_amd.sim.emitters import emit, emitter_params
@@ -29,7 +31,8 @@ class Stepper:
         """Advances from `time_s` to `time_s + dt` (sim/stepper.py:28-44)."""
         grid, recipe = self.grid, self.recipe
         ambient = recipe.domain.ambient_k
-        emit(grid, self.emitters, recipe.seed, time_s, dt)
+        if self._any_emitter_active(time_s):
+            emit(grid, self.emitters, recipe.seed, time_s, dt)
         forces = recipe.forces
         apply_buoyancy(grid, forces.gravity_mps2, forces.heat_buoyancy, forces.smoke_weight, ambient, dt)
         if self.confinement is not None:
@@ -43,6 +46,12 @@ class Stepper:
         self._advect_scalars(dt)
         dissipate(grid, recipe.dissipation.smoke_per_s, recipe.dissipation.cooling_per_s, ambient, dt)
 
+    def _any_emitter_active(self, time_s: float) -> bool:
+        """is_active (sim/emitters.py:80-82) of any emitter, compared in float32 as the kernels compare; when
+        none is active the emit kernels would change nothing, so they are not launched."""
+        t = np.float32(time_s)
+        return any(np.float32(p.start_s) <= t < np.float32(p.end_s) for p in self.emitters.params)
+
     def _advect_velocity(self, dt: float) -> None:
         """sim/stepper.py:46-52, as one fused launch pair over u, v and w."""
         advect_velocity(self.grid, dt)
@@ -50,10 +59,10 @@ class Stepper:
             field.swap()
 
     def _advect_scalars(self, dt: float) -> None:
-        """sim/stepper.py:54-60."""
+        """sim/stepper.py:54-60. Smoke and temperature share one trace (temperature does not read smoke, so
+        advecting it before the smoke budget changes nothing)."""
         velocity = tuple(field.data for field in self.grid.velocity_fields())
-        advect_maccormack(self.grid.smoke, velocity, self.grid, dt)
+        advect_scalars(self.grid, velocity, dt)
         self.smoke_budget.hold(self.grid)
         self.grid.smoke.swap()
-        advect_maccormack(self.grid.temperature, velocity, self.grid, dt)
         self.grid.temperature.swap()

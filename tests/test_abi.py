@@ -36,7 +36,8 @@ def test_library_exports_every_declared_symbol():
 def test_every_declared_entry_point_has_a_binding():
     import plume_amd._lib as lib
 
-    bound = set(lib.SIGNATURES) | {"plm_last_error", "plm_abi_version", "plm_norm_scratch_doubles"}
+    bound = set(lib.SIGNATURES) | {"plm_last_error", "plm_abi_version", "plm_norm_scratch_doubles",
+                                    "plm_sweep_tile_shapes"}
     assert set(declared_functions()) == bound
 
 

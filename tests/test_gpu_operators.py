@@ -116,6 +116,38 @@ def test_fused_velocity_advection_matches_per_field_oracle(device, oracle32, cel
         assert bit_equal(to_np(field.next), ofield.next)
 
 
+@pytest.mark.parametrize("cells, faces", GRIDS)
+def test_velocity_advection_with_carried_bounds_is_bit_exact(device, oracle32, cells, faces):
+    from oracle.sim import advect_maccormack as oracle_advect
+    from plume_amd.sim.advect import advect_velocity
+
+    gpu, orc = pair(device, oracle32, cells, faces, k=5)
+    gpu.u.data[2, 3, 4] = -0.0  # a -0.0 operand on an aligned axis takes its partner's sign
+    orc.u.data[2, 3, 4] = -0.0
+    advect_velocity(gpu, 2 * DT, carry_bounds=True)
+    velocity = tuple(f.data for f in orc.velocity_fields())
+    for field, ofield in zip(gpu.velocity_fields(), orc.velocity_fields()):
+        oracle_advect(ofield, velocity, orc, 2 * DT)
+        assert bit_equal(to_np(field.hat), ofield.hat)
+        assert bit_equal(to_np(field.next), ofield.next)
+
+
+@pytest.mark.parametrize("cells, faces", GRIDS)
+@pytest.mark.parametrize("fields", [("smoke", "temperature"), ("smoke",)], ids=["pair", "single"])
+def test_scalar_advection_sharing_one_trace_is_bit_exact(device, oracle32, cells, faces, fields):
+    from oracle.sim import advect_maccormack as oracle_advect
+    from plume_amd.sim.advect import advect_scalars
+
+    gpu, orc = pair(device, oracle32, cells, faces, k=6, ambient=293.0)
+    velocity = tuple(f.data for f in gpu.velocity_fields())
+    advect_scalars(gpu, velocity, 3 * DT, [getattr(gpu, name) for name in fields])
+    for name in fields:
+        ofield = getattr(orc, name)
+        oracle_advect(ofield, tuple(f.data for f in orc.velocity_fields()), orc, 3 * DT)
+        assert bit_equal(to_np(getattr(gpu, name).hat), ofield.hat), name
+        assert bit_equal(to_np(getattr(gpu, name).next), ofield.next), name
+
+
 # ---------------------------------------------------------------------- forces
 
 @pytest.mark.parametrize("cells, faces", GRIDS)
@@ -246,6 +278,51 @@ def test_fused_sweep_on_large_grids_is_bit_exact(device, oracle32, cells):
         red_black_sweeps(gpu.pressure, gpu.divergence, gpu.info, gpu.cells, device, 4, 1.0, scratch)
         oracle32.rb_sweeps(orc.info, orc.pressure, orc.divergence, 4, 1.0)
         assert bit_equal(to_np(gpu.pressure), orc.pressure)
+
+
+@pytest.mark.parametrize("level", [0, 1], ids=["top", "coarse"])
+def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level):
+    """The autotuner may pick any tile shape; each gives the reference's sweep, bit for bit (a coarse level
+    exercises the overhang couplings)."""
+    from oracle.sim import level_infos
+    from plume_amd._lib import LIB, call, int3, ptr, stream
+    from plume_amd.sim.multigrid import build_levels
+    from plume_amd.sim.pressure import red_black_sweeps
+
+    cells = (45, 38, 70)
+    gpu, orc = pair(device, oracle32, cells, DEFAULT_FACES)
+    lv = build_levels(gpu)[level]
+    info = level_infos(oracle32, cells, CELL_M, orc.origin_m, orc.open_lo, orc.open_hi)[level][1]
+    r = rng(21)
+    rhs = r.uniform(-1, 1, lv.cells).astype(np.float32)
+    q0 = r.uniform(-1e-3, 1e-3, lv.cells).astype(np.float32)
+    expected = q0.copy()
+    oracle32.rb_sweeps(info, expected, rhs.copy(), 2, 1.0)
+    shapes = int(LIB.plm_sweep_tile_shapes(None, None, None, 0))
+    rhs_t = torch.from_numpy(rhs).to(device)
+    scratch = torch.empty_like(rhs_t)
+    try:
+        for shape in range(shapes):
+            call("plm_set_sweep_tile", int3(lv.cells), shape)
+            q = torch.from_numpy(q0).to(device)
+            red_black_sweeps(q, rhs_t, lv.info, lv.cells, device, 2, 1.0, scratch)
+            assert bit_equal(to_np(q), expected), shape
+    finally:
+        call("plm_set_sweep_tile", int3(lv.cells), -1)
+
+
+def test_autotune_pins_a_shape_and_keeps_results(device, oracle32):
+    from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
+    from plume_amd.sim.multigrid import Multigrid
+    from plume_amd.sim.pressure import project
+
+    gpu, orc = pair(device, oracle32, (96, 128, 96), DEFAULT_FACES)
+    solve = Multigrid(gpu, 2, 2)
+    tuned = solve.autotune(repeats=2)
+    assert (96, 128, 96) in tuned and 0 <= tuned[(96, 128, 96)][0] < 6
+    project(gpu, solve)
+    oracle_project(orc, OracleMultigrid(orc, 2, 2))
+    assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
 @pytest.mark.parametrize("cells, faces", GRIDS)
