@@ -453,13 +453,31 @@ inline dim3 grid1(int64_t count) {
 
 constexpr int TAIL_THREADS = 1024;
 constexpr int MAX_TAIL = 8;
+constexpr int WAVE = 64;
+constexpr int WAVE_CELLS = 128;  // levels this small run on one wavefront, whose passes need no block barrier
+
+// Division by a level extent through a multiply-high: m = ceil(2^32 / d) gives floor(t / d) exactly for
+// t, d < 2^16 (the error t (m d - 2^32) / (d 2^32) stays below 1 / d).
+struct FastDiv {
+    uint32_t d, m;
+    __device__ __forceinline__ int operator()(int t) const { return d == 1 ? t : (int)__umulhi((uint32_t)t, m); }
+};
+
+FastDiv fast_div(int d) {
+    FastDiv f;
+    f.d = (uint32_t)d;
+    f.m = d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d);
+    return f;
+}
 
 struct TailLevels {
-    int count;  // levels in the tail
-    int zero_top;  // the top level's q starts at 0 (a coarse level the V-cycle just cleared)
+    int count;      // levels in the tail
+    int zero_top;   // the top level's q starts at 0 (a coarse level the V-cycle just cleared)
+    int wave_from;  // first level (within the tail) from which every level has <= WAVE_CELLS cells; count if none
     LevelArgs level[MAX_TAIL];
     int offset[MAX_TAIL];  // float offset of the level's q in LDS; its rhs follows at offset + cells
     int cells[MAX_TAIL];
+    FastDiv div_n1[MAX_TAIL], div_n2[MAX_TAIL], div_half[MAX_TAIL];  // by n1, n2 and (n2 + 1) / 2
 };
 
 struct LdsQ {
@@ -468,45 +486,143 @@ struct LdsQ {
     __device__ float operator()(int i, int j, int k) const { return q[n.at(i, j, k)]; }
 };
 
-__device__ __forceinline__ void decode(int c, Dims n, int &i, int &j, int &k) {
-    k = c % n.n2;
-    const int r = c / n.n2;
-    j = r % n.n1;
-    i = r / n.n1;
+struct BlockSync {
+    __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+
+// One wavefront's lanes run in lockstep and its LDS operations complete in order, so between two passes of
+// a single wave it is enough that the compiler keeps its LDS accesses on their side of this point.
+struct WaveSync {
+    __device__ __forceinline__ void operator()() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// One tail level's geometry in scalar registers for the duration of its passes (read once from the staged
+// table: a per-neighbour LDS read of these would serialise every pass on LDS latency).
+struct TailLevel {
+    int n[3];
+    int open_lo[3], open_hi[3];
+    float fill[3];
+    float cell_m, ghost_gap;
+    int unit;
+    FastDiv div_n1, div_n2, div_half;
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <bool UNIT>
-__device__ __forceinline__ void tail_sweeps_t(const LevelArgs &L, float *q, const float *rhs, int sweeps, float relaxation) {
-    const Dims n = cell_dims(L.g);
-    const int count = n.count();
-    for (int s = 0; s < sweeps; s++) {
-        for (int color = 0; color < 2; color++) {
-            for (int c = threadIdx.x; c < count; c += TAIL_THREADS) {
-                int i, j, k;
-                decode(c, n, i, j, k);
-                if (((i + j + k) & 1) != color) continue;
-                float t0, t1;
-                neighbor_sum<UNIT>(L, i, j, k, LdsQ{q, n}, t0, t1);
-                q[c] = gs_update(L, i, j, k, q[c], rhs[c], t0, t1, relaxation);
+__device__ __forceinline__ TailLevel load_level(const TailLevels &t, int m) {
+    const LevelArgs &L = t.level[m];
+    TailLevel r;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r.n[a] = uniform(L.g.dims[a]);
+        r.open_lo[a] = uniform(L.g.open_lo[a]);
+        r.open_hi[a] = uniform(L.g.open_hi[a]);
+        r.fill[a] = uniform(L.g.fill_hi[a]);
+    }
+    r.cell_m = uniform(L.g.cell_m);
+    r.ghost_gap = uniform(L.g.ghost_gap);
+    r.unit = uniform(L.unit);
+    r.div_n1 = FastDiv{uniform(t.div_n1[m].d), uniform(t.div_n1[m].m)};
+    r.div_n2 = FastDiv{uniform(t.div_n2[m].d), uniform(t.div_n2[m].m)};
+    r.div_half = FastDiv{uniform(t.div_half[m].d), uniform(t.div_half[m].m)};
+    return r;
+}
+
+// cell_fill, sim/pressure.py:39-44
+__device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
+    return index == L.n[a] - 1 ? L.fill[a] : 1.0f;
+}
+
+// neighbor_term / neighbor_sum (sim/pressure.py:52-77) written out from the fills, in the reference's
+// operations; UNIT levels (every fill 1, ghost gap 0.5) skip the exact divisions by 1.
+template <bool UNIT, typename Q>
+__device__ __forceinline__ void neighbor_sum_direct(const TailLevel &L, int i, int j, int k, Q q, float &t0,
+                                                    float &t1) {
+    const int cell[3] = {i, j, k};
+    t0 = 0.0f;
+    t1 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        float area = 1.0f;
+        area = area * fill_at(L, b, cell[b]);
+        area = area * fill_at(L, c, cell[c]);
+        const float own = fill_at(L, a, cell[a]);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int step = s == 0 ? -1 : 1;
+            int nb[3] = {i, j, k};
+            nb[a] += step;
+            float v0 = 0.0f, v1 = 0.0f;
+            if (nb[a] >= 0 && nb[a] < L.n[a]) {
+                const float coupling = UNIT ? 1.0f : area / (0.5f * (own + fill_at(L, a, nb[a])));
+                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : coupling * q(nb[0], nb[1], nb[2]);
+                v1 = coupling;
+            } else if (step < 0 ? L.open_lo[a] == 1 : L.open_hi[a] == 1) {
+                v1 = UNIT ? 1.0f : area / (0.5f * own + L.ghost_gap);
             }
-            __syncthreads();
+            t0 = t0 + v0;
+            t1 = t1 + v1;
         }
     }
 }
 
-__device__ __forceinline__ void tail_sweeps(const LevelArgs &L, float *q, const float *rhs, int sweeps, float relaxation) {
-    if (L.unit)
-        tail_sweeps_t<true>(L, q, rhs, sweeps, relaxation);
-    else
-        tail_sweeps_t<false>(L, q, rhs, sweeps, relaxation);
+__device__ __forceinline__ float volume_direct(const TailLevel &L, int i, int j, int k) {
+    return fill_at(L, 0, i) * fill_at(L, 1, j) * fill_at(L, 2, k);
 }
 
-template <bool UNIT>
-__device__ __forceinline__ void tail_restrict_t(const LevelArgs &L, const float *q, const float *rhs, Dims nc, float *crhs, float *cq) {
-    const Dims nf = cell_dims(L.g);
-    for (int c = threadIdx.x; c < nc.count(); c += TAIL_THREADS) {
-        int i, j, k;
-        decode(c, nc, i, j, k);
+// `sweeps` red-black sweeps of one tail level (red_black_sweeps, sim/pressure.py:135-141): each colour pass
+// visits only that colour's cells, (n2 + 1) / 2 slots per row.
+template <bool UNIT, int NT, typename Sync>
+__device__ __forceinline__ void tail_sweeps_t(const TailLevel &L, float *q, const float *rhs, int sweeps,
+                                              float relaxation, int tid, Sync sync) {
+    const Dims n{L.n[0], L.n[1], L.n[2]};
+    const int half = (n.n2 + 1) / 2, total = n.n0 * n.n1 * half;
+    for (int s = 0; s < sweeps; s++) {
+        for (int color = 0; color < 2; color++) {
+            for (int x = tid; x < total; x += NT) {
+                const int row = L.div_half(x), slot = x - row * half;
+                const int i = L.div_n1(row), j = row - i * n.n1;
+                const int k = 2 * slot + ((color ^ (i + j)) & 1);
+                if (k >= n.n2) continue;
+                float t0, t1;
+                neighbor_sum_direct<UNIT>(L, i, j, k, LdsQ{q, n}, t0, t1);
+                if (t1 == 0.0f) continue;
+                const int c = n.at(i, j, k);
+                const float source = volume_direct(L, i, j, k) * L.cell_m * L.cell_m * rhs[c];
+                const float target = (t0 - source) / t1;
+                q[c] = q[c] + relaxation * (target - q[c]);
+            }
+            sync();
+        }
+    }
+}
+
+template <int NT, typename Sync>
+__device__ __forceinline__ void tail_sweeps(const TailLevel &L, float *q, const float *rhs, int sweeps,
+                                            float relaxation, int tid, Sync sync) {
+    if (L.unit)
+        tail_sweeps_t<true, NT>(L, q, rhs, sweeps, relaxation, tid, sync);
+    else
+        tail_sweeps_t<false, NT>(L, q, rhs, sweeps, relaxation, tid, sync);
+}
+
+// residual + restrict from level L to C, and C's q cleared (sim/multigrid.py:36-60, 139-143)
+template <bool UNIT, int NT, typename Sync>
+__device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailLevel &C, const float *q,
+                                                const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
+    const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
+    for (int c = tid; c < nc.count(); c += NT) {
+        const int r = C.div_n2(c), k = c - r * nc.n2;
+        const int i = C.div_n1(r), j = r - i * nc.n1;
         float total = 0.0f, count = 0.0f;
         for (int di = 0; di < 2; di++)
             for (int dj = 0; dj < 2; dj++)
@@ -514,15 +630,44 @@ __device__ __forceinline__ void tail_restrict_t(const LevelArgs &L, const float 
                     const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
                     if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
                         float t0, t1;
-                        neighbor_sum<UNIT>(L, fi, fj, fk, LdsQ{q, nf}, t0, t1);
+                        neighbor_sum_direct<UNIT>(L, fi, fj, fk, LdsQ{q, nf}, t0, t1);
                         const int f = nf.at(fi, fj, fk);
-                        total = total + residual_value(L, fi, fj, fk, q[f], rhs[f], t0, t1);
+                        const float scale = volume_direct(L, fi, fj, fk) * L.cell_m * L.cell_m;
+                        total = total + (rhs[f] - (t0 - t1 * q[f]) / scale);
                         count = count + 1.0f;
                     }
                 }
         crhs[c] = total / count;
         cq[c] = 0.0f;
     }
+    sync();
+}
+
+template <int NT, typename Sync>
+__device__ __forceinline__ void tail_restrict(const TailLevel &L, const TailLevel &C, const float *q,
+                                              const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
+    if (L.unit)
+        tail_restrict_t<true, NT>(L, C, q, rhs, crhs, cq, tid, sync);
+    else
+        tail_restrict_t<false, NT>(L, C, q, rhs, crhs, cq, tid, sync);
+}
+
+// prolong_add from level C into L (sim/multigrid.py:63-68, 145)
+template <int NT, typename Sync>
+__device__ __forceinline__ void tail_prolong(const TailLevel &L, const TailLevel &C, float *q, const float *cq,
+                                             int tid, Sync sync) {
+    const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
+    for (int c = tid; c < nf.count(); c += NT) {
+        const int r = L.div_n2(c), k = c - r * nf.n2;
+        const int i = L.div_n1(r), j = r - i * nf.n1;
+        q[c] = q[c] + prolongation(cq, nc, i, j, k);
+    }
+    sync();
+}
+
+__device__ __forceinline__ int coarsest_sweeps(const TailLevel &L) {
+    const int longest = max(L.n[0], max(L.n[1], L.n[2]));
+    return max(20, 4 * longest);  // COARSEST_SWEEPS, COARSEST_SWEEPS_PER_CELL (sim/multigrid.py:30-31)
 }
 
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
@@ -546,43 +691,44 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
         lds[t.offset[0] + t.cells[0] + c] = rhs_top[c];
     }
     __syncthreads();
-    const int last = t.count - 1;
-    // down: smooth, residual + restrict, zero the coarse q (sim/multigrid.py:137-143)
-    for (int l = 0; l < last; l++) {
-        const LevelArgs &L = t.level[l];
-        float *q = lds + t.offset[l];
-        float *rhs = q + t.cells[l];
-        tail_sweeps(L, q, rhs, sweeps, 1.0f);
-        float *cq = lds + t.offset[l + 1];
-        float *crhs = cq + t.cells[l + 1];
-        if (L.unit)
-            tail_restrict_t<true>(L, q, rhs, cell_dims(t.level[l + 1].g), crhs, cq);
-        else
-            tail_restrict_t<false>(L, q, rhs, cell_dims(t.level[l + 1].g), crhs, cq);
-        __syncthreads();
+    const int last = t.count - 1, tid = threadIdx.x;
+    const int block_levels = min(t.wave_from, last);
+    auto q_of = [&](int m) { return lds + t.offset[m]; };
+    auto rhs_of = [&](int m) { return lds + t.offset[m] + t.cells[m]; };
+    // down on the whole block: smooth, residual + restrict, coarse q = 0 (sim/multigrid.py:137-143)
+    for (int m = 0; m < block_levels; m++) {
+        const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+        tail_sweeps<TAIL_THREADS>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
+        tail_restrict<TAIL_THREADS>(L, C, q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid, BlockSync{});
     }
-    // coarsest: max(20, 4 * longest) sweeps at 1.5 (sim/multigrid.py:30-33, 132-136)
-    {
-        const LevelArgs &L = t.level[last];
-        const int longest = max(L.g.dims[0], max(L.g.dims[1], L.g.dims[2]));
-        float *q = lds + t.offset[last];
-        tail_sweeps(L, q, q + t.cells[last], max(20, 4 * longest), 1.5f);
-    }
-    // up: prolong-add, smooth (sim/multigrid.py:145-147)
-    for (int l = last - 1; l >= 0; l--) {
-        const LevelArgs &L = t.level[l];
-        const Dims nf = cell_dims(L.g), nc = cell_dims(t.level[l + 1].g);
-        float *q = lds + t.offset[l];
-        const float *cq = lds + t.offset[l + 1];
-        for (int c = threadIdx.x; c < t.cells[l]; c += TAIL_THREADS) {
-            int i, j, k;
-            decode(c, nf, i, j, k);
-            q[c] = q[c] + prolongation(cq, nc, i, j, k);
+    if (t.wave_from <= last) {
+        // the smallest levels on wavefront 0: down, the coarsest solve, and back up
+        if (tid < WAVE) {
+            for (int m = t.wave_from; m < last; m++) {
+                const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+                tail_sweeps<WAVE>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
+                tail_restrict<WAVE>(L, C, q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid, WaveSync{});
+            }
+            const TailLevel B = load_level(t, last);
+            tail_sweeps<WAVE>(B, q_of(last), rhs_of(last), coarsest_sweeps(B), 1.5f, tid, WaveSync{});
+            for (int m = last - 1; m >= t.wave_from; m--) {
+                const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+                tail_prolong<WAVE>(L, C, q_of(m), q_of(m + 1), tid, WaveSync{});
+                tail_sweeps<WAVE>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
+            }
         }
         __syncthreads();
-        tail_sweeps(L, q, q + t.cells[l], sweeps, 1.0f);
+    } else {
+        const TailLevel B = load_level(t, last);
+        tail_sweeps<TAIL_THREADS>(B, q_of(last), rhs_of(last), coarsest_sweeps(B), 1.5f, tid, BlockSync{});
     }
-    for (int c = threadIdx.x; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
+    // up on the whole block: prolong-add, smooth (sim/multigrid.py:145-147)
+    for (int m = block_levels - 1; m >= 0; m--) {
+        const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+        tail_prolong<TAIL_THREADS>(L, C, q_of(m), q_of(m + 1), tid, BlockSync{});
+        tail_sweeps<TAIL_THREADS>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
+    }
+    for (int c = tid; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
 }
 
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
@@ -1015,8 +1161,13 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
             h.tail.cells[m] = g.dims[0] * g.dims[1] * g.dims[2];
             h.tail.offset[m] = offset;
             offset += 2 * h.tail.cells[m];
+            h.tail.div_n1[m] = fast_div(g.dims[1]);
+            h.tail.div_n2[m] = fast_div(g.dims[2]);
+            h.tail.div_half[m] = fast_div((g.dims[2] + 1) / 2);
         }
         h.tail_bytes = sizeof(float) * (size_t)offset;
+        h.tail.wave_from = h.tail.count;
+        for (int m = h.tail.count - 1; m >= 0 && h.tail.cells[m] <= WAVE_CELLS; m--) h.tail.wave_from = m;
     }
 
     TimingScope t("multigrid", s);
