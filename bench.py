@@ -86,6 +86,38 @@ def time_kernel_family(run, family: str) -> tuple[float, int]:
     return (total.value / count.value if count.value else float("nan")), count.value
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "rb_sweep_tile_128x192x128_traffic.json"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the top-level sweep from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, each scaled
+    by what the counter reads for a streaming kernel of known size (the file records both runs)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    facts = json.loads(TRAFFIC_FILE.read_text())
+    return facts.get("hbm_bytes_per_launch")
+
+
+def copy_probe(mib: int, repeats: int = 20) -> float:
+    """A plain streaming kernel (dst = -src over `mib` MiB of FP32) `repeats` times; returns GB/s counted as
+    read + write bytes. Also the known-size kernel the HBM counters are scaled by."""
+    import torch
+
+    count = mib * (1 << 20) // 4
+    src = torch.ones(count, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        torch.neg(src, out=dst)
+    end.record()
+    end.synchronize()
+    seconds = start.elapsed_time(end) * 1e-3 / repeats
+    return 2.0 * count * 4 / seconds / 1e9
+
+
 def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
     """The FP32 oracle (scalar C port, OpenMP) on the same warm state for `steps` substeps."""
     import numpy as np
@@ -113,9 +145,13 @@ def main() -> int:
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=48, help="timed substeps (K)")
     parser.add_argument("--warmup", type=int, default=48, help="untimed substeps first (W); 48 = 12 frames")
-    parser.add_argument("--cpu-steps", type=int, default=2, help="substeps of the CPU baseline sample (0 = skip)")
+    parser.add_argument("--cpu-steps", type=int, default=32, help="substeps of the CPU baseline sample (0 = skip)")
     parser.add_argument("--no-render", action="store_true")
     parser.add_argument("--tolerance", type=float, default=1e-5)
+    parser.add_argument("--top-tile", type=int, default=-1, metavar="SHAPE",
+                        help="pin the sweep tile shape of the 128x192x128 level instead of autotuning it")
+    parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                        help="only run the streaming-copy probe over MIB MiB (HBM counter calibration) and exit")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -123,6 +159,13 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_probe > 0:
+        init_gpu = torch.cuda.is_available()
+        if not init_gpu:
+            raise SystemExit("the copy probe needs a GPU")
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "GB/s": copy_probe(args.copy_probe, repeats=4)}))
+        return 0
 
     from plume_amd.recipe.derived import derive
     from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
@@ -144,7 +187,12 @@ def main() -> int:
     print(f"[bench] grid {GRID}, dt {dt:.6f} s, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)
 
-    tuned = solve.autotune() if hasattr(solve, "autotune") else {}
+    tuned = solve.autotune()
+    if args.top_tile >= 0:
+        from plume_amd._lib import call
+
+        call("plm_set_sweep_tile", (ctypes.c_int32 * 3)(*GRID), args.top_tile)
+        tuned[GRID] = (args.top_tile, tuned.get(GRID, (None, []))[1])
     print(f"[bench] sweep tiles: {tuned}", file=sys.stderr, flush=True)
     step_index = 0
 
@@ -190,9 +238,11 @@ def main() -> int:
                                f"mean {mean_cycles:.2f} cycles",
                    "confinement": 0.3, "buoyancy": "alpha 0.1, beta 1.0", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(),
                      "kernel": "k_rb_sweep_tile (128x192x128 level, 12 B/cell)",
-                     "avg_launch_us": sweep_ms * 1e3, "launches": sweep_launches},
+                     "algorithmic_bytes_per_launch": sweep_bytes(cells),
+                     "avg_launch_us": sweep_ms * 1e3, "launches": sweep_launches,
+                     "copy_kernel_GBps": copy_probe(1024)},
         "substep_roofline": {"bytes_per_substep": substep_bytes(cells, mean_cycles), "achieved": substep_gbps,
                              "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": substep_gbps / HBM_PEAK_GBPS,
                              "mean_cycles": mean_cycles},

@@ -138,31 +138,58 @@ LevelArgs level_args(const plm_grid_info &g) {
     return L;
 }
 
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// One of four level constants by two lane predicates. Each value passes through readfirstlane so that the
+// selects stay selects of scalar registers (as selects of loads they become one per-lane indexed load).
+__device__ __forceinline__ float sel4(float v0, float v1, float v2, float v3, bool last_b, bool last_c) {
+    v0 = uniform(v0);
+    v1 = uniform(v1);
+    v2 = uniform(v2);
+    v3 = uniform(v3);
+    return last_c ? (last_b ? v3 : v2) : (last_b ? v1 : v0);
+}
+
+// Coupling.in[a][area][pair_last] and Coupling.ghost[a][area][own_last], area = last_b + 2 last_c
+__device__ __forceinline__ float coupling_in(const Coupling &cp, int a, bool last_b, bool last_c, bool pair_last) {
+    const float x0 = sel4(cp.in[a][0][0], cp.in[a][1][0], cp.in[a][2][0], cp.in[a][3][0], last_b, last_c);
+    const float x1 = sel4(cp.in[a][0][1], cp.in[a][1][1], cp.in[a][2][1], cp.in[a][3][1], last_b, last_c);
+    return pair_last ? x1 : x0;
+}
+__device__ __forceinline__ float coupling_ghost(const Coupling &cp, int a, bool last_b, bool last_c, bool own_last) {
+    const float x0 = sel4(cp.ghost[a][0][0], cp.ghost[a][1][0], cp.ghost[a][2][0], cp.ghost[a][3][0], last_b, last_c);
+    const float x1 = sel4(cp.ghost[a][0][1], cp.ghost[a][1][1], cp.ghost[a][2][1], cp.ghost[a][3][1], last_b, last_c);
+    return own_last ? x1 : x0;
+}
+
 // neighbor_sum, sim/pressure.py:72-77: (sum of coupling * q over the six neighbours, sum of couplings), added
 // in the reference's order (x-, x+, y-, y+, z-, z+), ghosts and closed faces adding 0 to the first.
-// UNIT levels skip the multiplications by an exact 1.
+// UNIT levels skip the multiplications by an exact 1. q is read only inside the domain.
 template <bool UNIT, typename Q>
 __device__ __forceinline__ void neighbor_sum(const LevelArgs &L, int i, int j, int k, Q q, float &t0, float &t1) {
     const plm_grid_info &g = L.g;
     const int cell[3] = {i, j, k};
-    const int last[3] = {i == g.dims[0] - 1, j == g.dims[1] - 1, k == g.dims[2] - 1};
+    const bool last[3] = {i == g.dims[0] - 1, j == g.dims[1] - 1, k == g.dims[2] - 1};
     t0 = 0.0f;
     t1 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-        const int area = last[b] + 2 * last[c];
         // step -1
         {
             int nb[3] = {i, j, k};
             nb[a] -= 1;
             float v0 = 0.0f, v1 = 0.0f;
             if (nb[a] >= 0) {
-                const float cp = UNIT ? 1.0f : L.cp.in[a][area][last[a]];
+                const float cp = UNIT ? 1.0f : coupling_in(L.cp, a, last[b], last[c], last[a]);
                 v0 = UNIT ? q(nb[0], nb[1], nb[2]) : cp * q(nb[0], nb[1], nb[2]);
                 v1 = cp;
             } else if (g.open_lo[a] == 1) {
-                v1 = UNIT ? 1.0f : L.cp.ghost[a][area][last[a]];
+                v1 = UNIT ? 1.0f : coupling_ghost(L.cp, a, last[b], last[c], last[a]);
             }
             t0 = t0 + v0;
             t1 = t1 + v1;
@@ -173,11 +200,11 @@ __device__ __forceinline__ void neighbor_sum(const LevelArgs &L, int i, int j, i
             nb[a] += 1;
             float v0 = 0.0f, v1 = 0.0f;
             if (nb[a] < g.dims[a]) {
-                const float cp = UNIT ? 1.0f : L.cp.in[a][area][cell[a] + 1 == g.dims[a] - 1];
+                const float cp = UNIT ? 1.0f : coupling_in(L.cp, a, last[b], last[c], cell[a] + 1 == g.dims[a] - 1);
                 v0 = UNIT ? q(nb[0], nb[1], nb[2]) : cp * q(nb[0], nb[1], nb[2]);
                 v1 = cp;
             } else if (g.open_hi[a] == 1) {
-                v1 = UNIT ? 1.0f : L.cp.ghost[a][area][1];
+                v1 = UNIT ? 1.0f : coupling_ghost(L.cp, a, last[b], last[c], true);
             }
             t0 = t0 + v0;
             t1 = t1 + v1;
@@ -185,9 +212,58 @@ __device__ __forceinline__ void neighbor_sum(const LevelArgs &L, int i, int j, i
     }
 }
 
+// neighbor_sum for a q that reads +0.0 outside the domain (an LDS tile whose halo holds 0 there): the same
+// values in the same order, without branches. An outside neighbour's term is coupling * (+0.0) = +0.0 (or
+// +0.0 itself on UNIT levels), what the reference adds for a ghost or a closed face.
+template <bool UNIT, typename Q>
+__device__ __forceinline__ void neighbor_sum_padded(const LevelArgs &L, int i, int j, int k, Q q, float &t0,
+                                                    float &t1) {
+    const plm_grid_info &g = L.g;
+    const int cell[3] = {i, j, k};
+    const bool last[3] = {i == g.dims[0] - 1, j == g.dims[1] - 1, k == g.dims[2] - 1};
+    t0 = 0.0f;
+    t1 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const bool open_lo = uniform(g.open_lo[a]) == 1, open_hi = uniform(g.open_hi[a]) == 1;
+        {
+            int nb[3] = {i, j, k};
+            nb[a] -= 1;
+            const bool in = cell[a] > 0;
+            const float qn = q(nb[0], nb[1], nb[2]);
+            if (UNIT) {
+                t0 = t0 + qn;
+                t1 = t1 + ((in || open_lo) ? 1.0f : 0.0f);
+            } else {
+                const float cp = coupling_in(L.cp, a, last[b], last[c], last[a]);
+                t0 = t0 + cp * qn;
+                t1 = t1 + (in ? cp : open_lo ? coupling_ghost(L.cp, a, last[b], last[c], last[a]) : 0.0f);
+            }
+        }
+        {
+            int nb[3] = {i, j, k};
+            nb[a] += 1;
+            const bool in = !last[a];
+            const float qn = q(nb[0], nb[1], nb[2]);
+            if (UNIT) {
+                t0 = t0 + qn;
+                t1 = t1 + ((in || open_hi) ? 1.0f : 0.0f);
+            } else {
+                const float cp = coupling_in(L.cp, a, last[b], last[c], cell[a] + 1 == g.dims[a] - 1);
+                t0 = t0 + cp * qn;
+                t1 = t1 + (in ? cp : open_hi ? coupling_ghost(L.cp, a, last[b], last[c], true) : 0.0f);
+            }
+        }
+    }
+}
+
+// cell_volume, sim/pressure.py:47-49: (F_0 F_1) F_2 with F = fill_hi on the last cell of an axis, else 1
 __device__ __forceinline__ float cell_volume(const LevelArgs &L, int i, int j, int k) {
     const plm_grid_info &g = L.g;
-    return L.cp.vol[(i == g.dims[0] - 1) + 2 * (j == g.dims[1] - 1) + 4 * (k == g.dims[2] - 1)];
+    float v = (i == g.dims[0] - 1 ? uniform(g.fill_hi[0]) : 1.0f) * (j == g.dims[1] - 1 ? uniform(g.fill_hi[1]) : 1.0f);
+    v = v * (k == g.dims[2] - 1 ? uniform(g.fill_hi[2]) : 1.0f);
+    return v;
 }
 
 // One red_black_sweep_kernel update (sim/pressure.py:86-92); q unchanged when nothing couples
@@ -239,58 +315,132 @@ __device__ __forceinline__ float prolongation(const float *__restrict__ coarse, 
     return sample_index(coarse, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f, (float)k * 0.5f - 0.25f);
 }
 
-// One two-colour sweep over a TI x TJ x 64 tile (i, j, k) with THREADS threads; LDS holds the tile plus a
-// 2-cell halo, (TI + 4) x (TJ + 4) x 68 floats.
+// Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD with its own L2. Renumbering so
+// that XCD x runs one contiguous range of tiles keeps a tile's halo neighbours in the same L2.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_contiguous(int block, int blocks) {
+    const int per = blocks / XCDS, rem = blocks % XCDS, x = block % XCDS;
+    return x * per + min(x, rem) + block / XCDS;
+}
+
+// One two-colour sweep over a TI x TJ x 64 tile (i, j, k) with THREADS threads, row by row: a row is the
+// tile's 64 cells along k plus a 2-cell halo each side, 68 floats in LDS stored colour-split (even k - k0 + 2
+// in the first 34 slots, odd in the last 34), so the cells of one colour and their k neighbours sit in
+// consecutive LDS words. LDS holds (TI + 4) x (TJ + 4) rows. Loads: the 64 interior cells of a row are one
+// aligned 256-byte access by one wavefront, the 4 halo cells a small second loop. Red runs on the tile and
+// a 1-cell ring (the ring's new values are what black needs): 32 cells per row on half a wavefront, plus the
+// one red cell of each ring row that lies in the k halo; black runs on the tile's rows; then the rows are
+// written out. The rhs values both passes need are loaded into registers before the first barrier.
 template <int TI, int TJ, int THREADS, bool UNIT, int START>
 __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const float *__restrict__ q_in,
                                                            float *__restrict__ q_out, const float *__restrict__ rhs,
                                                            const float *__restrict__ coarse, Dims nc,
-                                                           float relaxation, const double *gate) {
-    constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH;
+                                                           float relaxation, const double *gate, int tiles_k,
+                                                           int tiles_j) {
+    constexpr int WAVES = THREADS / 64;
+    constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
+    constexpr int RI = TI + 2, RJ = TJ + 2;
+    constexpr int RED_PAIRS = (RI * RJ + 1) / 2, RED_ITERS = (RED_PAIRS + WAVES - 1) / WAVES;
+    constexpr int BLACK_ITERS = TI * TJ / 2 / WAVES;
+    static_assert(THREADS % 64 == 0 && (TI * TJ) % (2 * WAVES) == 0, "tile rows split evenly over wavefronts");
+    static_assert(RI * RJ <= THREADS, "one thread per ring row for the k-halo red cells");
     if (gated_off(gate)) return;
     __shared__ float s[LI * LJ * LK];
     const Dims n = cell_dims(L.g);
-    const int k0 = blockIdx.x * TK, j0 = blockIdx.y * TJ, i0 = blockIdx.z * TI;
-    for (int t = threadIdx.x; t < LI * LJ * LK; t += THREADS) {
-        const int kk = t % LK, r = t / LK, jj = r % LJ, ii = r / LJ;
-        const int i = i0 - TH + ii, j = j0 - TH + jj, k = k0 - TH + kk;
+    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int tk = tile % tiles_k, tr = tile / tiles_k, tj = tr % tiles_j, ti = tr / tiles_j;
+    const int k0 = tk * TK, j0 = tj * TJ, i0 = ti * TI;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int half_lane = lane & 31, upper = lane >> 5;
+    // slot of k in a row: kk = k - k0 + 2 in [0, 68), even kk first
+    auto slot = [&](int k) {
+        const int kk = k - k0 + TH;
+        return (kk & 1) * HALF + (kk >> 1);
+    };
+    auto at = [&](int i, int j, int k) -> float & {
+        return s[((i - i0 + TH) * LJ + (j - j0 + TH)) * LK + slot(k)];
+    };
+    auto inside = [&](int i, int j, int k) {
+        return i >= 0 && i < n.n0 && j >= 0 && j < n.n1 && k >= 0 && k < n.n2;
+    };
+    // red cell of ring row `row` (ii, jj over RI x RJ) at k0 + 2 m + ((i + j) & 1): (i + j + k) even
+    auto red_main = [&](int row, int m, int &i, int &j, int &k) {
+        i = i0 - 1 + row / RJ;
+        j = j0 - 1 + row % RJ;
+        k = k0 + 2 * m + ((i + j) & 1);
+        return row < RI * RJ && inside(i, j, k);
+    };
+    // the red cell of a ring row in the k halo: k0 - 1 when i + j is odd, else k0 + 64
+    auto red_edge = [&](int row, int &i, int &j, int &k) {
+        i = i0 - 1 + row / RJ;
+        j = j0 - 1 + row % RJ;
+        k = ((i + j) & 1) ? k0 - 1 : k0 + TK;
+        return row < RI * RJ && inside(i, j, k);
+    };
+    auto black_cell = [&](int row, int m, int &i, int &j, int &k) {
+        i = i0 + row / TJ;
+        j = j0 + row % TJ;
+        k = k0 + 2 * m + ((i + j + 1) & 1);
+        return i < n.n0 && j < n.n1 && k < n.n2;
+    };
+    float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS], rhs_edge = 0.0f;
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++) {
+        int i, j, k;
+        rhs_red[u] = red_main(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? rhs[n.at(i, j, k)] : 0.0f;
+    }
+    {
+        int i, j, k;
+        if (red_edge(threadIdx.x, i, j, k)) rhs_edge = rhs[n.at(i, j, k)];
+    }
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++) {
+        int i, j, k;
+        rhs_black[u] = black_cell(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? rhs[n.at(i, j, k)] : 0.0f;
+    }
+    auto start_value = [&](int i, int j, int k) {
         float value = 0.0f;
-        if (i >= 0 && i < n.n0 && j >= 0 && j < n.n1 && k >= 0 && k < n.n2) {
+        if (inside(i, j, k)) {
             if (START == FROM_Q) value = q_in[n.at(i, j, k)];
             if (START == FROM_PROLONG) value = q_in[n.at(i, j, k)] + prolongation(coarse, nc, i, j, k);
         }
-        s[t] = value;
-    }
-    __syncthreads();
-    auto at = [&](int i, int j, int k) -> float & {
-        return s[((i - i0 + TH) * LJ + (j - j0 + TH)) * LK + (k - k0 + TH)];
+        return value;
     };
-    // red (i + j + k even) on the tile and a 1-cell ring: 33 red cells in each 66-cell row
-    constexpr int RI = TI + 2, RJ = TJ + 2, RROW = (TK + 2) / 2;
-    for (int t = threadIdx.x; t < RI * RJ * RROW; t += THREADS) {
-        const int kr = t % RROW, row = t / RROW, jj = row % RJ, ii = row / RJ;
-        const int i = i0 - 1 + ii, j = j0 - 1 + jj;
-        const int k = k0 - 1 + 2 * kr + ((i + j + k0 - 1) & 1);
-        if (i < 0 || i >= n.n0 || j < 0 || j >= n.n1 || k < 0 || k >= n.n2) continue;
-        float t0, t1;
-        neighbor_sum<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
-        float &cell = at(i, j, k);
-        cell = gs_update(L, i, j, k, cell, rhs[n.at(i, j, k)], t0, t1, relaxation);
+    for (int row = wave; row < LI * LJ; row += WAVES) {
+        const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ, k = k0 + lane;
+        s[row * LK + slot(k)] = start_value(i, j, k);
+    }
+    for (int t = threadIdx.x; t < LI * LJ * 4; t += THREADS) {
+        const int row = t >> 2, h = t & 3;
+        const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ, k = h < 2 ? k0 - TH + h : k0 + TK - 2 + h;
+        s[row * LK + slot(k)] = start_value(i, j, k);
     }
     __syncthreads();
-    // black on the tile; every tile cell is written
-    for (int t = threadIdx.x; t < TI * TJ * TK; t += THREADS) {
-        const int kk = t % TK, r = t / TK, jj = r % TJ, ii = r / TJ;
-        const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
-        if (i >= n.n0 || j >= n.n1 || k >= n.n2) continue;
-        float value = at(i, j, k);
-        const int c = n.at(i, j, k);
-        if (((i + j + k) & 1) == 1) {
-            float t0, t1;
-            neighbor_sum<UNIT>(L, i, j, k, [&](int a, int b, int cc) { return at(a, b, cc); }, t0, t1);
-            value = gs_update(L, i, j, k, value, rhs[c], t0, t1, relaxation);
-        }
-        q_out[c] = value;
+    auto update = [&](int i, int j, int k, float rhs_c) {
+        float t0, t1;
+        neighbor_sum_padded<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
+        float &cell = at(i, j, k);
+        cell = gs_update(L, i, j, k, cell, rhs_c, t0, t1, relaxation);
+    };
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++) {
+        int i, j, k;
+        if (red_main(2 * (wave + u * WAVES) + upper, half_lane, i, j, k)) update(i, j, k, rhs_red[u]);
+    }
+    {
+        int i, j, k;
+        if (red_edge(threadIdx.x, i, j, k)) update(i, j, k, rhs_edge);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++) {
+        int i, j, k;
+        if (black_cell(2 * (wave + u * WAVES) + upper, half_lane, i, j, k)) update(i, j, k, rhs_black[u]);
+    }
+    __syncthreads();
+    for (int row = wave; row < TI * TJ; row += WAVES) {
+        const int i = i0 + row / TJ, j = j0 + row % TJ, k = k0 + lane;
+        if (i < n.n0 && j < n.n1 && k < n.n2) q_out[n.at(i, j, k)] = at(i, j, k);
     }
 }
 
@@ -304,14 +454,15 @@ constexpr int DEFAULT_TILE_SHAPE = 0;
 
 template <bool UNIT, int START>
 void launch_tile_shape(int shape, dim3 grid, hipStream_t s, const LevelArgs &L, const float *src, float *dst,
-                       const float *rhs, const float *coarse, Dims nc, float relaxation, const double *gate) {
+                       const float *rhs, const float *coarse, Dims nc, float relaxation, const double *gate,
+                       int tiles_k, int tiles_j) {
     switch (shape) {
-    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
-    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
-    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
-    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
-    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
-    default: k_rb_sweep_tile<4, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate); break;
+    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    default: k_rb_sweep_tile<4, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
     }
 }
 
@@ -506,15 +657,29 @@ struct TailLevel {
     int n[3];
     int open_lo[3], open_hi[3];
     float fill[3];
-    float cell_m, ghost_gap;
+    float cell_m;
     int unit;
     FastDiv div_n1, div_n2, div_half;
+    // Coupling.in[a][area][1] and Coupling.ghost[a][area][*]: the couplings that are not the area itself,
+    // selected per lane instead of divided out per neighbour
+    float in1[3][4];
+    float ghost[3][2][4];  // [axis][this cell is the last][area]
 };
 
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-__device__ __forceinline__ float uniform(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+
+// A level's extents alone, for the other level of a restrict or prolong pass
+struct TailDims {
+    int n[3];
+    FastDiv div_n1, div_n2;
+};
+
+__device__ __forceinline__ TailDims load_dims(const TailLevels &t, int m) {
+    TailDims r;
+#pragma unroll
+    for (int a = 0; a < 3; a++) r.n[a] = uniform(t.level[m].g.dims[a]);
+    r.div_n1 = FastDiv{uniform(t.div_n1[m].d), uniform(t.div_n1[m].m)};
+    r.div_n2 = FastDiv{uniform(t.div_n2[m].d), uniform(t.div_n2[m].m)};
+    return r;
 }
 
 __device__ __forceinline__ TailLevel load_level(const TailLevels &t, int m) {
@@ -526,9 +691,14 @@ __device__ __forceinline__ TailLevel load_level(const TailLevels &t, int m) {
         r.open_lo[a] = uniform(L.g.open_lo[a]);
         r.open_hi[a] = uniform(L.g.open_hi[a]);
         r.fill[a] = uniform(L.g.fill_hi[a]);
+#pragma unroll
+        for (int area = 0; area < 4; area++) {
+            r.in1[a][area] = uniform(L.cp.in[a][area][1]);
+            r.ghost[a][0][area] = uniform(L.cp.ghost[a][area][0]);
+            r.ghost[a][1][area] = uniform(L.cp.ghost[a][area][1]);
+        }
     }
     r.cell_m = uniform(L.g.cell_m);
-    r.ghost_gap = uniform(L.g.ghost_gap);
     r.unit = uniform(L.unit);
     r.div_n1 = FastDiv{uniform(t.div_n1[m].d), uniform(t.div_n1[m].m)};
     r.div_n2 = FastDiv{uniform(t.div_n2[m].d), uniform(t.div_n2[m].m)};
@@ -541,33 +711,60 @@ __device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
     return index == L.n[a] - 1 ? L.fill[a] : 1.0f;
 }
 
-// neighbor_term / neighbor_sum (sim/pressure.py:52-77) written out from the fills, in the reference's
-// operations; UNIT levels (every fill 1, ghost gap 0.5) skip the exact divisions by 1.
+// One of four level constants by two lane predicates. Each value passes through readfirstlane so that the
+// selects stay selects of registers (as selects of loads they become one indexed load of a stack copy).
+__device__ __forceinline__ float pick4(const float (&v)[4], bool last_b, bool last_c) {
+    const float v0 = uniform(v[0]), v1 = uniform(v[1]), v2 = uniform(v[2]), v3 = uniform(v[3]);
+    return last_c ? (last_b ? v3 : v2) : (last_b ? v1 : v0);
+}
+
+// neighbor_term / neighbor_sum (sim/pressure.py:52-77) on a tail level. The area (1 F_b) F_c is formed as
+// the reference forms it; an in-domain neighbour's coupling is area / (0.5 (own + its fill)), which is the
+// area itself when neither cell is the last on the axis (the divisor is exactly 1) and the level's
+// precomputed in1 otherwise; an open-face ghost's is the precomputed ghost value. UNIT levels (every
+// fill 1, ghost gap 0.5) skip the multiplications by an exact 1.
 template <bool UNIT, typename Q>
 __device__ __forceinline__ void neighbor_sum_direct(const TailLevel &L, int i, int j, int k, Q q, float &t0,
                                                     float &t1) {
-    const int cell[3] = {i, j, k};
+    const bool last[3] = {i == L.n[0] - 1, j == L.n[1] - 1, k == L.n[2] - 1};
     t0 = 0.0f;
     t1 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-        float area = 1.0f;
-        area = area * fill_at(L, b, cell[b]);
-        area = area * fill_at(L, c, cell[c]);
-        const float own = fill_at(L, a, cell[a]);
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int step = s == 0 ? -1 : 1;
+        float area = 1.0f, in1 = 1.0f;
+        if (!UNIT) {
+            area = area * (last[b] ? L.fill[b] : 1.0f);
+            area = area * (last[c] ? L.fill[c] : 1.0f);
+            in1 = pick4(L.in1[a], last[b], last[c]);
+        }
+        // step -1: the neighbour is never the last cell, so the pair involves a last cell iff this one is
+        {
             int nb[3] = {i, j, k};
-            nb[a] += step;
+            nb[a] -= 1;
             float v0 = 0.0f, v1 = 0.0f;
-            if (nb[a] >= 0 && nb[a] < L.n[a]) {
-                const float coupling = UNIT ? 1.0f : area / (0.5f * (own + fill_at(L, a, nb[a])));
+            if (nb[a] >= 0) {
+                const float coupling = UNIT ? 1.0f : (last[a] ? in1 : area);
                 v0 = UNIT ? q(nb[0], nb[1], nb[2]) : coupling * q(nb[0], nb[1], nb[2]);
                 v1 = coupling;
-            } else if (step < 0 ? L.open_lo[a] == 1 : L.open_hi[a] == 1) {
-                v1 = UNIT ? 1.0f : area / (0.5f * own + L.ghost_gap);
+            } else if (L.open_lo[a] == 1) {
+                v1 = UNIT ? 1.0f : (last[a] ? pick4(L.ghost[a][1], last[b], last[c])
+                                            : pick4(L.ghost[a][0], last[b], last[c]));
+            }
+            t0 = t0 + v0;
+            t1 = t1 + v1;
+        }
+        // step +1: this cell is not the last; the neighbour is iff it sits at n - 1
+        {
+            int nb[3] = {i, j, k};
+            nb[a] += 1;
+            float v0 = 0.0f, v1 = 0.0f;
+            if (nb[a] < L.n[a]) {
+                const float coupling = UNIT ? 1.0f : (nb[a] == L.n[a] - 1 ? in1 : area);
+                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : coupling * q(nb[0], nb[1], nb[2]);
+                v1 = coupling;
+            } else if (L.open_hi[a] == 1) {
+                v1 = UNIT ? 1.0f : pick4(L.ghost[a][1], last[b], last[c]);
             }
             t0 = t0 + v0;
             t1 = t1 + v1;
@@ -617,7 +814,7 @@ __device__ __forceinline__ void tail_sweeps(const TailLevel &L, float *q, const 
 
 // residual + restrict from level L to C, and C's q cleared (sim/multigrid.py:36-60, 139-143)
 template <bool UNIT, int NT, typename Sync>
-__device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailLevel &C, const float *q,
+__device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailDims &C, const float *q,
                                                 const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
     const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
     for (int c = tid; c < nc.count(); c += NT) {
@@ -644,7 +841,7 @@ __device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailLe
 }
 
 template <int NT, typename Sync>
-__device__ __forceinline__ void tail_restrict(const TailLevel &L, const TailLevel &C, const float *q,
+__device__ __forceinline__ void tail_restrict(const TailLevel &L, const TailDims &C, const float *q,
                                               const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
     if (L.unit)
         tail_restrict_t<true, NT>(L, C, q, rhs, crhs, cq, tid, sync);
@@ -654,7 +851,7 @@ __device__ __forceinline__ void tail_restrict(const TailLevel &L, const TailLeve
 
 // prolong_add from level C into L (sim/multigrid.py:63-68, 145)
 template <int NT, typename Sync>
-__device__ __forceinline__ void tail_prolong(const TailLevel &L, const TailLevel &C, float *q, const float *cq,
+__device__ __forceinline__ void tail_prolong(const TailDims &L, const TailDims &C, float *q, const float *cq,
                                              int tid, Sync sync) {
     const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
     for (int c = tid; c < nf.count(); c += NT) {
@@ -697,24 +894,25 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
     auto rhs_of = [&](int m) { return lds + t.offset[m] + t.cells[m]; };
     // down on the whole block: smooth, residual + restrict, coarse q = 0 (sim/multigrid.py:137-143)
     for (int m = 0; m < block_levels; m++) {
-        const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+        const TailLevel L = load_level(t, m);
         tail_sweeps<TAIL_THREADS>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
-        tail_restrict<TAIL_THREADS>(L, C, q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid, BlockSync{});
+        tail_restrict<TAIL_THREADS>(L, load_dims(t, m + 1), q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid,
+                                    BlockSync{});
     }
     if (t.wave_from <= last) {
         // the smallest levels on wavefront 0: down, the coarsest solve, and back up
         if (tid < WAVE) {
             for (int m = t.wave_from; m < last; m++) {
-                const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
+                const TailLevel L = load_level(t, m);
                 tail_sweeps<WAVE>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
-                tail_restrict<WAVE>(L, C, q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid, WaveSync{});
+                tail_restrict<WAVE>(L, load_dims(t, m + 1), q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid,
+                                    WaveSync{});
             }
             const TailLevel B = load_level(t, last);
             tail_sweeps<WAVE>(B, q_of(last), rhs_of(last), coarsest_sweeps(B), 1.5f, tid, WaveSync{});
             for (int m = last - 1; m >= t.wave_from; m--) {
-                const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
-                tail_prolong<WAVE>(L, C, q_of(m), q_of(m + 1), tid, WaveSync{});
-                tail_sweeps<WAVE>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
+                tail_prolong<WAVE>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, WaveSync{});
+                tail_sweeps<WAVE>(load_level(t, m), q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
             }
         }
         __syncthreads();
@@ -724,9 +922,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
     }
     // up on the whole block: prolong-add, smooth (sim/multigrid.py:145-147)
     for (int m = block_levels - 1; m >= 0; m--) {
-        const TailLevel L = load_level(t, m), C = load_level(t, m + 1);
-        tail_prolong<TAIL_THREADS>(L, C, q_of(m), q_of(m + 1), tid, BlockSync{});
-        tail_sweeps<TAIL_THREADS>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
+        tail_prolong<TAIL_THREADS>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, BlockSync{});
+        tail_sweeps<TAIL_THREADS>(load_level(t, m), q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
     }
     for (int c = tid; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
 }
@@ -835,14 +1032,20 @@ int launch_tile(const LevelArgs &L, int start, const float *src, float *dst, con
     const Dims n = cell_dims(L.g);
     if (shape < 0) shape = tile_shape_for(n);
     const TileShape &ts = TILE_SHAPES[shape];
-    const dim3 grid((n.n2 + TK - 1) / TK, (n.n1 + ts.tj - 1) / ts.tj, (n.n0 + ts.ti - 1) / ts.ti);
-    TimingScope launch_timing(UNIT ? "rb_sweep_tile_top" : "rb_sweep_tile", s);
+    const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + ts.tj - 1) / ts.tj, tiles_i = (n.n0 + ts.ti - 1) / ts.ti;
+    const dim3 grid((unsigned)(tiles_k * tiles_j * tiles_i));
+    // family "rb_sweep_tile_top": the plain two-colour sweep (q in, q out) on the simulation grid
+    TimingScope launch_timing(!UNIT ? "rb_sweep_tile" : start == FROM_Q ? "rb_sweep_tile_top" : "rb_sweep_tile_top_start",
+                              s);
     if (start == FROM_ZERO)
-        launch_tile_shape<UNIT, FROM_ZERO>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+        launch_tile_shape<UNIT, FROM_ZERO>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+                                               tiles_j);
     else if (start == FROM_PROLONG)
-        launch_tile_shape<UNIT, FROM_PROLONG>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+        launch_tile_shape<UNIT, FROM_PROLONG>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+                                               tiles_j);
     else
-        launch_tile_shape<UNIT, FROM_Q>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate);
+        launch_tile_shape<UNIT, FROM_Q>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+                                               tiles_j);
     return check_launch("rb sweep tile");
 }
 
