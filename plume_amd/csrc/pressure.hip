@@ -606,6 +606,9 @@ constexpr int TAIL_THREADS = 1024;
 constexpr int MAX_TAIL = 8;
 constexpr int WAVE = 64;
 constexpr int WAVE_CELLS = 128;  // levels this small run on one wavefront, whose passes need no block barrier
+// cells per colour per thread on block levels, with and without stored couplings (a level with more colour
+// slots than that times TAIL_THREADS stays out of the tail)
+constexpr int BLOCK_PLAN = 2, BLOCK_PLAN_ONES = 3;
 
 // Division by a level extent through a multiply-high: m = ceil(2^32 / d) gives floor(t / d) exactly for
 // t, d < 2^16 (the error t (m d - 2^32) / (d 2^32) stays below 1 / d).
@@ -613,6 +616,9 @@ struct FastDiv {
     uint32_t d, m;
     __device__ __forceinline__ int operator()(int t) const { return d == 1 ? t : (int)__umulhi((uint32_t)t, m); }
 };
+
+// Slots of the colour enumeration on a level: n0 n1 ceil(n2 / 2); a tail thread plans a fixed number of them
+int colour_slots(const plm_grid_info &g) { return g.dims[0] * g.dims[1] * ((g.dims[2] + 1) / 2); }
 
 FastDiv fast_div(int d) {
     FastDiv f;
@@ -626,6 +632,7 @@ struct TailLevels {
     int zero_top;   // the top level's q starts at 0 (a coarse level the V-cycle just cleared)
     int wave_from;  // first level (within the tail) from which every level has <= WAVE_CELLS cells; count if none
     LevelArgs level[MAX_TAIL];
+    int res_offset;  // float offset in LDS of the residual of the level being restricted
     int offset[MAX_TAIL];  // float offset of the level's q in LDS; its rhs follows at offset + cells
     int cells[MAX_TAIL];
     FastDiv div_n1[MAX_TAIL], div_n2[MAX_TAIL], div_half[MAX_TAIL];  // by n1, n2 and (n2 + 1) / 2
@@ -711,112 +718,154 @@ __device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
     return index == L.n[a] - 1 ? L.fill[a] : 1.0f;
 }
 
-// One of four level constants by two lane predicates. Each value passes through readfirstlane so that the
-// selects stay selects of registers (as selects of loads they become one indexed load of a stack copy).
 __device__ __forceinline__ float pick4(const float (&v)[4], bool last_b, bool last_c) {
-    const float v0 = uniform(v[0]), v1 = uniform(v[1]), v2 = uniform(v[2]), v3 = uniform(v[3]);
-    return last_c ? (last_b ? v3 : v2) : (last_b ? v1 : v0);
+    return sel4(v[0], v[1], v[2], v[3], last_b, last_c);
 }
 
-// neighbor_term / neighbor_sum (sim/pressure.py:52-77) on a tail level. The area (1 F_b) F_c is formed as
-// the reference forms it; an in-domain neighbour's coupling is area / (0.5 (own + its fill)), which is the
-// area itself when neither cell is the last on the axis (the divisor is exactly 1) and the level's
-// precomputed in1 otherwise; an open-face ghost's is the precomputed ghost value. UNIT levels (every
-// fill 1, ghost gap 0.5) skip the multiplications by an exact 1.
-template <bool UNIT, typename Q>
-__device__ __forceinline__ void neighbor_sum_direct(const TailLevel &L, int i, int j, int k, Q q, float &t0,
-                                                    float &t1) {
+// LDS word 0 holds +0.0: the q a pass reads for a neighbour outside the level (a ghost or a closed face,
+// whose term the reference adds as 0).
+constexpr int LDS_ZERO = 0;
+constexpr int LDS_HEAD = 64;  // floats before the first level
+
+// What one tail pass needs of one cell, fixed for a visit of its level (neighbor_sum's couplings and their
+// sum, sim/pressure.py:52-77; cell_volume * cell_m^2, :47-49 and :86-92): `packed` = cell index | bit 16 + s
+// set when neighbour s (x-, x+, y-, y+, z-, z+) is inside the level, or -1 for no cell. ONES: every fill_hi
+// of the level is 1, so every in-domain coupling is area / (0.5 (1 + 1)) = 1 exactly and only t1 (through
+// the open-face ghosts) differs from the simulation grid's; the couplings are then not stored.
+template <bool ONES>
+struct PlanCell {
+    int packed;
+    float t1, scale;
+    float cp[ONES ? 1 : 6];
+};
+
+// Builds a cell's plan from the level's geometry and coupling table, with the reference's operations: the
+// area (1 F_b) F_c, the in-domain coupling (the area itself unless the pair holds the axis' last cell, then
+// the precomputed in1), the open-face ghost coupling, t1 summed in neighbour order, (F_0 F_1) F_2 cm cm.
+template <bool ONES>
+__device__ __forceinline__ PlanCell<ONES> plan_cell(const TailLevel &L, int i, int j, int k) {
+    PlanCell<ONES> p;
+    const int cell[3] = {i, j, k};
     const bool last[3] = {i == L.n[0] - 1, j == L.n[1] - 1, k == L.n[2] - 1};
-    t0 = 0.0f;
-    t1 = 0.0f;
+    int mask = 0;
+    float t1 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
         float area = 1.0f, in1 = 1.0f;
-        if (!UNIT) {
+        if (!ONES) {
             area = area * (last[b] ? L.fill[b] : 1.0f);
             area = area * (last[c] ? L.fill[c] : 1.0f);
             in1 = pick4(L.in1[a], last[b], last[c]);
         }
-        // step -1: the neighbour is never the last cell, so the pair involves a last cell iff this one is
-        {
-            int nb[3] = {i, j, k};
-            nb[a] -= 1;
-            float v0 = 0.0f, v1 = 0.0f;
-            if (nb[a] >= 0) {
-                const float coupling = UNIT ? 1.0f : (last[a] ? in1 : area);
-                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : coupling * q(nb[0], nb[1], nb[2]);
-                v1 = coupling;
-            } else if (L.open_lo[a] == 1) {
-                v1 = UNIT ? 1.0f : (last[a] ? pick4(L.ghost[a][1], last[b], last[c])
-                                            : pick4(L.ghost[a][0], last[b], last[c]));
-            }
-            t0 = t0 + v0;
-            t1 = t1 + v1;
+        const bool in_lo = cell[a] > 0, in_hi = !last[a];
+        mask |= (in_lo ? 1 : 0) << (2 * a);
+        mask |= (in_hi ? 1 : 0) << (2 * a + 1);
+        const float cp_lo = ONES ? 1.0f : (last[a] ? in1 : area);
+        const float cp_hi = ONES ? 1.0f : (cell[a] + 1 == L.n[a] - 1 ? in1 : area);
+        const float ghost_lo = last[a] ? pick4(L.ghost[a][1], last[b], last[c]) : pick4(L.ghost[a][0], last[b], last[c]);
+        const float ghost_hi = pick4(L.ghost[a][1], last[b], last[c]);
+        t1 = t1 + (in_lo ? cp_lo : L.open_lo[a] == 1 ? ghost_lo : 0.0f);
+        t1 = t1 + (in_hi ? cp_hi : L.open_hi[a] == 1 ? ghost_hi : 0.0f);
+        if (!ONES) {
+            p.cp[2 * a] = cp_lo;
+            p.cp[2 * a + 1] = cp_hi;
         }
-        // step +1: this cell is not the last; the neighbour is iff it sits at n - 1
-        {
-            int nb[3] = {i, j, k};
-            nb[a] += 1;
-            float v0 = 0.0f, v1 = 0.0f;
-            if (nb[a] < L.n[a]) {
-                const float coupling = UNIT ? 1.0f : (nb[a] == L.n[a] - 1 ? in1 : area);
-                v0 = UNIT ? q(nb[0], nb[1], nb[2]) : coupling * q(nb[0], nb[1], nb[2]);
-                v1 = coupling;
-            } else if (L.open_hi[a] == 1) {
-                v1 = UNIT ? 1.0f : pick4(L.ghost[a][1], last[b], last[c]);
+    }
+    float vol = fill_at(L, 0, i) * fill_at(L, 1, j);
+    vol = vol * fill_at(L, 2, k);
+    p.scale = vol * L.cell_m * L.cell_m;
+    p.t1 = t1;
+    p.packed = ((i * L.n[1] + j) * L.n[2] + k) | (mask << 16);
+    return p;
+}
+
+// The thread's cells of one colour on one level: slots x = tid + u NT of the colour enumeration, (n2 + 1) / 2
+// slots per row, k = 2 slot + ((colour ^ (i + j)) & 1).
+template <bool ONES, int K, int NT>
+struct ColourPlan {
+    PlanCell<ONES> cell[K];
+    __device__ __forceinline__ void build(const TailLevel &L, int color, int tid) {
+        const int half = (L.n[2] + 1) / 2, total = L.n[0] * L.n[1] * half;
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            const int x = tid + u * NT;
+            const int row = L.div_half(x), slot = x - row * half;
+            const int i = L.div_n1(row), j = row - i * L.n[1];
+            const int k = 2 * slot + ((color ^ (i + j)) & 1);
+            if (x < total && k < L.n[2]) {
+                cell[u] = plan_cell<ONES>(L, i, j, k);
+            } else {
+                cell[u].packed = -1;
+                cell[u].t1 = 0.0f;
+                cell[u].scale = 1.0f;
+                if (!ONES)
+                    for (int s = 0; s < 6; s++) cell[u].cp[s] = 1.0f;
             }
-            t0 = t0 + v0;
-            t1 = t1 + v1;
         }
+    }
+};
+
+// t0 of neighbor_sum for a planned cell: the six terms added in the reference's order; a neighbour outside
+// reads the zero word, so its term is +0.0 as the reference's is.
+template <bool ONES>
+__device__ __forceinline__ float plan_t0(const PlanCell<ONES> &p, const float *q, const int (&stride)[6],
+                                         int zero) {
+    const int c = p.packed & 0xffff, mask = p.packed >> 16;
+    float t0 = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const float qn = q[(mask >> s) & 1 ? c + stride[s] : zero];
+        t0 = t0 + (ONES ? qn : p.cp[s] * qn);
+    }
+    return t0;
+}
+
+struct LevelStrides {
+    int s[6];
+    int zero;  // LDS_ZERO relative to the level's q
+};
+
+__device__ __forceinline__ LevelStrides level_strides(const TailLevel &L, int offset) {
+    const int plane = L.n[1] * L.n[2];
+    return LevelStrides{{-plane, plane, -L.n[2], L.n[2], -1, 1}, LDS_ZERO - offset};
+}
+
+// One colour pass over the planned cells (red_black_sweep_kernel, sim/pressure.py:80-92)
+template <bool ONES, int K, int NT>
+__device__ __forceinline__ void plan_pass(const ColourPlan<ONES, K, NT> &plan, float *q, const float *rhs,
+                                          const LevelStrides &st, float relaxation) {
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const PlanCell<ONES> &p = plan.cell[u];
+        if (p.packed < 0 || p.t1 == 0.0f) continue;
+        const int c = p.packed & 0xffff;
+        const float t0 = plan_t0<ONES>(p, q, st.s, st.zero);
+        const float source = p.scale * rhs[c];
+        const float target = (t0 - source) / p.t1;
+        q[c] = q[c] + relaxation * (target - q[c]);
     }
 }
 
-__device__ __forceinline__ float volume_direct(const TailLevel &L, int i, int j, int k) {
-    return fill_at(L, 0, i) * fill_at(L, 1, j) * fill_at(L, 2, k);
-}
-
-// `sweeps` red-black sweeps of one tail level (red_black_sweeps, sim/pressure.py:135-141): each colour pass
-// visits only that colour's cells, (n2 + 1) / 2 slots per row.
-template <bool UNIT, int NT, typename Sync>
-__device__ __forceinline__ void tail_sweeps_t(const TailLevel &L, float *q, const float *rhs, int sweeps,
-                                              float relaxation, int tid, Sync sync) {
-    const Dims n{L.n[0], L.n[1], L.n[2]};
-    const int half = (n.n2 + 1) / 2, total = n.n0 * n.n1 * half;
-    for (int s = 0; s < sweeps; s++) {
-        for (int color = 0; color < 2; color++) {
-            for (int x = tid; x < total; x += NT) {
-                const int row = L.div_half(x), slot = x - row * half;
-                const int i = L.div_n1(row), j = row - i * n.n1;
-                const int k = 2 * slot + ((color ^ (i + j)) & 1);
-                if (k >= n.n2) continue;
-                float t0, t1;
-                neighbor_sum_direct<UNIT>(L, i, j, k, LdsQ{q, n}, t0, t1);
-                if (t1 == 0.0f) continue;
-                const int c = n.at(i, j, k);
-                const float source = volume_direct(L, i, j, k) * L.cell_m * L.cell_m * rhs[c];
-                const float target = (t0 - source) / t1;
-                q[c] = q[c] + relaxation * (target - q[c]);
-            }
-            sync();
-        }
+// residual_kernel (sim/multigrid.py:36-43) of the planned cells into res
+template <bool ONES, int K, int NT>
+__device__ __forceinline__ void plan_residual(const ColourPlan<ONES, K, NT> &plan, const float *q, const float *rhs,
+                                              float *res, const LevelStrides &st) {
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const PlanCell<ONES> &p = plan.cell[u];
+        if (p.packed < 0) continue;
+        const int c = p.packed & 0xffff;
+        const float t0 = plan_t0<ONES>(p, q, st.s, st.zero);
+        res[c] = rhs[c] - (t0 - p.t1 * q[c]) / p.scale;
     }
 }
 
+// restrict_kernel (sim/multigrid.py:46-60) from the stored residuals, and the coarse q cleared (:143)
 template <int NT, typename Sync>
-__device__ __forceinline__ void tail_sweeps(const TailLevel &L, float *q, const float *rhs, int sweeps,
-                                            float relaxation, int tid, Sync sync) {
-    if (L.unit)
-        tail_sweeps_t<true, NT>(L, q, rhs, sweeps, relaxation, tid, sync);
-    else
-        tail_sweeps_t<false, NT>(L, q, rhs, sweeps, relaxation, tid, sync);
-}
-
-// residual + restrict from level L to C, and C's q cleared (sim/multigrid.py:36-60, 139-143)
-template <bool UNIT, int NT, typename Sync>
-__device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailDims &C, const float *q,
-                                                const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
-    const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
+__device__ __forceinline__ void tail_restrict(const TailDims &F, const TailDims &C, const float *res, float *crhs,
+                                              float *cq, int tid, Sync sync) {
+    const Dims nf{F.n[0], F.n[1], F.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
     for (int c = tid; c < nc.count(); c += NT) {
         const int r = C.div_n2(c), k = c - r * nc.n2;
         const int i = C.div_n1(r), j = r - i * nc.n1;
@@ -826,11 +875,7 @@ __device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailDi
                 for (int dk = 0; dk < 2; dk++) {
                     const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
                     if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
-                        float t0, t1;
-                        neighbor_sum_direct<UNIT>(L, fi, fj, fk, LdsQ{q, nf}, t0, t1);
-                        const int f = nf.at(fi, fj, fk);
-                        const float scale = volume_direct(L, fi, fj, fk) * L.cell_m * L.cell_m;
-                        total = total + (rhs[f] - (t0 - t1 * q[f]) / scale);
+                        total = total + res[nf.at(fi, fj, fk)];
                         count = count + 1.0f;
                     }
                 }
@@ -840,13 +885,28 @@ __device__ __forceinline__ void tail_restrict_t(const TailLevel &L, const TailDi
     sync();
 }
 
-template <int NT, typename Sync>
-__device__ __forceinline__ void tail_restrict(const TailLevel &L, const TailDims &C, const float *q,
-                                              const float *rhs, float *crhs, float *cq, int tid, Sync sync) {
-    if (L.unit)
-        tail_restrict_t<true, NT>(L, C, q, rhs, crhs, cq, tid, sync);
-    else
-        tail_restrict_t<false, NT>(L, C, q, rhs, crhs, cq, tid, sync);
+// A visit of one level with the thread's plans of both colours: `sweeps` red-black sweeps, and with a
+// coarser level, the residual stored and restricted onto it.
+template <bool ONES, int K, int NT, typename Sync>
+__device__ __forceinline__ void visit(const TailLevel &L, const TailDims &C, int offset, float *q,
+                                           const float *rhs, float *res, float *crhs, float *cq, int sweeps,
+                                           float relaxation, int tid, Sync sync) {
+    ColourPlan<ONES, K, NT> red, black;
+    red.build(L, 0, tid);
+    black.build(L, 1, tid);
+    const LevelStrides st = level_strides(L, offset);
+    for (int s = 0; s < sweeps; s++) {
+        plan_pass(red, q, rhs, st, relaxation);
+        sync();
+        plan_pass(black, q, rhs, st, relaxation);
+        sync();
+    }
+    if (crhs != nullptr) {
+        plan_residual(red, q, rhs, res, st);
+        plan_residual(black, q, rhs, res, st);
+        sync();
+        tail_restrict<NT>(TailDims{{L.n[0], L.n[1], L.n[2]}, L.div_n1, L.div_n2}, C, res, crhs, cq, tid, sync);
+    }
 }
 
 // prolong_add from level C into L (sim/multigrid.py:63-68, 145)
@@ -867,9 +927,12 @@ __device__ __forceinline__ int coarsest_sweeps(const TailLevel &L) {
     return max(20, 4 * longest);  // COARSEST_SWEEPS, COARSEST_SWEEPS_PER_CELL (sim/multigrid.py:30-31)
 }
 
+// ONES: every tail level has fill_hi (1, 1, 1) (its in-domain couplings are all exactly 1)
+template <bool ONES>
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
                                                           const float *__restrict__ rhs_top, int sweeps,
                                                           const double *gate) {
+    constexpr int K = ONES ? BLOCK_PLAN_ONES : BLOCK_PLAN;
     if (gated_off(gate)) return;
     extern __shared__ float lds[];
     // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
@@ -882,6 +945,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
         uint32_t *dst = reinterpret_cast<uint32_t *>(&t);
         for (int w = threadIdx.x; w < words; w += TAIL_THREADS) dst[w] = src[w];
     }
+    if (threadIdx.x == 0) lds[LDS_ZERO] = 0.0f;
     __syncthreads();
     for (int c = threadIdx.x; c < t.cells[0]; c += TAIL_THREADS) {
         lds[t.offset[0] + c] = t.zero_top ? 0.0f : q_top[c];
@@ -890,40 +954,39 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
     __syncthreads();
     const int last = t.count - 1, tid = threadIdx.x;
     const int block_levels = min(t.wave_from, last);
+    float *res = lds + t.res_offset;
     auto q_of = [&](int m) { return lds + t.offset[m]; };
     auto rhs_of = [&](int m) { return lds + t.offset[m] + t.cells[m]; };
     // down on the whole block: smooth, residual + restrict, coarse q = 0 (sim/multigrid.py:137-143)
-    for (int m = 0; m < block_levels; m++) {
-        const TailLevel L = load_level(t, m);
-        tail_sweeps<TAIL_THREADS>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
-        tail_restrict<TAIL_THREADS>(L, load_dims(t, m + 1), q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid,
-                                    BlockSync{});
-    }
+    for (int m = 0; m < block_levels; m++)
+        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m + 1), t.offset[m], q_of(m), rhs_of(m), res,
+                                        rhs_of(m + 1), q_of(m + 1), sweeps, 1.0f, tid, BlockSync{});
     if (t.wave_from <= last) {
         // the smallest levels on wavefront 0: down, the coarsest solve, and back up
         if (tid < WAVE) {
-            for (int m = t.wave_from; m < last; m++) {
-                const TailLevel L = load_level(t, m);
-                tail_sweeps<WAVE>(L, q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
-                tail_restrict<WAVE>(L, load_dims(t, m + 1), q_of(m), rhs_of(m), rhs_of(m + 1), q_of(m + 1), tid,
-                                    WaveSync{});
-            }
+            for (int m = t.wave_from; m < last; m++)
+                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m + 1), t.offset[m], q_of(m), rhs_of(m), res,
+                               rhs_of(m + 1), q_of(m + 1), sweeps, 1.0f, tid, WaveSync{});
             const TailLevel B = load_level(t, last);
-            tail_sweeps<WAVE>(B, q_of(last), rhs_of(last), coarsest_sweeps(B), 1.5f, tid, WaveSync{});
+            visit<ONES, 1, WAVE>(B, load_dims(t, last), t.offset[last], q_of(last), rhs_of(last), res, nullptr, nullptr,
+                           coarsest_sweeps(B), 1.5f, tid, WaveSync{});
             for (int m = last - 1; m >= t.wave_from; m--) {
                 tail_prolong<WAVE>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, WaveSync{});
-                tail_sweeps<WAVE>(load_level(t, m), q_of(m), rhs_of(m), sweeps, 1.0f, tid, WaveSync{});
+                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m), t.offset[m], q_of(m), rhs_of(m), res, nullptr,
+                               nullptr, sweeps, 1.0f, tid, WaveSync{});
             }
         }
         __syncthreads();
     } else {
         const TailLevel B = load_level(t, last);
-        tail_sweeps<TAIL_THREADS>(B, q_of(last), rhs_of(last), coarsest_sweeps(B), 1.5f, tid, BlockSync{});
+        visit<ONES, K, TAIL_THREADS>(B, load_dims(t, last), t.offset[last], q_of(last), rhs_of(last), res, nullptr,
+                                        nullptr, coarsest_sweeps(B), 1.5f, tid, BlockSync{});
     }
     // up on the whole block: prolong-add, smooth (sim/multigrid.py:145-147)
     for (int m = block_levels - 1; m >= 0; m--) {
         tail_prolong<TAIL_THREADS>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, BlockSync{});
-        tail_sweeps<TAIL_THREADS>(load_level(t, m), q_of(m), rhs_of(m), sweeps, 1.0f, tid, BlockSync{});
+        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m), t.offset[m], q_of(m), rhs_of(m), res,
+                                        nullptr, nullptr, sweeps, 1.0f, tid, BlockSync{});
     }
     for (int c = tid; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
 }
@@ -1102,15 +1165,31 @@ struct Hierarchy {
     int tail_start;
     TailLevels tail;
     size_t tail_bytes;
+    bool tail_ones;  // every tail level has fill_hi (1, 1, 1): k_mg_tail<true>
     const double *gate;
 };
+
+// Lets a tail launch use more than the default 64 KB of dynamic LDS (gfx950 has 160 KB per CU).
+template <bool ONES>
+int allow_tail_lds(size_t bytes) {
+    static size_t allowed = 0;
+    if (bytes <= 65536 || bytes <= allowed) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mg_tail<ONES>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+        return fail(PLM_EINVAL, "multigrid tail: %zu bytes of LDS refused", bytes);
+    allowed = bytes;
+    return 0;
+}
 
 // _v_cycle, sim/multigrid.py:130-147, from level `index`; `cleared`: this level's q is the reference's
 // freshly zeroed coarse q (the caller did not write it)
 int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     if (index == h.tail_start) {
         h.tail.zero_top = cleared ? 1 : 0;
-        k_mg_tail<<<1, TAIL_THREADS, h.tail_bytes, s>>>(h.tail, h.q[index], h.rhs[index], h.sweeps, h.gate);
+        if (h.tail_ones)
+            k_mg_tail<true><<<1, TAIL_THREADS, h.tail_bytes, s>>>(h.tail, h.q[index], h.rhs[index], h.sweeps, h.gate);
+        else
+            k_mg_tail<false><<<1, TAIL_THREADS, h.tail_bytes, s>>>(h.tail, h.q[index], h.rhs[index], h.sweeps, h.gate);
         return check_launch("multigrid tail");
     }
     const LevelArgs &L = h.level[index];
@@ -1347,17 +1426,25 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
     h.sweeps = sweeps;
     h.gate = tolerance_mode ? norm_scratch + 1 : nullptr;
     // the tail starts at the first level from which every coarser level fits in LDS together
+    // (and whose colour slots fit the tail's per-thread plans; levels are added coarsest first, so the one
+    // just added is the largest)
     int total = 0;
+    bool ones = true;
     h.tail_start = n_levels;
     for (int l = n_levels - 1; l >= 0; l--) {
-        const int cells = infos[l].dims[0] * infos[l].dims[1] * infos[l].dims[2];
-        if (total + cells > PLM_LDS_CELLS || n_levels - l > MAX_TAIL) break;
+        const plm_grid_info &g = infos[l];
+        const int cells = g.dims[0] * g.dims[1] * g.dims[2];
+        const bool with = ones && g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f;
+        const int plan = with ? BLOCK_PLAN_ONES : BLOCK_PLAN;
+        if (total + cells > PLM_LDS_CELLS || n_levels - l > MAX_TAIL || colour_slots(g) > plan * TAIL_THREADS) break;
         total += cells;
+        ones = with;
         h.tail_start = l;
     }
+    h.tail_ones = ones;
     if (h.tail_start < n_levels) {
         h.tail.count = n_levels - h.tail_start;
-        int offset = 0;
+        int offset = LDS_HEAD, restricted = 0;
         for (int m = 0; m < h.tail.count; m++) {
             h.tail.level[m] = h.level[h.tail_start + m];
             const plm_grid_info &g = infos[h.tail_start + m];
@@ -1367,10 +1454,16 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
             h.tail.div_n1[m] = fast_div(g.dims[1]);
             h.tail.div_n2[m] = fast_div(g.dims[2]);
             h.tail.div_half[m] = fast_div((g.dims[2] + 1) / 2);
+            if (m + 1 < h.tail.count) restricted = std::max(restricted, h.tail.cells[m]);
         }
-        h.tail_bytes = sizeof(float) * (size_t)offset;
+        h.tail.res_offset = offset;
+        h.tail_bytes = sizeof(float) * (size_t)(offset + restricted);
         h.tail.wave_from = h.tail.count;
-        for (int m = h.tail.count - 1; m >= 0 && h.tail.cells[m] <= WAVE_CELLS; m--) h.tail.wave_from = m;
+        for (int m = h.tail.count - 1; m >= 0; m--) {
+            if (h.tail.cells[m] > WAVE_CELLS || colour_slots(infos[h.tail_start + m]) > WAVE) break;
+            h.tail.wave_from = m;
+        }
+        PLM_TRY(h.tail_ones ? allow_tail_lds<true>(h.tail_bytes) : allow_tail_lds<false>(h.tail_bytes));
     }
 
     TimingScope t("multigrid", s);
