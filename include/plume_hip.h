@@ -255,6 +255,24 @@ int64_t plm_norm_scratch_doubles(const plm_grid_info *info);
 int plm_residual_norms(const plm_grid_info *info, const float *q, const float *rhs, double *scratch, double *out,
                        void *stream);
 
+/* ------------------------------------------------------------------ mixed-precision pressure solve (no
+   reference counterpart: the reference solves in FP32 only). Storage of the correction solve's q and rhs. */
+#define PLM_STORAGE_F32 0
+#define PLM_STORAGE_BF16 1
+#define PLM_STORAGE_F16 2
+
+/* `cycles` V(sweeps, sweeps) cycles of Multigrid._v_cycle (sim/multigrid.py:130-147) with q, rhs and scratch of
+   every level stored as `storage` (FP32 arithmetic in the sweeps and the FP32 residual inside the fused
+   residual + restrict), and the coarsest level (at most 4096 cells) solved in FP64 with the reference's
+   max(20, 4 * longest) sweeps at 1.5. zero_start: the first cycle starts from q = 0 (q is not read). q/rhs/
+   scratch: device arrays of the storage type; the coarsest level needs no scratch. sweeps >= 1. */
+int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage, void *const *q, void *const *rhs,
+                  void *const *scratch, int32_t cycles, int32_t sweeps, int32_t zero_start, void *stream);
+/* dst = storage(src * scale), rounded to nearest even. */
+int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage, void *dst, void *stream);
+/* x = x + float(e) * inv_scale. */
+int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, float inv_scale, void *stream);
+
 /* ------------------------------------------------------------------ budget: sim/reduce.py, conserve.py */
 
 /* line_sums_kernel, sim/reduce.py:12-25: lines[3][ny][nz] = per-(j,k) sums over i, sequential FP32. */

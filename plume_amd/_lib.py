@@ -102,6 +102,9 @@ SIGNATURES = {
     "plm_zero": [_L, _P, _P],
     "plm_vcycle": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _P, _P, _P],
     "plm_residual_norms": [_INFO, _P, _P, _P, _P, _P],
+    "plm_vcycle_lp": [_I, _P, _I, _P, _P, _P, _I, _I, _I, _P],
+    "plm_to_storage": [_L, _P, _F, _I, _P, _P],
+    "plm_add_storage": [_L, _P, _P, _I, _F, _P],
     "plm_line_sums": [_P, _P, _P, _P, _P, _P],
     "plm_smoke_budget": [_P, _P, _P, _P, _P, _P, _P],
     "plm_scale": [_L, _P, _F, _P],

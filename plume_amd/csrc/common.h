@@ -5,6 +5,8 @@
 // bit wherever no transcendental function is involved.
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -107,18 +109,29 @@ __device__ __forceinline__ float clamp_(float x, float lo, float hi) { return fm
 __device__ __forceinline__ float lerp_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ __forceinline__ float length3(float x, float y, float z) { return __fsqrt_rn(x * x + y * y + z * z); }
 
+// Stored field values: FP32, or BF16 / FP16 storage of the mixed-precision solve (converted to FP32 on
+// load, rounded to nearest even on store)
+__device__ __forceinline__ float to_f32(float x) { return x; }
+__device__ __forceinline__ float to_f32(__hip_bfloat16 x) { return __bfloat162float(x); }
+__device__ __forceinline__ float to_f32(__half x) { return __half2float(x); }
+template <typename T> __device__ __forceinline__ T from_f32(float x);
+template <> __device__ __forceinline__ float from_f32<float>(float x) { return x; }
+template <> __device__ __forceinline__ __hip_bfloat16 from_f32<__hip_bfloat16>(float x) { return __float2bfloat16(x); }
+template <> __device__ __forceinline__ __half from_f32<__half>(float x) { return __float2half(x); }
+
 // sample_index_space, sim/sample.py:10-29
-__device__ __forceinline__ float sample_index(const float *__restrict__ f, Dims n, float gx, float gy, float gz) {
+template <typename T>
+__device__ __forceinline__ float sample_index(const T *__restrict__ f, Dims n, float gx, float gy, float gz) {
     float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
     float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
     float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
     int i0 = (int)x, j0 = (int)y, k0 = (int)z;
     int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
     float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
-    float c00 = lerp_(f[n.at(i0, j0, k0)], f[n.at(i1, j0, k0)], tx);
-    float c10 = lerp_(f[n.at(i0, j1, k0)], f[n.at(i1, j1, k0)], tx);
-    float c01 = lerp_(f[n.at(i0, j0, k1)], f[n.at(i1, j0, k1)], tx);
-    float c11 = lerp_(f[n.at(i0, j1, k1)], f[n.at(i1, j1, k1)], tx);
+    float c00 = lerp_(to_f32(f[n.at(i0, j0, k0)]), to_f32(f[n.at(i1, j0, k0)]), tx);
+    float c10 = lerp_(to_f32(f[n.at(i0, j1, k0)]), to_f32(f[n.at(i1, j1, k0)]), tx);
+    float c01 = lerp_(to_f32(f[n.at(i0, j0, k1)]), to_f32(f[n.at(i1, j0, k1)]), tx);
+    float c11 = lerp_(to_f32(f[n.at(i0, j1, k1)]), to_f32(f[n.at(i1, j1, k1)]), tx);
     return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
 }
 

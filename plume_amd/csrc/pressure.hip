@@ -288,6 +288,13 @@ struct GlobalQ {
     __device__ float operator()(int i, int j, int k) const { return q[n.at(i, j, k)]; }
 };
 
+template <typename T>
+struct StoredQ {
+    const T *q;
+    Dims n;
+    __device__ float operator()(int i, int j, int k) const { return to_f32(q[n.at(i, j, k)]); }
+};
+
 // red_black_sweep_kernel, sim/pressure.py:80-92 (one colour, in place)
 template <bool UNIT>
 __global__ __launch_bounds__(256) void k_rb_sweep(LevelArgs L, float *__restrict__ q, const float *__restrict__ rhs,
@@ -311,7 +318,8 @@ constexpr int TK = 64, TH = 2, LK = TK + 2 * TH;
 enum SweepStart { FROM_Q = 0, FROM_ZERO = 1, FROM_PROLONG = 2 };
 
 // sample_index_space of the coarse correction at a fine cell (prolong_add_kernel, sim/multigrid.py:63-68)
-__device__ __forceinline__ float prolongation(const float *__restrict__ coarse, Dims nc, int i, int j, int k) {
+template <typename T>
+__device__ __forceinline__ float prolongation(const T *__restrict__ coarse, Dims nc, int i, int j, int k) {
     return sample_index(coarse, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f, (float)k * 0.5f - 0.25f);
 }
 
@@ -330,11 +338,13 @@ __device__ __forceinline__ int xcd_contiguous(int block, int blocks) {
 // aligned 256-byte access by one wavefront, the 4 halo cells a small second loop. Red runs on the tile and
 // a 1-cell ring (the ring's new values are what black needs): 32 cells per row on half a wavefront, plus the
 // one red cell of each ring row that lies in the k halo; black runs on the tile's rows; then the rows are
-// written out. The rhs values both passes need are loaded into registers before the first barrier.
-template <int TI, int TJ, int THREADS, bool UNIT, int START>
-__global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const float *__restrict__ q_in,
-                                                           float *__restrict__ q_out, const float *__restrict__ rhs,
-                                                           const float *__restrict__ coarse, Dims nc,
+// written out. The rhs values both passes need are loaded into registers before the first barrier. T is the
+// stored type of q, rhs and the coarse correction (FP32, or BF16 / FP16 in the mixed-precision solve); the
+// tile and all arithmetic are FP32.
+template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T = float>
+__global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T *__restrict__ q_in,
+                                                           T *__restrict__ q_out, const T *__restrict__ rhs,
+                                                           const T *__restrict__ coarse, Dims nc,
                                                            float relaxation, const double *gate, int tiles_k,
                                                            int tiles_j) {
     constexpr int WAVES = THREADS / 64;
@@ -387,33 +397,53 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const fl
 #pragma unroll
     for (int u = 0; u < RED_ITERS; u++) {
         int i, j, k;
-        rhs_red[u] = red_main(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? rhs[n.at(i, j, k)] : 0.0f;
+        rhs_red[u] = red_main(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? to_f32(rhs[n.at(i, j, k)]) : 0.0f;
     }
     {
         int i, j, k;
-        if (red_edge(threadIdx.x, i, j, k)) rhs_edge = rhs[n.at(i, j, k)];
+        if (red_edge(threadIdx.x, i, j, k)) rhs_edge = to_f32(rhs[n.at(i, j, k)]);
     }
 #pragma unroll
     for (int u = 0; u < BLACK_ITERS; u++) {
         int i, j, k;
-        rhs_black[u] = black_cell(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? rhs[n.at(i, j, k)] : 0.0f;
+        rhs_black[u] = black_cell(2 * (wave + u * WAVES) + upper, half_lane, i, j, k) ? to_f32(rhs[n.at(i, j, k)]) : 0.0f;
     }
+    // The tile's start values. Reads outside the level use a clamped address and are replaced by 0, so the
+    // loads carry no branches; rows go in batches whose loads are all in flight before the LDS writes.
     auto start_value = [&](int i, int j, int k) {
-        float value = 0.0f;
-        if (inside(i, j, k)) {
-            if (START == FROM_Q) value = q_in[n.at(i, j, k)];
-            if (START == FROM_PROLONG) value = q_in[n.at(i, j, k)] + prolongation(coarse, nc, i, j, k);
-        }
-        return value;
+        if (START == FROM_ZERO) return 0.0f;
+        const bool in = inside(i, j, k);
+        const int ci = min(max(i, 0), n.n0 - 1), cj = min(max(j, 0), n.n1 - 1), ck = min(max(k, 0), n.n2 - 1);
+        float value = to_f32(q_in[n.at(ci, cj, ck)]);
+        if (START == FROM_PROLONG) value = value + prolongation(coarse, nc, ci, cj, ck);
+        return in ? value : 0.0f;
     };
-    for (int row = wave; row < LI * LJ; row += WAVES) {
-        const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ, k = k0 + lane;
-        s[row * LK + slot(k)] = start_value(i, j, k);
+    constexpr int ROWS = LI * LJ / WAVES, BATCH = START == FROM_PROLONG ? 2 : ROWS % 6 == 0 ? 6 : 8;
+    static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
+    for (int b = 0; b < ROWS; b += BATCH) {
+        float v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) {
+            const int row = wave + (b + u) * WAVES;
+            v[u] = start_value(i0 - TH + row / LJ, j0 - TH + row % LJ, k0 + lane);
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + slot(k0 + lane)] = v[u];
     }
-    for (int t = threadIdx.x; t < LI * LJ * 4; t += THREADS) {
-        const int row = t >> 2, h = t & 3;
-        const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ, k = h < 2 ? k0 - TH + h : k0 + TK - 2 + h;
-        s[row * LK + slot(k)] = start_value(i, j, k);
+    {
+        constexpr int HALO = LI * LJ * 4, HALO_ITERS = (HALO + THREADS - 1) / THREADS;
+        float v[HALO_ITERS];
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int t = min((int)threadIdx.x + u * THREADS, HALO - 1), row = t >> 2, h = t & 3;
+            const int k = h < 2 ? k0 - TH + h : k0 + TK - 2 + h;
+            v[u] = start_value(i0 - TH + row / LJ, j0 - TH + row % LJ, k);
+        }
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int t = threadIdx.x + u * THREADS, row = t >> 2, h = t & 3;
+            if (t < HALO) s[row * LK + slot(h < 2 ? k0 - TH + h : k0 + TK - 2 + h)] = v[u];
+        }
     }
     __syncthreads();
     auto update = [&](int i, int j, int k, float rhs_c) {
@@ -438,9 +468,18 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const fl
         if (black_cell(2 * (wave + u * WAVES) + upper, half_lane, i, j, k)) update(i, j, k, rhs_black[u]);
     }
     __syncthreads();
-    for (int row = wave; row < TI * TJ; row += WAVES) {
+    constexpr int OUT_ROWS = TI * TJ / WAVES;
+    float v[OUT_ROWS];
+#pragma unroll
+    for (int u = 0; u < OUT_ROWS; u++) {
+        const int row = wave + u * WAVES;
+        v[u] = s[(row / TJ + TH) * LJ * LK + (row % TJ + TH) * LK + slot(k0 + lane)];
+    }
+#pragma unroll
+    for (int u = 0; u < OUT_ROWS; u++) {
+        const int row = wave + u * WAVES;
         const int i = i0 + row / TJ, j = j0 + row % TJ, k = k0 + lane;
-        if (i < n.n0 && j < n.n1 && k < n.n2) q_out[n.at(i, j, k)] = at(i, j, k);
+        if (i < n.n0 && j < n.n1 && k < n.n2) q_out[n.at(i, j, k)] = from_f32<T>(v[u]);
     }
 }
 
@@ -452,17 +491,16 @@ constexpr TileShape TILE_SHAPES[] = {{4, 4, 256}, {2, 8, 256}, {8, 4, 256}, {4, 
 constexpr int N_TILE_SHAPES = 6;
 constexpr int DEFAULT_TILE_SHAPE = 0;
 
-template <bool UNIT, int START>
-void launch_tile_shape(int shape, dim3 grid, hipStream_t s, const LevelArgs &L, const float *src, float *dst,
-                       const float *rhs, const float *coarse, Dims nc, float relaxation, const double *gate,
-                       int tiles_k, int tiles_j) {
+template <bool UNIT, int START, typename T>
+void launch_tile_shape(int shape, dim3 grid, hipStream_t s, const LevelArgs &L, const T *src, T *dst, const T *rhs,
+                       const T *coarse, Dims nc, float relaxation, const double *gate, int tiles_k, int tiles_j) {
     switch (shape) {
-    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    default: k_rb_sweep_tile<4, 4, 256, UNIT, START><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START, T><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START, T><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    default: k_rb_sweep_tile<4, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
     }
 }
 
@@ -545,11 +583,10 @@ __global__ __launch_bounds__(256) void k_restrict(const float *__restrict__ fine
 
 // residual_kernel + restrict_kernel fused: each coarse cell forms its children's residuals in registers.
 // coarse_q (may be NULL) is set to 0, the reference's coarse.q.zero_() (sim/multigrid.py:143).
-template <bool UNIT>
-__global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const float *__restrict__ q,
-                                                           const float *__restrict__ rhs,
-                                                           float *__restrict__ coarse_rhs, float *__restrict__ coarse_q,
-                                                           Dims nc, const double *gate) {
+template <bool UNIT, typename T = float>
+__global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T *__restrict__ q,
+                                                           const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
+                                                           T *__restrict__ coarse_q, Dims nc, const double *gate) {
     if (gated_off(gate)) return;
     const Dims nf = cell_dims(L.g);
     int i, j, k;
@@ -561,15 +598,15 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const fl
                 const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
                 if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
                     float t0, t1;
-                    neighbor_sum<UNIT>(L, fi, fj, fk, GlobalQ{q, nf}, t0, t1);
+                    neighbor_sum<UNIT>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
                     const int c = nf.at(fi, fj, fk);
-                    total = total + residual_value(L, fi, fj, fk, q[c], rhs[c], t0, t1);
+                    total = total + residual_value(L, fi, fj, fk, to_f32(q[c]), to_f32(rhs[c]), t0, t1);
                     count = count + 1.0f;
                 }
             }
     const int c = nc.at(i, j, k);
-    coarse_rhs[c] = total / count;
-    if (coarse_q != nullptr) coarse_q[c] = 0.0f;
+    coarse_rhs[c] = from_f32<T>(total / count);
+    if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
 }
 
 // prolong_add_kernel, sim/multigrid.py:63-68
@@ -1089,25 +1126,27 @@ int launch_norms(const LevelArgs &L, const float *q, const float *rhs, double *s
     return check_launch("norm finish");
 }
 
-template <bool UNIT>
-int launch_tile(const LevelArgs &L, int start, const float *src, float *dst, const float *rhs, const float *coarse,
-                Dims nc, float relaxation, const double *gate, hipStream_t s, int shape = -1) {
+template <bool UNIT, typename T = float>
+int launch_tile(const LevelArgs &L, int start, const T *src, T *dst, const T *rhs, const T *coarse, Dims nc,
+                float relaxation, const double *gate, hipStream_t s, int shape = -1) {
     const Dims n = cell_dims(L.g);
     if (shape < 0) shape = tile_shape_for(n);
     const TileShape &ts = TILE_SHAPES[shape];
     const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + ts.tj - 1) / ts.tj, tiles_i = (n.n0 + ts.ti - 1) / ts.ti;
     const dim3 grid((unsigned)(tiles_k * tiles_j * tiles_i));
-    // family "rb_sweep_tile_top": the plain two-colour sweep (q in, q out) on the simulation grid
-    TimingScope launch_timing(!UNIT ? "rb_sweep_tile" : start == FROM_Q ? "rb_sweep_tile_top" : "rb_sweep_tile_top_start",
+    // family "rb_sweep_tile_top": the plain FP32 two-colour sweep (q in, q out) on the simulation grid
+    TimingScope launch_timing(!UNIT || sizeof(T) != 4 ? "rb_sweep_tile"
+                              : start == FROM_Q      ? "rb_sweep_tile_top"
+                                                     : "rb_sweep_tile_top_start",
                               s);
     if (start == FROM_ZERO)
-        launch_tile_shape<UNIT, FROM_ZERO>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+        launch_tile_shape<UNIT, FROM_ZERO, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
                                                tiles_j);
     else if (start == FROM_PROLONG)
-        launch_tile_shape<UNIT, FROM_PROLONG>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+        launch_tile_shape<UNIT, FROM_PROLONG, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
                                                tiles_j);
     else
-        launch_tile_shape<UNIT, FROM_Q>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
+        launch_tile_shape<UNIT, FROM_Q, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
                                                tiles_j);
     return check_launch("rb sweep tile");
 }
@@ -1206,15 +1245,175 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     const Dims nc = cell_dims(h.level[index + 1].g);
     // the coarse q is cleared by the next level's first sweep (or the tail), so it is not written here
     if (L.unit)
-        k_residual_restrict<true><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+        k_residual_restrict<true, float><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
                                                                 nullptr, nc, h.gate);
     else
-        k_residual_restrict<false><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+        k_residual_restrict<false, float><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
                                                                  nullptr, nc, h.gate);
     PLM_TRY(check_launch("residual restrict"));
     PLM_TRY(v_cycle(h, index + 1, true, s));
     return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, FROM_PROLONG,
                          h.q[index + 1], nc, h.gate, s);
+}
+
+// ------------------------------------------------------------------ mixed-precision V-cycle
+
+// The coarsest level of the mixed-precision solve in FP64: red_black_sweeps (sim/pressure.py:80-92) with the
+// coarsest level's max(20, 4 longest) sweeps at 1.5 (sim/multigrid.py:131-135), q and rhs staged in LDS as
+// doubles and neighbor_term's couplings (sim/pressure.py:52-69) formed in double from the level's geometry.
+constexpr int COARSE_F64_THREADS = 1024;
+constexpr int COARSE_F64_CELLS = 4096;
+
+__device__ __forceinline__ double fill_f64(const plm_grid_info &g, int a, int index) {
+    return index == g.dims[a] - 1 ? (double)g.fill_hi[a] : 1.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(COARSE_F64_THREADS) void k_coarse_f64(plm_grid_info g, T *__restrict__ q,
+                                                                   const T *__restrict__ rhs, int sweeps,
+                                                                   int from_zero) {
+    extern __shared__ double dl[];
+    const Dims n = cell_dims(g);
+    const int cells = n.count();
+    double *dq = dl, *drhs = dl + cells;
+    for (int c = threadIdx.x; c < cells; c += COARSE_F64_THREADS) {
+        dq[c] = from_zero ? 0.0 : (double)to_f32(q[c]);
+        drhs[c] = (double)to_f32(rhs[c]);
+    }
+    __syncthreads();
+    const double cm = (double)g.cell_m, gap = (double)g.ghost_gap;
+    for (int s = 0; s < sweeps; s++)
+        for (int color = 0; color < 2; color++) {
+            for (int c = threadIdx.x; c < cells; c += COARSE_F64_THREADS) {
+                const int k = c % n.n2, r = c / n.n2, j = r % n.n1, i = r / n.n1;
+                if (((i + j + k) & 1) != color) continue;
+                const int cell[3] = {i, j, k};
+                double t0 = 0.0, t1 = 0.0;
+                for (int a = 0; a < 3; a++) {
+                    const int b = a == 0 ? 1 : 0, cc = a == 2 ? 1 : 2;
+                    const double area = 1.0 * fill_f64(g, b, cell[b]) * fill_f64(g, cc, cell[cc]);
+                    const double own = fill_f64(g, a, cell[a]);
+                    for (int side = 0; side < 2; side++) {
+                        int nb[3] = {i, j, k};
+                        nb[a] += side == 0 ? -1 : 1;
+                        if (nb[a] >= 0 && nb[a] < g.dims[a]) {
+                            const double coupling = area / (0.5 * (own + fill_f64(g, a, nb[a])));
+                            t0 += coupling * dq[n.at(nb[0], nb[1], nb[2])];
+                            t1 += coupling;
+                        } else if (side == 0 ? g.open_lo[a] == 1 : g.open_hi[a] == 1) {
+                            t1 += area / (0.5 * own + gap);
+                        }
+                    }
+                }
+                if (t1 == 0.0) continue;
+                const double vol = fill_f64(g, 0, i) * fill_f64(g, 1, j) * fill_f64(g, 2, k);
+                const double target = (t0 - vol * cm * cm * drhs[c]) / t1;
+                dq[c] = dq[c] + 1.5 * (target - dq[c]);
+            }
+            __syncthreads();
+        }
+    for (int c = threadIdx.x; c < cells; c += COARSE_F64_THREADS) q[c] = from_f32<T>((float)dq[c]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_t(int64_t count, const T *__restrict__ src, T *__restrict__ dst) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
+        dst[c] = src[c];
+}
+
+// dst = T(src * scale): the FP32 residual into the storage type of the correction solve
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_storage(int64_t count, const float *__restrict__ src, float scale,
+                                                    T *__restrict__ dst) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
+        dst[c] = from_f32<T>(src[c] * scale);
+}
+
+// x += float(e) * inv_scale: the correction back into the FP32 solution
+template <typename T>
+__global__ __launch_bounds__(256) void k_add_storage(int64_t count, float *__restrict__ x, const T *__restrict__ e,
+                                                     float inv_scale) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
+        x[c] = x[c] + to_f32(e[c]) * inv_scale;
+}
+
+template <typename T>
+struct LpHierarchy {
+    int n_levels;
+    LevelArgs level[16];
+    T *const *q;
+    T *const *rhs;
+    T *const *scratch;
+    int sweeps;
+};
+
+// `sweeps` (>= 1) two-colour tile sweeps of one level in storage T, ping-ponging q and scratch
+template <typename T>
+int lp_sweeps(const LevelArgs &L, T *q, const T *rhs, T *scratch, int sweeps, int start, const T *coarse, Dims nc,
+              hipStream_t s) {
+    for (int sw = 0; sw < sweeps; sw++) {
+        const bool to_scratch = sw % 2 == 0;
+        const T *src = to_scratch ? q : scratch;
+        T *dst = to_scratch ? scratch : q;
+        const int first = sw == 0 ? start : FROM_Q;
+        if (L.unit)
+            PLM_TRY((launch_tile<true, T>(L, first, src, dst, rhs, coarse, nc, 1.0f, nullptr, s)));
+        else
+            PLM_TRY((launch_tile<false, T>(L, first, src, dst, rhs, coarse, nc, 1.0f, nullptr, s)));
+    }
+    if (sweeps % 2 == 1) {
+        const int64_t count = cell_dims(L.g).count();
+        k_copy_t<T><<<grid1(count), 256, 0, s>>>(count, scratch, q);
+        PLM_TRY(check_launch("copy back"));
+    }
+    return 0;
+}
+
+// _v_cycle (sim/multigrid.py:130-147) with every level stored in T and the coarsest solved in FP64
+template <typename T>
+int v_cycle_lp(const LpHierarchy<T> &h, int index, bool cleared, hipStream_t s) {
+    const LevelArgs &L = h.level[index];
+    const Dims n = cell_dims(L.g);
+    if (index == h.n_levels - 1) {
+        const int longest = max(n.n0, max(n.n1, n.n2));
+        k_coarse_f64<T><<<1, COARSE_F64_THREADS, 2 * sizeof(double) * (size_t)n.count(), s>>>(
+            L.g, h.q[index], h.rhs[index], max(20, 4 * longest), cleared ? 1 : 0);
+        return check_launch("coarsest FP64 solve");
+    }
+    const Dims none{1, 1, 1};
+    PLM_TRY(lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, cleared ? FROM_ZERO : FROM_Q,
+                         nullptr, none, s));
+    const Dims nc = cell_dims(h.level[index + 1].g);
+    if (L.unit)
+        k_residual_restrict<true, T><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+                                                                   nullptr, nc, nullptr);
+    else
+        k_residual_restrict<false, T><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
+                                                                    nullptr, nc, nullptr);
+    PLM_TRY(check_launch("residual restrict"));
+    PLM_TRY(v_cycle_lp(h, index + 1, true, s));
+    return lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, FROM_PROLONG, h.q[index + 1], nc, s);
+}
+
+template <typename T>
+int vcycle_lp(int32_t n_levels, const plm_grid_info *infos, void *const *q, void *const *rhs, void *const *scratch,
+              int32_t cycles, int32_t sweeps, int32_t zero_start, hipStream_t s) {
+    LpHierarchy<T> h{};
+    h.n_levels = n_levels;
+    for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
+    h.q = reinterpret_cast<T *const *>(q);
+    h.rhs = reinterpret_cast<T *const *>(rhs);
+    h.scratch = reinterpret_cast<T *const *>(scratch);
+    h.sweeps = sweeps;
+    const Dims nb = cell_dims(infos[n_levels - 1]);
+    const size_t coarse_bytes = 2 * sizeof(double) * (size_t)nb.count();
+    if (coarse_bytes > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_f64<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarse_bytes) != hipSuccess)
+        return fail(PLM_EINVAL, "plm_vcycle_lp: %zu bytes of LDS refused", coarse_bytes);
+    TimingScope t("multigrid_lp", s);
+    for (int c = 0; c < cycles; c++) PLM_TRY(v_cycle_lp<T>(h, 0, c == 0 && zero_start, s));
+    return 0;
 }
 
 }  // namespace
@@ -1357,9 +1556,9 @@ int plm_autotune_sweep(const plm_grid_info *info, float *q, const float *rhs, fl
     for (int v = 0; v < N_TILE_SHAPES && rc == 0; v++) {
         auto sweep = [&](int r) {
             const bool even = r % 2 == 0;
-            return L.unit ? launch_tile<true>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
+            return L.unit ? launch_tile<true, float>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
                                               1.0f, nullptr, s, v)
-                          : launch_tile<false>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
+                          : launch_tile<false, float>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
                                                1.0f, nullptr, s, v);
         };
         rc = sweep(0);  // warm-up
@@ -1475,6 +1674,62 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
             PLM_TRY(launch_norms(h.level[0], q[0], rhs[0], norm_scratch, h.gate, tolerance, 2, cycles_run, s));
     }
     return 0;
+}
+
+
+int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage, void *const *q, void *const *rhs,
+                  void *const *scratch, int32_t cycles, int32_t sweeps, int32_t zero_start, void *stream) {
+    PLM_REQUIRE(n_levels >= 2 && n_levels <= 16 && infos && q && rhs && scratch, "plm_vcycle_lp: bad level arrays");
+    PLM_REQUIRE(storage >= PLM_STORAGE_F32 && storage <= PLM_STORAGE_F16, "plm_vcycle_lp: storage %d unknown",
+                storage);
+    PLM_REQUIRE(cycles >= 0 && sweeps >= 1, "plm_vcycle_lp: cycles must be >= 0 and sweeps >= 1");
+    for (int l = 0; l < n_levels; l++) {
+        PLM_TRY(validate_info(&infos[l]));
+        PLM_REQUIRE(q[l] && rhs[l] && (scratch[l] || l == n_levels - 1), "plm_vcycle_lp: NULL array at level %d", l);
+        if (l > 0)
+            for (int a = 0; a < 3; a++)
+                PLM_REQUIRE(infos[l].dims[a] == (infos[l - 1].dims[a] + 1) / 2,
+                            "plm_vcycle_lp: level %d does not halve level %d", l, l - 1);
+    }
+    const plm_grid_info &last = infos[n_levels - 1];
+    PLM_REQUIRE(last.dims[0] * last.dims[1] * last.dims[2] <= COARSE_F64_CELLS,
+                "plm_vcycle_lp: the coarsest level (FP64, one workgroup) holds at most %d cells", COARSE_F64_CELLS);
+    hipStream_t s = as_stream(stream);
+    if (storage == PLM_STORAGE_BF16)
+        return vcycle_lp<__hip_bfloat16>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
+    if (storage == PLM_STORAGE_F16)
+        return vcycle_lp<__half>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
+    return vcycle_lp<float>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
+}
+
+int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage, void *dst, void *stream) {
+    PLM_REQUIRE(count >= 0 && src && dst, "plm_to_storage: bad arguments");
+    hipStream_t s = as_stream(stream);
+    if (count == 0) return 0;
+    if (storage == PLM_STORAGE_BF16)
+        k_to_storage<__hip_bfloat16><<<grid1(count), 256, 0, s>>>(count, src, scale, (__hip_bfloat16 *)dst);
+    else if (storage == PLM_STORAGE_F16)
+        k_to_storage<__half><<<grid1(count), 256, 0, s>>>(count, src, scale, (__half *)dst);
+    else if (storage == PLM_STORAGE_F32)
+        k_to_storage<float><<<grid1(count), 256, 0, s>>>(count, src, scale, (float *)dst);
+    else
+        return fail(PLM_EINVAL, "plm_to_storage: storage %d unknown", storage);
+    return check_launch("to storage");
+}
+
+int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, float inv_scale, void *stream) {
+    PLM_REQUIRE(count >= 0 && x && e, "plm_add_storage: bad arguments");
+    hipStream_t s = as_stream(stream);
+    if (count == 0) return 0;
+    if (storage == PLM_STORAGE_BF16)
+        k_add_storage<__hip_bfloat16><<<grid1(count), 256, 0, s>>>(count, x, (const __hip_bfloat16 *)e, inv_scale);
+    else if (storage == PLM_STORAGE_F16)
+        k_add_storage<__half><<<grid1(count), 256, 0, s>>>(count, x, (const __half *)e, inv_scale);
+    else if (storage == PLM_STORAGE_F32)
+        k_add_storage<float><<<grid1(count), 256, 0, s>>>(count, x, (const float *)e, inv_scale);
+    else
+        return fail(PLM_EINVAL, "plm_add_storage: storage %d unknown", storage);
+    return check_launch("add storage");
 }
 
 }  // extern "C"

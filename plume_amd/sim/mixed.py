@@ -1,0 +1,159 @@
+"""Mixed-precision pressure solve: iterative refinement around a low-precision multigrid.
+
+The reference solves the pressure Poisson equation in FP32 only (sim/multigrid.py:97-147); this is an addition
+with no reference counterpart, for large grids where the solve is HBM bound. The FP32 pressure x is refined
+until |b - A x| / |b| <= tolerance, the norms reduced in FP64 on the device. Each round
+
+1. forms r = b - A x in FP32 (residual_kernel, sim/multigrid.py:36-43);
+2. scales r by a power of two (exact) that brings its RMS near 1, so FP16 neither overflows nor underflows,
+   and rounds it into the storage type (BF16 or FP16, round to nearest even);
+3. runs `inner_cycles` V(sweeps, sweeps) cycles of the reference's V-cycle on A e = r from e = 0, every
+   level's q and rhs stored in that type, FP32 arithmetic in the sweeps, FP32 residuals inside the fused
+   residual + restrict, and the coarsest level solved in FP64 (plm_vcycle_lp);
+4. adds e back into x in FP32.
+
+The FP32 pressure itself bounds the reachable residual: even the exact solution rounded to FP32 leaves a
+relative residual of order 1e-6 on grids of 10^5 cells and more. A round that does not halve the residual
+therefore ends the solve (`log.stalled`) instead of spinning at that floor.
+
+The level hierarchy is the reference's (Multigrid._build_levels, sim/multigrid.py:106-122), optionally cut
+at `max_levels` so that the coarsest level is the one the FP64 solve handles (8 x 12 x 8 at 256 x 384 x 256).
+"""
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+from plume_amd._lib import LIB, GridInfo, call, ptr, stream
+from plume_amd.sim.multigrid import MAX_LEVELS, build_levels
+
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2}  # PLM_STORAGE_* in include/plume_hip.h
+STALL = 0.5  # a round that does not at least halve the residual ends the solve
+DTYPES = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}
+
+
+def _vptr(t: torch.Tensor) -> int:
+    if not t.is_contiguous() or t.device.type != "cuda":
+        raise ValueError("expected a contiguous device tensor")
+    return t.data_ptr()
+
+
+@dataclass
+class RefinementLog:
+    """Relative residual before each refinement round and after the last; `stalled` when the solve stopped
+    because a round no longer halved the residual (the FP32 pressure's representation floor)."""
+
+    residuals: list = field(default_factory=list)
+    stalled: bool = False
+
+    @property
+    def rounds(self) -> int:
+        return max(0, len(self.residuals) - 1)
+
+
+class MixedPrecisionMultigrid:
+    """Pressure solve in the reference's PressureSolve role (grid.pressure refined in place from its current
+    value, right-hand side grid.divergence), see the module docstring."""
+
+    def __init__(self, grid, storage: str = "bf16", tolerance: float = 1e-6, sweeps: int = 2,
+                 inner_cycles: int = 1, max_rounds: int = 60, max_levels: int = MAX_LEVELS):
+        if storage not in STORAGE:
+            raise ValueError(f"storage must be one of {tuple(STORAGE)}")
+        if sweeps < 1 or inner_cycles < 1:
+            raise ValueError("sweeps and inner_cycles must be >= 1")
+        self.storage = storage
+        self.tolerance = float(tolerance)
+        self.sweeps = sweeps
+        self.inner_cycles = inner_cycles
+        self.max_rounds = max_rounds
+        levels = build_levels(grid, max_levels)
+        if len(levels) < 2:
+            raise ValueError("the mixed-precision solve needs at least two levels")
+        self.infos = (GridInfo * len(levels))(*(level.info for level in levels))
+        self.cells = [level.cells for level in levels]
+        dtype = DTYPES[storage]
+        device = grid.device
+        self.q = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells]
+        self.rhs = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells]
+        self.scratch = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells[:-1]]
+        self.residual = torch.zeros(self.cells[0], dtype=torch.float32, device=device)
+        self.norm_scratch = torch.zeros(LIB.plm_norm_scratch_doubles(ctypes.byref(self.infos[0])),
+                                        dtype=torch.float64, device=device)
+        self.norms = torch.zeros(2, dtype=torch.float64, device=device)
+        self.log = RefinementLog()
+
+    def _arrays(self):
+        n = len(self.cells)
+        array = ctypes.c_void_p * n
+        return (array(*(_vptr(t) for t in self.q)), array(*(_vptr(t) for t in self.rhs)),
+                array(*([_vptr(t) for t in self.scratch] + [None])))
+
+    def relative_residual(self, grid) -> tuple[float, float]:
+        """(|b - A x| / |b|, RMS of b - A x) in FP64 (synchronises)."""
+        call("plm_residual_norms", ctypes.byref(self.infos[0]), ptr(grid.pressure), ptr(grid.divergence),
+             ptr(self.norm_scratch, torch.float64), ptr(self.norms, torch.float64), stream())
+        r2, b2 = self.norms.tolist()
+        count = grid.pressure.numel()
+        rel = math.sqrt(r2 / b2) if b2 > 0 else (0.0 if r2 == 0 else math.inf)
+        return rel, math.sqrt(r2 / count)
+
+    def __call__(self, grid) -> None:
+        x, b = grid.pressure, grid.divergence
+        count = x.numel()
+        code = STORAGE[self.storage]
+        q, rhs, scratch = self._arrays()
+        self.log = RefinementLog()
+        for _ in range(self.max_rounds + 1):
+            rel, rms = self.relative_residual(grid)
+            self.log.residuals.append(rel)
+            if rel <= self.tolerance or rms == 0.0 or len(self.log.residuals) > self.max_rounds:
+                return
+            if len(self.log.residuals) >= 2 and rel > STALL * self.log.residuals[-2]:
+                self.log.stalled = True
+                return
+            call("plm_residual", ctypes.byref(self.infos[0]), ptr(x), ptr(b), ptr(self.residual), stream())
+            scale = 2.0 ** (-round(math.log2(rms)))
+            call("plm_to_storage", count, ptr(self.residual), scale, code, _vptr(self.rhs[0]), stream())
+            call("plm_vcycle_lp", len(self.cells), self.infos, code, q, rhs, scratch, self.inner_cycles, self.sweeps,
+                 1, stream())
+            call("plm_add_storage", count, ptr(x), _vptr(self.q[0]), code, 1.0 / scale, stream())
+
+
+def divergent_velocity(grid, seed: int, blobs: int = 64, emitter_radius_cells: float = 12.0,
+                       emitter_speed: float = 1.0) -> None:
+    """Fills grid.u/v/w with a seeded divergent field: `blobs` Gaussian vortex blobs (random centre, axis,
+    strength and radius) plus a radial emitter source at the domain's floor centre, 10 cells up. Evaluated
+    in FP64 at each face position, stored as FP32."""
+    g = torch.Generator().manual_seed(seed)
+    nx, ny, nz = grid.cells
+    size = torch.tensor([nx, ny, nz], dtype=torch.float64)
+    centers = torch.rand(blobs, 3, generator=g, dtype=torch.float64) * size
+    axes = torch.nn.functional.normalize(torch.randn(blobs, 3, generator=g, dtype=torch.float64), dim=1)
+    strengths = (torch.rand(blobs, generator=g, dtype=torch.float64) * 2.0 - 1.0) * 0.2
+    radii = (0.05 + 0.1 * torch.rand(blobs, generator=g, dtype=torch.float64)) * float(min(nx, ny, nz))
+    source = torch.tensor([nx / 2.0, 10.0, nz / 2.0], dtype=torch.float64)
+    device = grid.device
+    for axis, f in enumerate(grid.velocity_fields()):
+        shape = tuple(f.data.shape)
+        offset = [0.5, 0.5, 0.5]
+        offset[axis] = 0.0
+        coords = [torch.arange(shape[a], dtype=torch.float64, device=device) + offset[a] for a in range(3)]
+        X, Y, Z = torch.meshgrid(*coords, indexing="ij")
+        out = torch.zeros(shape, dtype=torch.float64, device=device)
+        for b in range(blobs):
+            c = centers[b].tolist()
+            dx, dy, dz = X - c[0], Y - c[1], Z - c[2]
+            weight = torch.exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * float(radii[b]) ** 2)) * float(strengths[b])
+            a = axes[b].tolist()
+            # (axis x d)[axis component] of a vortex about `a` through the centre
+            d = (dx, dy, dz)
+            cross = (a[1] * d[2] - a[2] * d[1], a[2] * d[0] - a[0] * d[2], a[0] * d[1] - a[1] * d[0])
+            out += weight * cross[axis] / float(radii[b])
+        s = source.tolist()
+        dx, dy, dz = X - s[0], Y - s[1], Z - s[2]
+        dist = torch.sqrt(dx * dx + dy * dy + dz * dz).clamp_min(1e-9)
+        radial = (dx, dy, dz)[axis] / dist
+        out += emitter_speed * radial * torch.exp(-(dist * dist) / (2.0 * emitter_radius_cells**2))
+        f.data.copy_(out.to(torch.float32))

@@ -1,0 +1,152 @@
+"""The mixed-precision pressure solve (BF16 / FP16 multigrid storage, FP32 iterative refinement, FP64 norms and
+coarsest level) against the FP64 oracle: it reaches the target relative residual, and its pressure and
+post-projection divergence match the FP64 solution as closely as the plain FP32 solve at the same residual.
+
+The target is max(1e-6, 2 x the FP32 floor), the floor being the relative residual of the FP64 oracle's
+solution rounded to FP32: no solve that keeps the pressure in FP32 gets below it (1.8e-6 on this grid)."""
+
+import numpy as np
+import pytest
+
+from support import DEFAULT_FACES, gpu_grid, oracle_grid
+
+pytestmark = pytest.mark.gpu
+
+CELLS = (48, 64, 40)
+TOLERANCE = 1e-6
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_storage_conversion_rounds_to_nearest_even(device, storage):
+    import torch
+
+    from plume_amd._lib import call, ptr, stream
+    from plume_amd.sim.mixed import DTYPES, STORAGE
+
+    g = torch.Generator().manual_seed(5)
+    x = (torch.randn(100_003, generator=g) * 3.0).to(device)
+    out = torch.empty(x.shape, dtype=DTYPES[storage], device=device)
+    for scale in (1.0, 0.25, 8.0):
+        call("plm_to_storage", x.numel(), ptr(x), scale, STORAGE[storage], out.data_ptr(), stream())
+        expected = (x * scale).to(DTYPES[storage])
+        assert torch.equal(out.view(torch.int16), expected.view(torch.int16)), scale
+    y = torch.randn(x.shape, generator=g).to(device)
+    expected = y + out.float() * 0.125
+    call("plm_add_storage", y.numel(), ptr(y), out.data_ptr(), STORAGE[storage], 0.125, stream())
+    assert torch.equal(y, expected)
+
+
+def divergent_pair(device, oracle64):
+    from oracle.sim import compute_divergence as oracle_divergence
+    from plume_amd.sim.mixed import divergent_velocity
+    from plume_amd.sim.pressure import compute_divergence, zero_closed_faces
+
+    gpu = gpu_grid(CELLS, 0.01, DEFAULT_FACES, device=device)
+    divergent_velocity(gpu, seed=3, blobs=16, emitter_radius_cells=4.0)
+    zero_closed_faces(gpu)
+    compute_divergence(gpu)
+    orc = oracle_grid(oracle64, CELLS, 0.01, DEFAULT_FACES)
+    for f, g in zip(orc.velocity_fields(), gpu.velocity_fields()):
+        f.data[...] = to_np(g.data).astype(np.float64)
+    oracle_divergence(orc)
+    orc.divergence[...] = to_np(gpu.divergence).astype(np.float64)  # the same FP32 right-hand side
+    return gpu, orc
+
+
+def fp64_solution(orc):
+    from oracle.sim import Multigrid, poisson_residual
+
+    solve = Multigrid(orc, 1, 2)
+    for _ in range(60):
+        solve(orc)
+        if poisson_residual(orc) < 1e-13:
+            break
+    return orc.pressure.copy()
+
+
+def post_projection_divergence(gpu, velocity0):
+    import torch
+
+    from plume_amd.sim.pressure import compute_divergence, subtract_gradient
+
+    rhs = gpu.divergence.clone()
+    for f, v0 in zip(gpu.velocity_fields(), velocity0):
+        f.data.copy_(v0)
+    subtract_gradient(gpu)
+    compute_divergence(gpu)
+    ratio = float(torch.linalg.vector_norm(gpu.divergence.double()) / torch.linalg.vector_norm(rhs.double()))
+    gpu.divergence.copy_(rhs)
+    return ratio
+
+
+def fp32_floor(gpu, q64) -> float:
+    import torch
+
+    from plume_amd.sim.multigrid import relative_residual
+
+    gpu.pressure.copy_(torch.from_numpy(q64.astype(np.float32)))
+    return relative_residual(gpu.info, gpu.pressure, gpu.divergence)
+
+
+def test_mixed_precision_solves_match_the_fp64_solution(device, oracle64):
+    from plume_amd.sim.mixed import MixedPrecisionMultigrid
+    from plume_amd.sim.multigrid import Multigrid, relative_residual
+
+    gpu, orc = divergent_pair(device, oracle64)
+    q64 = fp64_solution(orc)
+    floor = fp32_floor(gpu, q64)
+    assert 1e-7 < floor < 1e-5, floor
+    target = max(TOLERANCE, 2.0 * floor)
+    velocity0 = [f.data.clone() for f in gpu.velocity_fields()]
+    errors, divergences = {}, {}
+    for storage in ("f32-plain", "f32", "bf16", "f16"):
+        gpu.pressure.zero_()
+        if storage == "f32-plain":
+            solve = Multigrid(gpu, 2, 2, tolerance=target, max_cycles=60)
+        else:
+            solve = MixedPrecisionMultigrid(gpu, storage, tolerance=target)
+        solve(gpu)
+        rel = relative_residual(gpu.info, gpu.pressure, gpu.divergence)
+        assert rel <= target, (storage, rel, target)
+        if storage != "f32-plain":
+            assert solve.log.residuals[-1] == pytest.approx(rel, rel=1e-6)
+            assert solve.log.rounds <= 10 and not solve.log.stalled, (storage, solve.log.residuals)
+        q = to_np(gpu.pressure).astype(np.float64)
+        errors[storage] = np.linalg.norm(q - q64) / np.linalg.norm(q64)
+        divergences[storage] = post_projection_divergence(gpu, velocity0)
+    print("pressure error vs FP64:", errors, "post-projection divergence:", divergences)
+    for storage, err in errors.items():
+        # a 1e-6 residual bounds the error through the operator's conditioning; every form lands within a
+        # small factor of the plain FP32 solve at the same residual
+        assert err <= max(1e-4, 10.0 * errors["f32-plain"]), (storage, errors)
+        assert divergences[storage] <= max(1e-4, 10.0 * divergences["f32-plain"]), (storage, divergences)
+
+
+def test_refinement_continues_from_a_warm_start(device, oracle64):
+    """A second solve from the converged pressure does no refinement round."""
+    from plume_amd.sim.mixed import MixedPrecisionMultigrid
+
+    gpu, _ = divergent_pair(device, oracle64)
+    solve = MixedPrecisionMultigrid(gpu, "bf16", tolerance=1e-5)
+    solve(gpu)
+    first = solve.log.rounds
+    solve(gpu)
+    assert first >= 1 and solve.log.rounds == 0
+
+
+def test_refinement_stops_at_the_fp32_floor(device, oracle64):
+    """Asked for less than the FP32 pressure can represent, the solve stops once a round no longer halves the
+    residual, at the floor's level, instead of running every round."""
+    from plume_amd.sim.mixed import MixedPrecisionMultigrid
+
+    gpu, orc = divergent_pair(device, oracle64)
+    floor = fp32_floor(gpu, fp64_solution(orc))
+    gpu.pressure.zero_()
+    solve = MixedPrecisionMultigrid(gpu, "bf16", tolerance=1e-9, max_rounds=40)
+    solve(gpu)
+    assert solve.log.stalled and solve.log.rounds < 15, solve.log.residuals
+    assert solve.log.residuals[-1] < 3.0 * floor
