@@ -33,21 +33,10 @@ CELL_M = 0.01
 
 
 def scene_table(cells=GRID, frames=96, radius_cells=12, tolerance=1e-5, cycles=2):
-    """Config 2 mapped onto recipe keys: 1 cm cells, floor closed, normalised temperature (ambient 1,
-    emitter 2) so buoyancy is beta (T - 1) - alpha s with alpha = 0.1, beta = 1; confinement 0.3."""
-    size = [n * CELL_M for n in cells]
-    return {
-        "seed": 7,
-        "domain": {"size_m": size, "cell_m": CELL_M, "ambient_k": 1.0},
-        "time": {"duration_s": frames / 24.0, "frames": frames, "substeps": 4},
-        "emitter": [{"name": "puff", "center_m": [0.0, 0.10, 0.0], "radius_m": radius_cells * CELL_M,
-                     "edge_softness": 0.6, "end_s": 0.5, "smoke_per_s": 2.0, "temperature_k": 2.0}],
-        "forces": {"gravity_mps2": 9.81, "heat_buoyancy": 1.0 / 9.81, "smoke_weight": 0.1 / 9.81,
-                   "vorticity_confinement": 0.3},
-        "solver": {"multigrid_cycles": cycles, "multigrid_sweeps": 2, "multigrid_tolerance": tolerance,
-                   "multigrid_max_cycles": 24},
-        "render": {"glow_start_k": 1.0, "opacity_per_density": 20.0, "light_dir": [0.5, 0.7, 0.5]},
-    }
+    """Config 2: the smoke puff (plume_amd.sim.scenes.puff_table) with alpha 0.1, beta 1, confinement 0.3."""
+    from plume_amd.sim.scenes import puff_table
+
+    return puff_table(cells, frames=frames, radius_cells=radius_cells, tolerance=tolerance, cycles=cycles)
 
 
 def substep_bytes(cells: int, cycles: float) -> float:
@@ -258,6 +247,56 @@ def mixed_bench(args) -> dict:
             "blobs + radial emitter source)", "forms": results}
 
 
+def ensemble_bench(args) -> dict:
+    """Batched ensemble (benchmark config 4): 32 smoke puffs with seeded sizes over 48^3, 64x96x64, 96x128x96
+    and 128x192x128, emitter radii 4-12 cells, alpha in [0.05, 0.2] and epsilon in [0, 0.5]. The 12 emitting
+    frames and one more run eagerly; then the same number of frames is timed twice from the state reached:
+    batched (one captured HIP graph launch per two substeps of all 32 members, members on 4 streams) and plain
+    (every member stepped eagerly, one after the other, as a single-grid run does)."""
+    import torch
+
+    from plume_amd.sim.device import init_device
+    from plume_amd.sim.ensemble import Ensemble, ensemble_tables
+
+    device = init_device("cuda")
+    ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device, streams=4)
+    ens.autotune()
+    cells = ens.cells
+    print(f"[bench] ensemble of {len(ens.members)}: {cells} cells", file=sys.stderr, flush=True)
+    while ens.frames_done < 13:
+        ens.run_frame(replay=False)
+    ens.run_frame(replay=True)  # captures the graph
+    torch.cuda.synchronize()
+
+    def timed(replay: bool) -> tuple[float, int]:
+        steps = 0
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(args.ensemble_frames):
+            steps += ens.run_frame(replay=replay)
+        torch.cuda.synchronize()
+        return time.perf_counter() - start, steps
+
+    batched_s, batched_steps = timed(True)
+    print(f"[bench] batched: {batched_s:.3f} s for {batched_steps} substeps", file=sys.stderr, flush=True)
+    plain_s, plain_steps = timed(False)
+    print(f"[bench] plain: {plain_s:.3f} s for {plain_steps} substeps", file=sys.stderr, flush=True)
+    residuals = [m.stepper.pressure_solve.last_relative_residual() for m in ens.members]
+    batched = cells * batched_steps / batched_s
+    plain = cells * plain_steps / plain_s
+    sizes = {}
+    for m in ens.members:
+        key = "x".join(map(str, m.grid.cells))
+        sizes[key] = sizes.get(key, 0) + 1
+    return {"metric": "ensemble cell-substeps/s, 32 uneven smoke instances (batched hipGraph vs plain eager)",
+            "members": len(ens.members), "cells": cells, "sizes": sizes,
+            "batched": {"cell_substeps_per_s": batched, "seconds": batched_s, "substeps": batched_steps},
+            "plain": {"cell_substeps_per_s": plain, "seconds": plain_s, "substeps": plain_steps},
+            "speedup": batched / plain, "frames_timed": args.ensemble_frames,
+            "max_final_relative_residual": max(residuals),
+            "data": "synthetic (seeded puff recipes; frames 13 onwards, after the emitters stop)"}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -268,6 +307,10 @@ def main() -> int:
     parser.add_argument("--tolerance", type=float, default=1e-5)
     parser.add_argument("--top-tile", type=int, default=-1, metavar="SHAPE",
                         help="pin the sweep tile shape of the 128x192x128 level instead of autotuning it")
+    parser.add_argument("--ensemble", action="store_true",
+                        help="run the 32-member batched ensemble benchmark instead and exit")
+    parser.add_argument("--ensemble-members", type=int, default=32)
+    parser.add_argument("--ensemble-frames", type=int, default=4)
     parser.add_argument("--mixed", action="store_true",
                         help="run the 256x384x256 mixed-precision projection benchmark instead and exit")
     parser.add_argument("--mixed-tolerance", type=float, default=1e-6)
@@ -285,6 +328,9 @@ def main() -> int:
 
     if args.mixed:
         print(json.dumps(mixed_bench(args)), flush=True)
+        return 0
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(args)), flush=True)
         return 0
 
     if args.copy_probe > 0:

@@ -85,6 +85,10 @@ class Multigrid:
         # [0] cycles of the last solve, [1] running total (tolerance mode)
         self.cycles_run = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._predicted = 1
+        self.max_cycles_seen = 0
+        # Set for graph capture: every tolerance-mode solve then launches exactly this many gated cycles and
+        # reads nothing back (cycles past convergence exit on the device gate).
+        self.graph_cycles = None
 
     def _pointers(self):
         array = ctypes.c_void_p * len(self.levels)
@@ -97,6 +101,9 @@ class Multigrid:
         if self.tolerance <= 0.0:
             self._cycles(self.cycles, resume=False)
             return
+        if self.graph_cycles is not None:
+            self._cycles(min(self.graph_cycles, self.max_cycles), resume=False)
+            return
         # Tolerance mode: run the cycle count the last solve needed (cycles past convergence exit at once
         # on the device gate), then add single cycles while the gate says the target is not met. One host
         # read per solve in the usual case, instead of a launch per cycle up to the cap.
@@ -107,6 +114,7 @@ class Multigrid:
             self._cycles(1, resume=True)
             ran += 1
         self._predicted = max(1, int(self.cycles_run[0].item()))
+        self.max_cycles_seen = max(self.max_cycles_seen, self._predicted)
 
     def _cycles(self, count: int, resume: bool) -> None:
         q, rhs, residual = self._pointers()
