@@ -273,6 +273,53 @@ int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage
 /* x = x + float(e) * inv_scale. */
 int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, float inv_scale, void *stream);
 
+/* ------------------------------------------------------------------ adjoints (no reference counterpart: the
+   reference runs no backward pass, sim/device.py:28). Reverse-mode derivatives of the differentiable substep
+   (emission by per-cell sources, buoyancy, confinement, MacCormack advection, projection) and of the ray
+   march's heat and thickness channels. Each *_bar array is an adjoint of the same shape as its field; unless
+   stated otherwise the entry points add into them. */
+
+/* buoyancy_kernel (sim/forces.py:13-31): smoke_bar and temperature_bar += the adjoint of v's update given v_bar;
+   weight_terms (v faces) = v_bar * d v / d smoke_weight, for the caller's sum (the forward's v passes through). */
+int plm_adj_buoyancy(const plm_grid_info *info, const float *v_bar, const float *smoke, float gravity,
+                     float heat_buoyancy, float smoke_weight, float ambient_k, float dt, float *smoke_bar,
+                     float *temperature_bar, float *weight_terms, void *stream);
+/* VorticityConfinement.apply (sim/vorticity.py:95-106) at the forward input velocity u, v, w: the face
+   adjoints (in: of the output, out: of the input). curl (vec3), push_bar (vec3), curl_bar (vec3), spin_bar
+   (cells) and center_bar (vec3) are scratch. */
+int plm_adj_confinement(const plm_grid_info *info, const float *u, const float *v, const float *w, float strength,
+                        float dt, float *u_bar, float *v_bar, float *w_bar, float *curl, float *push_bar,
+                        float *curl_bar, float *spin_bar, float *center_bar, void *stream);
+/* MacCormack of u, v, w by the old velocity (sim/stepper.py:46-52, sim/advect.py:24-68): next_*_bar in,
+   *_bar (zeroed by the caller) out, through both the advected fields and the velocity that traces them.
+   hat_* and hat_*_bar are scratch of the face shapes; dto = dt / cell_m. */
+int plm_adj_advect_velocity(const plm_grid_info *info, const float *u, const float *v, const float *w, float *hat_u,
+                            float *hat_v, float *hat_w, const float *next_u_bar, const float *next_v_bar,
+                            const float *next_w_bar, float *u_bar, float *v_bar, float *w_bar, float *hat_u_bar,
+                            float *hat_v_bar, float *hat_w_bar, float dto, void *stream);
+/* MacCormack of two cell-centred fields with outside values (smoke, temperature) by u, v, w: a_bar and b_bar
+   (zeroed by the caller) from the next adjoints; u_bar, v_bar, w_bar accumulate the velocity's part. */
+int plm_adj_advect_scalars(const plm_grid_info *info, const float *u, const float *v, const float *w,
+                           const float *a_data, const float *b_data, float a_outside, float b_outside, float *a_hat,
+                           float *b_hat, const float *a_next_bar, const float *b_next_bar, float *a_bar,
+                           float *b_bar, float *u_bar, float *v_bar, float *w_bar, float *a_hat_bar,
+                           float *b_hat_bar, float dto, void *stream);
+/* Projection transposes (sim/pressure.py:95-121, 144-154): rhs = G^T f_bar (cells, written), and
+   f_bar = Z (f_bar - D^T y) (closed faces end at 0). With y the Poisson solve of rhs, the two give the
+   transpose of the projection. */
+int plm_adj_gradient_t(const plm_grid_info *info, const float *u_bar, const float *v_bar, const float *w_bar,
+                       float *rhs, void *stream);
+int plm_adj_divergence_t(const plm_grid_info *info, const float *y, float *u_bar, float *v_bar, float *w_bar,
+                         void *stream);
+/* raymarch_kernel's heat and thickness (render/raymarch.py:86-140), orthographic: smoke_bar and
+   temperature_bar += the adjoint of image_bar (height, width, 5; channels 0 and 1 are read). max_steps as in
+   plm_raymarch (0: half-cell steps). record: scratch of height * width * stride * 4 floats, stride from
+   plm_adj_raymarch_record_stride. */
+int plm_adj_raymarch_record_stride(const plm_grid_info *info, int32_t max_steps);
+int plm_adj_raymarch(const plm_grid_info *info, const float *smoke, const float *temperature, const plm_view *view,
+                     const plm_look *look, const float *image_bar, int32_t height, int32_t width, int32_t max_steps,
+                     float *record, float *smoke_bar, float *temperature_bar, void *stream);
+
 /* ------------------------------------------------------------------ budget: sim/reduce.py, conserve.py */
 
 /* line_sums_kernel, sim/reduce.py:12-25: lines[3][ny][nz] = per-(j,k) sums over i, sequential FP32. */

@@ -235,8 +235,8 @@ class Oracle:
     def gradient(self, src, gradient):
         self.lib.orc_gradient(self._shape(src), self._p(src), self._p(gradient))
 
-    def raymarch(self, info, smoke, temperature, gradient, light, view, look, image):
+    def raymarch(self, info, smoke, temperature, gradient, light, view, look, image, max_steps=0):
         height, width = image.shape[:2]
         self.lib.orc_raymarch(ctypes.byref(info), self._p(smoke), self._p(temperature), self._p(gradient),
                               self._shape(gradient), self._p(light), self._shape(light), ctypes.byref(view),
-                              ctypes.byref(look), self._p(image), int(height), int(width))
+                              ctypes.byref(look), self._p(image), int(height), int(width), int(max_steps))

@@ -921,10 +921,12 @@ void orc_gradient(const int *n, const REAL *src, REAL *gradient) {
             }
 }
 
-/* raymarch_kernel, render/raymarch.py:72-140 (pixel_ray at :72-83, FAR_M = 1e4 at :21) */
+/* raymarch_kernel, render/raymarch.py:72-140 (pixel_ray at :72-83, FAR_M = 1e4 at :21). max_steps > 0 fixes
+   every ray at that many samples (the fixed-count mode of the GPU march); 0 is the reference's half-cell
+   stepping. */
 void orc_raymarch(const orc_info *info, const REAL *smoke, const REAL *temperature, const REAL *gradient,
                   const int *ng, const REAL *light, const int *nl, const orc_view *view, const orc_look *look,
-                  REAL *image, int height, int width) {
+                  REAL *image, int height, int width, int max_steps) {
     const int *n = info->dims;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int py = 0; py < height; py++)
@@ -955,6 +957,7 @@ void orc_raymarch(const orc_info *info, const REAL *smoke, const REAL *temperatu
             if (span[1] > span[0]) {
                 REAL length = span[1] - span[0];
                 int steps = (int)CEIL(length / (R(0.5) * info->cell_m));
+                if (max_steps > 0) steps = max_steps;
                 REAL step_m = length / (REAL)steps;
                 for (int s = 0; s < steps; s++) {
                     REAL t = span[0] + ((REAL)s + R(0.5)) * step_m;

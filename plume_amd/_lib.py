@@ -105,6 +105,14 @@ SIGNATURES = {
     "plm_vcycle_lp": [_I, _P, _I, _P, _P, _P, _I, _I, _I, _P],
     "plm_to_storage": [_L, _P, _F, _I, _P, _P],
     "plm_add_storage": [_L, _P, _P, _I, _F, _P],
+    "plm_adj_buoyancy": [_INFO, _P, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P],
+    "plm_adj_confinement": [_INFO, _P, _P, _P, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "plm_adj_advect_velocity": [_INFO] + [_P] * 15 + [_F, _P],
+    "plm_adj_advect_scalars": [_INFO, _P, _P, _P, _P, _P, _F, _F] + [_P] * 11 + [_F, _P],
+    "plm_adj_gradient_t": [_INFO, _P, _P, _P, _P, _P],
+    "plm_adj_divergence_t": [_INFO, _P, _P, _P, _P, _P],
+    "plm_adj_raymarch": [_INFO, _P, _P, ctypes.POINTER(ViewParams), ctypes.POINTER(LookParams), _P, _I, _I, _I, _P,
+                         _P, _P, _P],
     "plm_line_sums": [_P, _P, _P, _P, _P, _P],
     "plm_smoke_budget": [_P, _P, _P, _P, _P, _P, _P],
     "plm_scale": [_L, _P, _F, _P],
@@ -129,6 +137,8 @@ def _load() -> ctypes.CDLL:
     lib.plm_abi_version.restype = ctypes.c_int
     lib.plm_norm_scratch_doubles.restype = ctypes.c_int64
     lib.plm_norm_scratch_doubles.argtypes = [_INFO]
+    lib.plm_adj_raymarch_record_stride.restype = ctypes.c_int32
+    lib.plm_adj_raymarch_record_stride.argtypes = [_INFO, _I]
     lib.plm_sweep_tile_shapes.restype = ctypes.c_int32
     lib.plm_sweep_tile_shapes.argtypes = [_P, _P, _P, _I]
     for name, argtypes in SIGNATURES.items():

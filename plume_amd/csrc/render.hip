@@ -10,29 +10,6 @@
 namespace plm {
 namespace {
 
-constexpr float FAR_M = 1.0e4f;  // render/raymarch.py:21
-
-// slab_range, render/raymarch.py:55-69
-__device__ __forceinline__ void slab_range(const float o[3], const float d[3], const float lo[3], const float hi[3],
-                                           float &t_enter, float &t_exit) {
-    t_enter = -1.0e30f;
-    t_exit = 1.0e30f;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        if (fabsf(d[a]) < 1.0e-12f) {
-            if (o[a] < lo[a] || o[a] > hi[a]) {
-                t_enter = 1.0f;
-                t_exit = 0.0f;
-                return;
-            }
-        } else {
-            const float t0 = (lo[a] - o[a]) / d[a];
-            const float t1 = (hi[a] - o[a]) / d[a];
-            t_enter = fmax_(t_enter, fmin_(t0, t1));
-            t_exit = fmin_(t_exit, fmax_(t0, t1));
-        }
-    }
-}
 
 // The corner offsets and weights of sample_index_space (sim/sample.py:10-29) at one point.
 struct Trilinear {
