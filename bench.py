@@ -134,117 +134,35 @@ MIXED_GRID = (256, 384, 256)
 
 def mixed_bench(args) -> dict:
     """Mixed-precision pressure projection at 256x384x256 (benchmark config 5): the seeded divergent field
-    (64 Gaussian vortex blobs plus a radial emitter source), solved to 1e-6 relative residual from q = 0 by
-    the plain FP32 multigrid (tolerance mode) and by iterative refinement around BF16 and FP16 multigrid
-    with an FP64 coarsest 8x12x8 level. Solve time, FP64 relative residual, pressure error and the
-    post-projection divergence against an FP64 oracle solution (CPU, C oracle in double)."""
-    import numpy as np
-    import torch
-
-    from plume_amd.recipe.derived import derive
-    from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
+    (64 Gaussian vortex blobs plus a radial emitter source), solved from q = 0 to a common relative residual
+    by the plain FP32 multigrid (tolerance mode) and by FP32 iterative refinement around BF16 and FP16
+    multigrid with an FP64 coarsest 8x12x8 level. The common target is max(tolerance, 2 x the FP32 floor),
+    the floor being where plain FP32 V-cycles stall (below it no FP32 pressure gets; the FP64 oracle's
+    solution rounded to FP32 sits at the same level). Solve time, FP64 relative residual and
+    post-projection divergence per form; the errors against an FP64 oracle solution are
+    tests/report_mixed.py."""
     from plume_amd.sim.device import init_device
-    from plume_amd.sim.grid import make_grid
-    from plume_amd.sim.mixed import MixedPrecisionMultigrid, divergent_velocity
-    from plume_amd.sim.multigrid import Multigrid
-    from plume_amd.sim.pressure import compute_divergence, subtract_gradient, zero_closed_faces
+    from plume_amd.sim.mixed import divergent_problem, fp32_floor, solve_forms
 
     device = init_device("cuda")
-    recipe = fill_dependent_defaults(parse_recipe(scene_table(cells=MIXED_GRID)), "bench")
-    derived = derive(recipe)
-    grid = make_grid(derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k, device)
-    cells = grid.pressure.numel()
-    divergent_velocity(grid, seed=20261208)
-    zero_closed_faces(grid)
-    compute_divergence(grid)
-    velocity0 = [f.data.clone() for f in grid.velocity_fields()]
-    div0 = grid.divergence.clone()
-    div0_norm = float(torch.linalg.vector_norm(div0.double()))
-    from plume_amd.sim.multigrid import relative_residual
-
-    results, q64, floor = {}, None, None
-    if args.mixed_oracle_cycles > 0:
-        # the FP64 oracle's solution (C oracle in double on the host), the FP32 floor it implies, and its own
-        # post-projection divergence
-        os.environ.setdefault("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16)))
-        from oracle import Oracle
-        from oracle.sim import Multigrid as OracleMultigrid, compute_divergence as oracle_divergence
-        from oracle.sim import make_grid as oracle_make_grid, poisson_residual
-
-        o = Oracle("f64")
-        ogrid = oracle_make_grid(o, derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k)
-        rhs64 = div0.cpu().numpy().astype(np.float64)
-        ogrid.divergence[...] = rhs64
-        solve64 = OracleMultigrid(ogrid, 1, 2)
-        start = time.perf_counter()
-        cycles64, residual64 = 0, 1.0
-        for _ in range(args.mixed_oracle_cycles):
-            solve64(ogrid)
-            cycles64 += 1
-            residual64 = poisson_residual(ogrid)
-            print(f"[bench] FP64 oracle cycle {cycles64}: residual {residual64:.3g}", file=sys.stderr, flush=True)
-            if residual64 <= 1e-13:
-                break
-        oracle_seconds = time.perf_counter() - start
-        q64 = ogrid.pressure.copy()
-        grid.pressure.copy_(torch.from_numpy(q64.astype(np.float32)))
-        floor = relative_residual(grid.info, grid.pressure, grid.divergence)
-        for axis, (f, v0) in enumerate(zip(ogrid.velocity_fields(), velocity0)):
-            f.data[...] = v0.cpu().numpy().astype(np.float64)
-            o.subtract_gradient(ogrid.info, f.data, q64, axis)
-        post64 = float(np.linalg.norm(oracle_divergence(ogrid))) / div0_norm
-        results["fp64_oracle"] = {"v_cycles": cycles64, "relative_residual": residual64,
-                                  "post_projection_divergence": post64, "cpu_seconds": oracle_seconds,
-                                  "fp32_floor": floor}
-        del ogrid, solve64
-    target = max(args.mixed_tolerance, 2.0 * floor) if floor is not None else args.mixed_tolerance
-    forms = {
-        "f32": lambda: Multigrid(grid, 2, 2, tolerance=target, max_cycles=80),
-        "bf16": lambda: MixedPrecisionMultigrid(grid, "bf16", tolerance=target, max_levels=6),
-        "f16": lambda: MixedPrecisionMultigrid(grid, "f16", tolerance=target, max_levels=6),
-    }
-    for name, make in forms.items():
-        solve = make()
-        if name == "f32":
-            solve.autotune()
-        for attempt in range(2):  # the first solve warms the code paths
-            grid.pressure.zero_()
-            torch.cuda.synchronize()
-            start = time.perf_counter()
-            solve(grid)
-            torch.cuda.synchronize()
-            seconds = time.perf_counter() - start
-        rel = relative_residual(grid.info, grid.pressure, grid.divergence)
-        q = grid.pressure.clone()
-        for f, v0 in zip(grid.velocity_fields(), velocity0):
-            f.data.copy_(v0)
-        subtract_gradient(grid)
-        compute_divergence(grid)
-        post = float(torch.linalg.vector_norm(grid.divergence.double())) / div0_norm
-        grid.divergence.copy_(div0)
-        entry = {"seconds": seconds, "cell_solves_per_s": cells / seconds, "relative_residual": rel,
-                 "post_projection_divergence": post}
-        if q64 is not None:
-            entry["pressure_error_vs_fp64"] = float(np.linalg.norm(q.cpu().numpy().astype(np.float64) - q64) /
-                                                    np.linalg.norm(q64))
-        if name == "f32":
-            entry["v_cycles"] = solve.last_cycles()
-        else:
-            entry["refinement_rounds"] = solve.log.rounds
-            entry["inner_v_cycles"] = solve.log.rounds * solve.inner_cycles
-            entry["stalled"] = solve.log.stalled
+    problem = divergent_problem(MIXED_GRID, device)
+    floor = fp32_floor(problem.grid)
+    target = max(args.mixed_tolerance, 2.0 * floor)
+    print(f"[bench] mixed: FP32 floor {floor:.3g}, target {target:.3g}", file=sys.stderr, flush=True)
+    results = {}
+    for name, (entry, _) in solve_forms(problem, target).items():
         results[name] = entry
-        print(f"[bench] mixed {name}: {seconds * 1e3:.2f} ms, residual {rel:.3g}, post div {post:.3g}",
-              file=sys.stderr, flush=True)
+        print(f"[bench] mixed {name}: {entry['seconds'] * 1e3:.2f} ms, residual {entry['relative_residual']:.3g}, "
+              f"post div {entry['post_projection_divergence']:.3g}", file=sys.stderr, flush=True)
     plain = results["f32"]["cell_solves_per_s"]
     for name in ("bf16", "f16"):
         results[name]["speedup_vs_f32"] = results[name]["cell_solves_per_s"] / plain
     return {"metric": "pressure-solve cells/s at 256x384x256 to a common relative residual (FP32 vs BF16/FP16 "
                       "multigrid with FP32 iterative refinement)", "grid": list(MIXED_GRID),
-            "tolerance": args.mixed_tolerance, "target": target,
-            "target_rule": "max(tolerance, 2 x FP32 floor); the floor is the relative residual of the FP64 "
-                           "oracle solution rounded to FP32", "data": "synthetic (seeded divergent field: 64 Gaussian vortex "
-            "blobs + radial emitter source)", "forms": results}
+            "tolerance": args.mixed_tolerance, "fp32_floor": floor, "target": target,
+            "target_rule": "max(tolerance, 2 x FP32 floor); floor = where plain FP32 V-cycles stall",
+            "data": "synthetic (seeded divergent field: 64 Gaussian vortex blobs + radial emitter source)",
+            "forms": results}
 
 
 def ensemble_bench(args) -> dict:
@@ -297,6 +215,80 @@ def ensemble_bench(args) -> dict:
             "data": "synthetic (seeded puff recipes; frames 13 onwards, after the emitters stop)"}
 
 
+def adjoint_bench(args) -> dict:
+    """Differentiable rollout (benchmark config 3): the puff at 64x96x64 from rest, 32 substeps, the (heat,
+    thickness) images of 128x128 pixels after every 8 substeps (the state is stored only there), the L2 loss
+    against the images of the same rollout with smoke, heat and lift sources 10% stronger; gradients with
+    respect to the three per-cell source fields and alpha. Times the forward rollout alone and forward plus
+    backward, and checks the gradient along each source's own direction and alpha against a central difference
+    of the same FP32 rollout at h = 1e-3 (the FP64 comparison is tests/report_adjoint.py)."""
+    import torch
+
+    from plume_amd.sim.adjoint import Sources, rollout
+    from plume_amd.sim.device import init_device
+    from plume_amd.sim.scenes import differentiable_puff
+
+    device = init_device("cuda")
+    cells, substeps, checkpoint, alpha0 = (64, 96, 64), 32, 8, 0.1
+    recipe, derived, sim, sources = differentiable_puff(device, cells, pixels=128, march_steps=192)
+    sim.solve.autotune()
+    with torch.no_grad():
+        target = rollout(Sources(*(1.1 * t for t in sources.tensors())), torch.tensor(alpha0, dtype=torch.float64),
+                         sim, substeps, checkpoint).clone()
+    target64 = target.double()
+
+    def loss_at(srcs, a) -> float:
+        with torch.no_grad():
+            images = rollout(Sources(*srcs), torch.tensor(a, dtype=torch.float64), sim, substeps, checkpoint)
+            return float(((images.double() - target64) ** 2).sum())
+
+    def forward_backward():
+        params = [t.clone().requires_grad_(True) for t in sources.tensors()]
+        alpha = torch.tensor(alpha0, dtype=torch.float64, requires_grad=True)
+        images = rollout(Sources(*params), alpha, sim, substeps, checkpoint)
+        ((images - target) ** 2).sum().backward()
+        return params, alpha
+
+    def timed(fn, repeats):
+        fn()
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(repeats):
+            result = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - start) / repeats, result
+
+    forward_s, _ = timed(lambda: loss_at(list(sources.tensors()), alpha0), args.adjoint_repeats)
+    both_s, (params, alpha) = timed(forward_backward, args.adjoint_repeats)
+    print(f"[bench] adjoint: forward {forward_s * 1e3:.1f} ms, forward + backward {both_s * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    h = 1e-3
+    checks = {}
+    base = list(sources.tensors())
+    for k, name in enumerate(("smoke", "temperature", "lift", "alpha")):
+        if k < 3:
+            adj = float((params[k].grad.double() * base[k].double()).sum())
+            plus = [t * (1 + h) if i == k else t for i, t in enumerate(base)]
+            minus = [t * (1 - h) if i == k else t for i, t in enumerate(base)]
+            fd = (loss_at(plus, alpha0) - loss_at(minus, alpha0)) / (2 * h)
+        else:
+            adj = float(alpha.grad) * alpha0
+            fd = (loss_at(base, alpha0 * (1 + h)) - loss_at(base, alpha0 * (1 - h))) / (2 * h)
+        checks[name] = {"adjoint": adj, "fp32_central_difference": fd, "relative_error": abs(adj - fd) / abs(fd)}
+        print(f"[bench] d loss along {name}: adjoint {adj:.6e}, FD {fd:.6e}", file=sys.stderr, flush=True)
+    n = cells[0] * cells[1] * cells[2]
+    return {"metric": "differentiable rollout at 64x96x64: 32 substeps + 4 images of 128x128, forward and "
+                      "forward+backward (gradients w.r.t. per-cell sources and alpha)",
+            "cells": n, "substeps": substeps, "checkpoint": checkpoint, "pixels": 128, "march_steps": 192,
+            "forward_ms": forward_s * 1e3, "forward_backward_ms": both_s * 1e3,
+            "backward_over_forward": (both_s - forward_s) / forward_s,
+            "forward_cell_substeps_per_s": n * substeps / forward_s,
+            "gradient_cell_substeps_per_s": n * substeps / both_s,
+            "gradient_check_h": h, "gradient_check": checks,
+            "max_relative_error": max(c["relative_error"] for c in checks.values()),
+            "data": "synthetic (seeded puff recipe, Gaussian sphere sources, from rest)"}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -314,8 +306,9 @@ def main() -> int:
     parser.add_argument("--mixed", action="store_true",
                         help="run the 256x384x256 mixed-precision projection benchmark instead and exit")
     parser.add_argument("--mixed-tolerance", type=float, default=1e-6)
-    parser.add_argument("--mixed-oracle-cycles", type=int, default=40,
-                        help="V-cycles cap of the FP64 CPU oracle solution (0 = skip the error report)")
+    parser.add_argument("--adjoint", action="store_true",
+                        help="run the 64x96x64 differentiable-rollout benchmark instead and exit")
+    parser.add_argument("--adjoint-repeats", type=int, default=3)
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run the streaming-copy probe over MIB MiB (HBM counter calibration) and exit")
     args = parser.parse_args()
@@ -331,6 +324,9 @@ def main() -> int:
         return 0
     if args.ensemble:
         print(json.dumps(ensemble_bench(args)), flush=True)
+        return 0
+    if args.adjoint:
+        print(json.dumps(adjoint_bench(args)), flush=True)
         return 0
 
     if args.copy_probe > 0:

@@ -60,7 +60,14 @@ typedef struct plm_emitter {
     int32_t stream;
 } plm_emitter;
 
-/* ViewParams, render/raymarch.py:30-42. */
+/* ViewParams, render/raymarch.py:30-42. perspective: PLM_VIEW_ORTHO (0) and PLM_VIEW_PERSPECTIVE (1) as the
+   reference; PLM_VIEW_ORTHO_NEAR (2, no reference counterpart) is the orthographic view with every ray started
+   on the camera plane through the world origin instead of FAR_M = 1e4 m toward the camera. The rays and their
+   samples are the same lines and points; FP32 positions then carry no rounding of a 1e4 m offset (~5e-4 m),
+   which the differentiable rollout needs on centimetre cells. */
+#define PLM_VIEW_ORTHO 0
+#define PLM_VIEW_PERSPECTIVE 1
+#define PLM_VIEW_ORTHO_NEAR 2
 typedef struct plm_view {
     float right[3];
     float up[3];

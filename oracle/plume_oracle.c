@@ -935,7 +935,9 @@ void orc_raymarch(const orc_info *info, const REAL *smoke, const REAL *temperatu
             REAL plane_y = view->top_m - ((REAL)py + R(0.5)) * view->pixel_m;
             REAL origin[3], dir[3];
             for (int a = 0; a < 3; a++) {
-                origin[a] = view->right[a] * plane_x + view->up[a] * plane_y + view->toward[a] * R(1.0e4);
+                /* perspective 2: the orthographic ray started on the camera plane (no FAR_M offset) */
+                origin[a] = view->right[a] * plane_x + view->up[a] * plane_y +
+                            view->toward[a] * (view->perspective == 2 ? R(0.0) : R(1.0e4));
                 dir[a] = -view->toward[a];
             }
             if (view->perspective == 1) {

@@ -150,6 +150,9 @@ def _load() -> ctypes.CDLL:
 
 LIB = _load()
 ABI_VERSION = 2
+
+# plm_view.perspective (include/plume_hip.h)
+VIEW_ORTHO, VIEW_PERSPECTIVE, VIEW_ORTHO_NEAR = 0, 1, 2
 if LIB.plm_abi_version() != ABI_VERSION:
     raise ImportError(f"{LIBRARY} has ABI {LIB.plm_abi_version()}, expected {ABI_VERSION}: rebuild it")
 

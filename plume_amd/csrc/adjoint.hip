@@ -565,8 +565,9 @@ __device__ __forceinline__ bool pixel_ray(const plm_grid_info &g, const plm_view
                                           float origin[3], float dir[3], float &span0, float &span1) {
     const float plane_x = view.left_m + ((float)px + 0.5f) * view.pixel_m;
     const float plane_y = view.top_m - ((float)py + 0.5f) * view.pixel_m;
+    const float start_m = ortho_start_m(view);
     for (int a = 0; a < 3; a++) {
-        origin[a] = view.right[a] * plane_x + view.up[a] * plane_y + view.toward[a] * FAR_M;
+        origin[a] = view.right[a] * plane_x + view.up[a] * plane_y + view.toward[a] * start_m;
         dir[a] = -view.toward[a];
     }
     float box_max[3];
@@ -775,7 +776,8 @@ int plm_adj_raymarch(const plm_grid_info *info, const float *smoke, const float 
     PLM_REQUIRE(smoke && temperature && view && look && image_bar && record && smoke_bar && temperature_bar,
                 "plm_adj_raymarch: NULL argument");
     PLM_REQUIRE(height > 0 && width > 0 && max_steps >= 0, "plm_adj_raymarch: bad image size or step count");
-    PLM_REQUIRE(view->perspective == 0, "plm_adj_raymarch: orthographic views only");
+    PLM_REQUIRE(view->perspective == PLM_VIEW_ORTHO || view->perspective == PLM_VIEW_ORTHO_NEAR,
+                "plm_adj_raymarch: orthographic views only");
     const int stride = plm_adj_raymarch_record_stride(info, max_steps);
     const dim3 grid((width + 15) / 16, (height + 15) / 16);
     k_adj_raymarch<<<grid, 256, 0, as_stream(stream)>>>(*info, smoke, temperature, *view, *look, image_bar, height,

@@ -201,6 +201,12 @@ __device__ __forceinline__ float sample_field(const float *__restrict__ f, Dims 
 
 constexpr float FAR_M = 1.0e4f;  // render/raymarch.py:21
 
+// How far toward the camera an orthographic ray starts: FAR_M (the reference), or on the camera plane
+// through the world origin (PLM_VIEW_ORTHO_NEAR; the same line, better conditioned in FP32)
+__host__ __device__ inline float ortho_start_m(const plm_view &view) {
+    return view.perspective == PLM_VIEW_ORTHO_NEAR ? 0.0f : FAR_M;
+}
+
 // slab_range, render/raymarch.py:55-69
 __device__ __forceinline__ void slab_range(const float o[3], const float d[3], const float lo[3], const float hi[3],
                                            float &t_enter, float &t_exit) {

@@ -131,12 +131,13 @@ __global__ __launch_bounds__(256) void k_raymarch(plm_grid_info g, const float *
     const float plane_x = view.left_m + ((float)px + 0.5f) * view.pixel_m;
     const float plane_y = view.top_m - ((float)py + 0.5f) * view.pixel_m;
     float origin[3], dir[3];
+    const float start_m = ortho_start_m(view);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        origin[a] = view.right[a] * plane_x + view.up[a] * plane_y + view.toward[a] * FAR_M;
+        origin[a] = view.right[a] * plane_x + view.up[a] * plane_y + view.toward[a] * start_m;
         dir[a] = -view.toward[a];
     }
-    if (view.perspective == 1) {
+    if (view.perspective == PLM_VIEW_PERSPECTIVE) {
         float d[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_raymarch(plm_grid_info g, const float *
     for (int a = 0; a < 3; a++) box_max[a] = g.origin_m[a] + (float)g.dims[a] * g.cell_m;
     float span0, span1;
     slab_range(origin, dir, g.origin_m, box_max, span0, span1);
-    if (view.perspective == 1) span0 = fmax_(span0, 0.0f);
+    if (view.perspective == PLM_VIEW_PERSPECTIVE) span0 = fmax_(span0, 0.0f);
 
     float tr = 1.0f, heat = 0.0f, shade_sum = 0.0f;
     float ns[3] = {0.0f, 0.0f, 0.0f};

@@ -29,7 +29,7 @@ from types import SimpleNamespace
 import numpy as np
 import torch
 
-from plume_amd._lib import LIB, GridInfo, LookParams, ViewParams, call, float3, ptr, stream
+from plume_amd._lib import LIB, VIEW_ORTHO_NEAR, GridInfo, LookParams, ViewParams, call, float3, ptr, stream
 from plume_amd.render.camera import make_camera
 from plume_amd.render.frame_renderer import look_params
 from plume_amd.render.raymarch import march
@@ -56,15 +56,21 @@ class Sources:
 
 
 def sphere_sources(grid, center_cells, radius_cells: float, smoke_per_s: float = 2.0, heat_per_s: float = 4.0,
-                   lift_mps2: float = 1.0, softness: float = 0.6) -> Sources:
-    """Sources shaped like the reference's soft sphere emitter (sim/emitters.py:41-66): strength times a
-    smoothstep coverage that falls from 1 inside to 0 at the radius."""
+                   lift_mps2: float = 1.0, softness: float = 0.6, falloff: str = "smoothstep") -> Sources:
+    """Sources on a sphere: strength times a coverage that is 0 beyond the radius. falloff "smoothstep" is the
+    reference's soft edge (sim/emitters.py:41-66), 1 inside and falling to 0 over the outer `softness` of the
+    radius; "gaussian" is exp(-d^2 / (2 (radius / 2)^2)), which has no flat top."""
+    if falloff not in ("smoothstep", "gaussian"):
+        raise ValueError(f"unknown falloff {falloff!r}")
     nx, ny, nz = grid.cells
 
     def coverage(shape, offset):
         coords = [torch.arange(shape[a], dtype=torch.float32, device=grid.device) + offset[a] for a in range(3)]
         X, Y, Z = torch.meshgrid(*coords, indexing="ij")
         d = torch.sqrt((X - center_cells[0]) ** 2 + (Y - center_cells[1]) ** 2 + (Z - center_cells[2]) ** 2)
+        if falloff == "gaussian":
+            sigma = 0.5 * radius_cells
+            return torch.where(d < radius_cells, torch.exp(-d * d / (2.0 * sigma * sigma)), torch.zeros_like(d))
         x = ((radius_cells - d) / (softness * radius_cells)).clamp(0.0, 1.0)
         return x * x * (3.0 - 2.0 * x)
 
@@ -75,7 +81,9 @@ def sphere_sources(grid, center_cells, radius_cells: float, smoke_per_s: float =
 
 def _square_view(grid, camera, pixels: int) -> ViewParams:
     """An orthographic view of `pixels` x `pixels` over the domain's projection (render/frame_renderer.py:72-83
-    with a square pixel grid)."""
+    with a square pixel grid), its rays started on the camera plane (VIEW_ORTHO_NEAR): the reference's start
+    1e4 m out rounds FP32 sample positions by ~5e-4 m, a twentieth of a centimetre cell, which the gradient
+    would inherit."""
     cell = grid.info.cell_m
     origin = np.array(grid.info.origin_m, dtype=np.float64)
     size = np.array(grid.cells, dtype=np.float64) * cell
@@ -89,7 +97,7 @@ def _square_view(grid, camera, pixels: int) -> ViewParams:
     view.left_m = float((xs.min() + xs.max()) / 2 - span / 2)
     view.top_m = float((ys.min() + ys.max()) / 2 + span / 2)
     view.pixel_m = float(pixel_m)
-    view.perspective = 0
+    view.perspective = VIEW_ORTHO_NEAR
     view.eye = float3((0.0, 0.0, 0.0))
     return view
 

@@ -22,6 +22,7 @@ at `max_levels` so that the coarsest level is the one the FP64 solve handles (8 
 
 import ctypes
 import math
+import time
 from dataclasses import dataclass, field
 
 import torch
@@ -157,3 +158,95 @@ def divergent_velocity(grid, seed: int, blobs: int = 64, emitter_radius_cells: f
         radial = (dx, dy, dz)[axis] / dist
         out += emitter_speed * radial * torch.exp(-(dist * dist) / (2.0 * emitter_radius_cells**2))
         f.data.copy_(out.to(torch.float32))
+
+
+def fp32_floor(grid, max_cycles: int = 60) -> float:
+    """The lowest FP64 relative residual the plain FP32 multigrid reaches on grid.divergence from q = 0: V-cycles
+    until one no longer cuts the residual by 10%. That stall is set by FP32 itself, the pressure's rounding
+    and the rounding of A q in the FP32 residual (terms of size q / h^2 that cancel), so no FP32 pressure
+    reports much less. Leaves grid.pressure at the stalled solution."""
+    from plume_amd.sim.multigrid import Multigrid, relative_residual
+
+    solve = Multigrid(grid, 1, 2)
+    grid.pressure.zero_()
+    best = previous = math.inf
+    for _ in range(max_cycles):
+        solve(grid)
+        residual = relative_residual(grid.info, grid.pressure, grid.divergence)
+        best = min(best, residual)
+        if residual > 0.9 * previous:
+            break
+        previous = residual
+    return best
+
+
+@dataclass
+class Problem:
+    """A projection problem: the grid (its divergence set), the divergent velocity and its divergence."""
+
+    grid: object
+    velocity: list
+    divergence: torch.Tensor
+
+
+def divergent_problem(cells, device: str, seed: int = 20261208) -> Problem:
+    """The mixed-precision projection problem on a `cells` puff grid (1 cm cells, floor closed, the rest
+    open): divergent_velocity with `seed`, closed faces zeroed, its divergence as the right-hand side."""
+    from plume_amd.recipe.derived import derive
+    from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
+    from plume_amd.sim.grid import make_grid
+    from plume_amd.sim.pressure import compute_divergence, zero_closed_faces
+    from plume_amd.sim.scenes import puff_table
+
+    recipe = fill_dependent_defaults(parse_recipe(puff_table(tuple(cells))), "mixed")
+    derived = derive(recipe)
+    grid = make_grid(derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k, device)
+    divergent_velocity(grid, seed=seed)
+    zero_closed_faces(grid)
+    compute_divergence(grid)
+    return Problem(grid, [f.data.clone() for f in grid.velocity_fields()], grid.divergence.clone())
+
+
+def solve_forms(problem: Problem, target: float, forms=("f32", "bf16", "f16"), max_levels: int = 6,
+                repeats: int = 2) -> dict:
+    """Solves the problem from q = 0 to `target` with each form: "f32" the plain FP32 multigrid in tolerance
+    mode (autotuned tiles), "bf16" / "f16" iterative refinement around that storage. The last of `repeats`
+    solves is timed (the first warms the code paths). Returns {form: (entry, pressure)} where entry holds the
+    seconds, cells per second, FP64 relative residual and post-projection divergence |div u| / |div u0|."""
+    from plume_amd.sim.multigrid import Multigrid, relative_residual
+    from plume_amd.sim.pressure import compute_divergence, subtract_gradient
+
+    grid = problem.grid
+    cells = grid.pressure.numel()
+    div0_norm = float(torch.linalg.vector_norm(problem.divergence.double()))
+    out = {}
+    for name in forms:
+        if name == "f32":
+            solve = Multigrid(grid, 2, 2, tolerance=target, max_cycles=80)
+            solve.autotune()
+        else:
+            solve = MixedPrecisionMultigrid(grid, name, tolerance=target, max_levels=max_levels)
+        for _ in range(repeats):
+            grid.pressure.zero_()
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            solve(grid)
+            torch.cuda.synchronize()
+            seconds = time.perf_counter() - start
+        residual = relative_residual(grid.info, grid.pressure, grid.divergence)
+        q = grid.pressure.clone()
+        for f, v0 in zip(grid.velocity_fields(), problem.velocity):
+            f.data.copy_(v0)
+        subtract_gradient(grid)
+        compute_divergence(grid)
+        post = float(torch.linalg.vector_norm(grid.divergence.double())) / div0_norm
+        grid.divergence.copy_(problem.divergence)
+        entry = {"seconds": seconds, "cell_solves_per_s": cells / seconds, "relative_residual": residual,
+                 "post_projection_divergence": post}
+        if name == "f32":
+            entry["v_cycles"] = solve.last_cycles()
+        else:
+            entry.update(refinement_rounds=solve.log.rounds, inner_v_cycles=solve.log.rounds * solve.inner_cycles,
+                         stalled=solve.log.stalled)
+        out[name] = (entry, q)
+    return out

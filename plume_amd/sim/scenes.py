@@ -28,3 +28,31 @@ def puff_table(cells, frames: int = 96, radius_cells: float = 12.0, alpha: float
                    "multigrid_max_cycles": max_cycles},
         "render": {"glow_start_k": 1.0, "opacity_per_density": 20.0, "light_dir": [0.5, 0.7, 0.5]},
     }
+
+
+def differentiable_recipe(cells=(64, 96, 64), radius_cells: float = 6.0, tolerance: float = 1e-5):
+    """(recipe, derived) of the differentiable-rollout scene: the puff (alpha 0.1, beta 1, confinement 0.3) on
+    `cells` with the glow starting below the ambient 1, so that the heat channel's max(T - glow, 0) has no kink
+    in still air."""
+    from plume_amd.recipe.derived import derive
+    from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
+
+    table = puff_table(cells, frames=2, radius_cells=radius_cells, tolerance=tolerance)
+    table["render"]["glow_start_k"] = 0.5
+    recipe = fill_dependent_defaults(parse_recipe(table), "differentiable")
+    return recipe, derive(recipe)
+
+
+def differentiable_puff(device: str, cells=(64, 96, 64), pixels: int = 128, march_steps: int = 192,
+                        tolerance: float = 1e-5, radius_cells: float = 6.0):
+    """The differentiable-rollout scene (differentiable_recipe) with per-cell sources of a Gaussian-falloff
+    sphere of `radius_cells` centred 10 cells above the floor (smoke 2 /s, heat 4 K/s on the normalised scale,
+    lift 1 m/s^2 on v faces), and a DiffSim rendering `pixels`^2 heat and thickness images with `march_steps`
+    samples per ray. Returns (recipe, derived, sim, sources)."""
+    from plume_amd.sim.adjoint import DiffSim, sphere_sources
+
+    recipe, derived = differentiable_recipe(cells, radius_cells, tolerance)
+    sim = DiffSim(recipe, derived, device, pixels=pixels, march_steps=march_steps, tolerance=tolerance)
+    center = (cells[0] / 2.0, 10.0, cells[2] / 2.0)
+    sources = sphere_sources(sim.grid, center, radius_cells, falloff="gaussian")
+    return recipe, derived, sim, sources

@@ -245,18 +245,18 @@ def test_raymarch_adjoint(device, oracle64):
     from plume_amd.render.raymarch import march
     from plume_amd.sim.adjoint import _square_view
 
-    # 10 m cells: the reference's rays start FAR_M = 1e4 m out, so FP32 sample positions carry ~1e-3 m of
-    # rounding that FP64 does not (the adjoint is exact for the FP32 positions); on 10 m cells that is ~1e-4
-    # of a cell. The opacity per density is scaled to keep the optical depth per step of a 1 m grid.
+    # 1 cm cells with the near-origin orthographic view of the differentiable rollout (rays started on the
+    # camera plane, not 1e4 m out, so FP32 sample positions carry no ~5e-4 m rounding); the opacity per
+    # density is that of a 1 m grid scaled to keep the optical depth per step
     rng = np.random.default_rng(6)
-    gpu = gpu_grid(CELLS, 10.0, DEFAULT_FACES, ambient=1.0, device=device)
-    orc = oracle_grid(oracle64, CELLS, 10.0, DEFAULT_FACES, ambient=1.0)
+    gpu = gpu_grid(CELLS, 0.01, DEFAULT_FACES, ambient=1.0, device=device)
+    orc = oracle_grid(oracle64, CELLS, 0.01, DEFAULT_FACES, ambient=1.0)
     s, t = smooth_field(rng, CELLS, 0.3, 0.5), smooth_field(rng, CELLS, 0.3, 1.5)
     set_gpu(gpu, {"smoke": s, "temperature": t})
     pixels, steps = 24, 40
     view = _square_view(gpu, make_camera(20.0, 30.0), pixels)
     look = LookParams()
-    look.opacity_per_density, look.glow_start_k, look.smoke_hides_heat = 0.02, 0.5, 1.0
+    look.opacity_per_density, look.glow_start_k, look.smoke_hides_heat = 20.0, 0.5, 1.0
     look.normal_fade_thickness, look.shade_layout, look.shade_fill = 0.5, 1, 0.3
     W = rng.standard_normal((pixels, pixels, 5))
     W[..., 2:] = 0.0
@@ -274,8 +274,8 @@ def test_raymarch_adjoint(device, oracle64):
     march(gpu.smoke.data, gpu.temperature.data, torch.zeros((1, 1, 1, 3), device=device),
           torch.ones((1, 1, 1), device=device), gpu.info, view, look, image, steps)
     oview = oracle64.view(tuple(view.right), tuple(view.up), tuple(view.toward), view.left_m, view.top_m, view.pixel_m,
-                          0, (0.0, 0.0, 0.0))
-    olook = oracle64.look(0.02, 0.5, 1.0, 0.5, 1, 0.3)
+                          view.perspective, (0.0, 0.0, 0.0))
+    olook = oracle64.look(20.0, 0.5, 1.0, 0.5, 1, 0.3)
     light = np.ones((1, 1, 1))
     grad = np.zeros((1, 1, 1, 3))
 
@@ -347,56 +347,11 @@ def gpu_initial(initial, device):
 
 
 def oracle_images(o, recipe, derived, sim, sources64, alpha, initial):
-    """The same rollout in FP64 on the oracle from `initial`: per-cell emission, buoyancy, confinement,
-    MacCormack of the velocity, projection with multigrid to 1e-13, MacCormack of smoke and temperature, and
-    the render."""
-    from oracle.sim import Multigrid, VorticityConfinement, advect_maccormack, apply_buoyancy, make_grid
-    from oracle.sim import poisson_residual, project
+    """The same rollout in FP64 on the oracle from `initial` (tests/adjoint_oracle.py)."""
+    from adjoint_oracle import Camera, rollout_images
 
-    grid = make_grid(o, derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k)
-    conf = VorticityConfinement(grid)
-    for name, values in initial.items():
-        getattr(grid, name).data[...] = values.astype(np.float64)
-    amb, g, dt = recipe.domain.ambient_k, recipe.forces.gravity_mps2, sim.dt
-
-    def solve(gr):
-        s = Multigrid(gr, 1, 2)
-        for _ in range(60):
-            s(gr)
-            if poisson_residual(gr) < 1e-13:
-                break
-
-    view = sim.view
-    oview = o.view(tuple(view.right), tuple(view.up), tuple(view.toward), view.left_m, view.top_m, view.pixel_m, 0,
-                   (0.0, 0.0, 0.0))
-    lk = sim.look
-    olook = o.look(lk.opacity_per_density, lk.glow_start_k, lk.smoke_hides_heat, lk.normal_fade_thickness, 1,
-                   lk.shade_fill)
-    images = []
-    for _ in range(SUBSTEPS // CHECKPOINT):
-        for _ in range(CHECKPOINT):
-            grid.smoke.data[...] += dt * sources64[0]
-            grid.temperature.data[...] += dt * sources64[1]
-            grid.v.data[...] += dt * sources64[2]
-            apply_buoyancy(grid, g, recipe.forces.heat_buoyancy, alpha / g, amb, dt)
-            conf.apply(grid, recipe.forces.vorticity_confinement, dt)
-            fields = grid.velocity_fields()
-            old = tuple(f.data for f in fields)
-            for f in fields:
-                advect_maccormack(f, old, grid, dt)
-            for f in fields:
-                f.swap()
-            project(grid, solve)
-            vel = tuple(f.data for f in grid.velocity_fields())
-            advect_maccormack(grid.smoke, vel, grid, dt)
-            advect_maccormack(grid.temperature, vel, grid, dt)
-            grid.smoke.swap()
-            grid.temperature.swap()
-        out = np.zeros((PIXELS, PIXELS, 5))
-        o.raymarch(grid.info, grid.smoke.data, grid.temperature.data, np.zeros((1, 1, 1, 3)), np.ones((1, 1, 1)),
-                   oview, olook, out, MARCH)
-        images.append(out[..., :2].copy())
-    return np.stack(images)
+    return rollout_images(o, recipe, derived, sim.dt, Camera.of(sim), sources64, alpha, SUBSTEPS, CHECKPOINT,
+                          PIXELS, MARCH, initial)
 
 
 def test_rollout_gradient_matches_fp64_finite_differences(device, oracle64):

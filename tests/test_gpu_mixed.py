@@ -96,10 +96,16 @@ def test_mixed_precision_solves_match_the_fp64_solution(device, oracle64):
     from plume_amd.sim.mixed import MixedPrecisionMultigrid
     from plume_amd.sim.multigrid import Multigrid, relative_residual
 
+    from plume_amd.sim.mixed import fp32_floor as stall_floor
+
     gpu, orc = divergent_pair(device, oracle64)
     q64 = fp64_solution(orc)
     floor = fp32_floor(gpu, q64)
     assert 1e-7 < floor < 1e-5, floor
+    # where plain FP32 V-cycles stall (the bench's floor) sits at the FP64 solution's FP32 floor
+    stall = stall_floor(gpu)
+    print("FP32 floor", floor, "plain FP32 stall", stall)
+    assert 0.5 * floor < stall < 2.0 * floor, (floor, stall)
     target = max(TOLERANCE, 2.0 * floor)
     velocity0 = [f.data.clone() for f in gpu.velocity_fields()]
     errors, divergences = {}, {}

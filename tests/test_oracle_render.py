@@ -198,3 +198,22 @@ def test_view_params_keep_the_camera_basis(oracle32):
     params = view_params(oracle32, view)
     assert tuple(params.right) == pytest.approx(tuple(view.camera.right), rel=1e-6)
     assert params.perspective == 0
+
+
+def test_near_origin_orthographic_rays_are_the_same_lines(oracle64):
+    """View mode 2 starts each orthographic ray on the camera plane instead of FAR_M toward the camera: the
+    same line and samples, so in FP64 (where the 1e4 m offset costs ~1e-12 m) the images agree to rounding,
+    in both layouts."""
+    g = grid(oracle64)
+    g.smoke.data[...] = 3.0 * blob(CELLS, (8.0, 7.0, 9.0), 3.0)
+    g.temperature.data[...] = 293.0 + 900.0 * blob(CELLS, (8.0, 7.0, 9.0), 2.0)
+    view = make_domain_view(make_camera(25.0, 40.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), CELL_M, 2.0)
+    camera = view.camera
+    near = oracle64.view(camera.right, camera.up, camera.toward, view.left_m, view.top_m, view.pixel_m, 2,
+                         (0.0, 0.0, 0.0))
+    settings = RenderSettings()
+    for layout in (SHADE_LAYOUT, NORMAL_LAYOUT):
+        far = render_oracle(oracle64, g, view, settings, layout)
+        close = render_oracle(oracle64, g, view, settings, layout, params=near)
+        assert far[..., THICKNESS].max() > 0.9
+        assert np.allclose(close, far, rtol=1e-9, atol=1e-9)
