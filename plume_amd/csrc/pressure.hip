@@ -17,6 +17,7 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -331,6 +332,209 @@ __device__ __forceinline__ int xcd_contiguous(int block, int blocks) {
     return x * per + min(x, rem) + block / XCDS;
 }
 
+// x / d, correctly rounded, for the integer sum of couplings d = 4, 5 or 6 of a unit-coupling level's cell,
+// from r = RN(1 / d): q = RN(x r), then Markstein's correction q + RN(x - q d) r, both FMAs. Checked
+// exhaustively against the IEEE quotient over every float x: equal for d = 4 and 5 except x = -0, and for
+// d = 6 except |x| < 2^-125 (subnormal quotients). Such x (and zeros) take the IEEE division.
+__device__ __forceinline__ float divide_small(float x, float d, float r) {
+    float q = x * r;
+    const float e = __builtin_fmaf(-q, d, x);
+    q = __builtin_fmaf(e, r, q);
+    if (__builtin_expect(__builtin_fabsf(x) < 0x1p-100f, 0)) q = x / d;
+    return q;
+}
+
+// The two-colour tile sweep on a unit-coupling level (every coupling and cell volume exactly 1) for a tile
+// whose i and j halo rows all lie inside the level and whose 64 cells along k all exist, the common case on
+// the large levels. The same values as the general path (k_rb_sweep_tile below), with an update reduced to
+// its arithmetic:
+//   - a wavefront updates two rows at once, one per half, of one class: the parity sigma of the cells' k - k0,
+//     which fixes where a cell's k neighbours sit in the colour-split row. The pair is rows (2a, 2t + x) and
+//     (2a + 1, 2t + 1 - x) of the colour's row block, so the second row is a fixed offset from the first that
+//     depends on sigma alone: per-lane constants hold the half's offset and the cell's slot, and the seven
+//     LDS reads of an update are one scalar row base plus one add, then constant offsets;
+//   - Σc per lane is 4 plus the k neighbours that couple (a closed k face takes one from the cells at k = 0
+//     or nz - 1), the volume is 1 (source = cell_m² rhs, cell_m² rounded once, as (1 cell_m) cell_m), and the
+//     division by Σc is divide_small;
+//   - loads and stores of a row go from the row's scalar base pointer.
+// `edge` updates the ring rows' red cells in the k halo (k0 - 1 or k0 + 64) with the general update.
+template <int TI, int TJ, int THREADS, int START, typename T, typename EdgeUpdate>
+__device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *__restrict__ s, const T *__restrict__ q_in,
+                                                    T *__restrict__ q_out, const T *__restrict__ rhs,
+                                                    const T *__restrict__ coarse, Dims nc, float relaxation, int i0,
+                                                    int j0, int k0, EdgeUpdate edge) {
+    constexpr int WAVES = THREADS / 64;
+    constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
+    constexpr uint32_t B = sizeof(T);
+    // red: the ring (TI + 2) x (TJ + 2) rows; black: the tile's TI x TJ rows. Pairs per class = rows / 4.
+    constexpr int RED_T = (TJ + 2) / 2, RED_CLASS = (TI + 2) / 2 * RED_T, RED_PAIRS = 2 * RED_CLASS;
+    constexpr int BLACK_T = TJ / 2, BLACK_CLASS = TI / 2 * BLACK_T, BLACK_PAIRS = 2 * BLACK_CLASS;
+    constexpr int RED_ITERS = (RED_PAIRS + WAVES - 1) / WAVES, BLACK_ITERS = (BLACK_PAIRS + WAVES - 1) / WAVES;
+    static_assert(TI % 2 == 0 && TJ % 2 == 0 && WAVES <= LJ, "rows pair up within each class");
+    const Dims n = cell_dims(L.g);
+    const int n12 = n.n1 * n.n2;
+    const int lane = threadIdx.x & 63, wave = uniform((int)(threadIdx.x >> 6));
+    const int m = lane & 31;
+    const bool upper = lane >= 32;
+    const float cm2 = uniform(L.g.cell_m) * uniform(L.g.cell_m);
+    const bool open_lo = uniform(L.g.open_lo[2]) == 1, open_hi = uniform(L.g.open_hi[2]) == 1;
+    float t1[2], rcp[2];
+    // per colour c (0 red, 1 black) and class sigma: the lane's LDS index relative to its pair's first row's
+    // i-1 neighbour row, and its rhs byte offset relative to the first row's (i n1 + j) n2
+    int lds_lane[2][2];
+    uint32_t rhs_lane[2][2];
+#pragma unroll
+    for (int sigma = 0; sigma < 2; sigma++) {
+        const int k = k0 + 2 * m + sigma;
+        const float lo = (k > 0 || open_lo) ? 1.0f : 0.0f, hi = (k < n.n2 - 1 || open_hi) ? 1.0f : 0.0f;
+        t1[sigma] = (4.0f + lo) + hi;  // integers: the reference's sum in any order
+        rcp[sigma] = 1.0f / t1[sigma];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int step = c == 0 ? 1 - 2 * sigma : 2 * sigma - 1;  // the second row's j offset
+            lds_lane[c][sigma] = (upper ? (LJ + step) * LK : 0) + sigma * HALF + m + 1;
+            rhs_lane[c][sigma] = (uint32_t)((upper ? n12 + step * n.n2 : 0) + k) * B;
+        }
+    }
+    const int g_tile = (i0 * n.n1 + j0) * n.n2;
+    // pair p of colour c: its class, the LDS row index of its first row's i-1 neighbour row, and the first
+    // row's (i n1 + j) n2
+    auto decode = [&](int c, int pair, int &sigma, int &lds_row, int &g_row) {
+        const int per_class = c == 0 ? RED_CLASS : BLACK_CLASS, per_a = c == 0 ? RED_T : BLACK_T;
+        sigma = pair >= per_class;
+        const int q = pair - sigma * per_class, a = q / per_a, t = q - a * per_a;
+        // red rows (2a, 2t + sigma) of the ring (tile row - 1); black rows (2a, 2t + 1 - sigma) of the tile
+        const int ii = c == 0 ? 2 * a - 1 : 2 * a, jj = c == 0 ? 2 * t + sigma - 1 : 2 * t + 1 - sigma;
+        lds_row = (ii + TH - 1) * LJ + (jj + TH);
+        g_row = g_tile + ii * n12 + jj * n.n2;
+    };
+    auto ld = [](const T *base, uint32_t byte_off) {
+        return to_f32(*reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off));
+    };
+    float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS];
+    int red_sigma[RED_ITERS], red_row[RED_ITERS], black_sigma[BLACK_ITERS], black_row[BLACK_ITERS];
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++) {
+        int g_row;
+        decode(0, wave + u * WAVES, red_sigma[u], red_row[u], g_row);
+        rhs_red[u] = 0.0f;
+        if (wave + u * WAVES < RED_PAIRS)
+            rhs_red[u] = ld(rhs, (uint32_t)g_row * B + (red_sigma[u] ? rhs_lane[0][1] : rhs_lane[0][0]));
+    }
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++) {
+        int g_row;
+        decode(1, wave + u * WAVES, black_sigma[u], black_row[u], g_row);
+        rhs_black[u] = 0.0f;
+        if (wave + u * WAVES < BLACK_PAIRS)
+            rhs_black[u] = ld(rhs, (uint32_t)g_row * B + (black_sigma[u] ? rhs_lane[1][1] : rhs_lane[1][0]));
+    }
+    // The tile's start values, row by row: row r = wave + u WAVES of the (TI + 4) x (TJ + 4) rows, its offset
+    // carried from row to row in scalar registers; then the k halo (4 cells a row) by a second loop.
+    const int own_slot = ((lane + TH) & 1) * HALF + ((lane + TH) >> 1);
+    const uint32_t lane_b = (uint32_t)(k0 + lane) * B;
+    int jj = wave, g = g_tile - TH * n12 + (wave - TH) * n.n2;  // row (ii, jj) = (0, wave), at (i0 - 2, j0 - 2 + jj)
+    constexpr int ROWS = LI * LJ / WAVES, BATCH = START == FROM_PROLONG ? 2 : ROWS % 6 == 0 ? 6 : 8;
+    static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
+    for (int b = 0; b < ROWS; b += BATCH) {
+        float v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) {
+            if (START == FROM_ZERO) {
+                v[u] = 0.0f;
+            } else {
+                v[u] = ld(q_in, (uint32_t)g * B + lane_b);
+                if (START == FROM_PROLONG) {
+                    const int i = g / n12, j = (g - i * n12) / n.n2;
+                    v[u] = v[u] + prolongation(coarse, nc, i, j, k0 + lane);
+                }
+            }
+            // next row: WAVES rows on, wrapping into the next i row at most once (WAVES <= LJ)
+            jj += WAVES;
+            const bool wrap = jj >= LJ;
+            jj -= wrap ? LJ : 0;
+            g += wrap ? n12 + (WAVES - LJ) * n.n2 : WAVES * n.n2;
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + own_slot] = v[u];
+    }
+    {
+        constexpr int HALO = LI * LJ * 4, HALO_ITERS = (HALO + THREADS - 1) / THREADS;
+        float v[HALO_ITERS];
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int t = min((int)threadIdx.x + u * THREADS, HALO - 1), row = t >> 2, h = t & 3;
+            const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ;
+            const int k = h < 2 ? k0 - TH + h : k0 + TK - 2 + h;
+            float value = 0.0f;
+            if (START != FROM_ZERO && k >= 0 && k < n.n2) {
+                value = ld(q_in, (uint32_t)((i * n.n1 + j) * n.n2 + k) * B);
+                if (START == FROM_PROLONG) value = value + prolongation(coarse, nc, i, j, k);
+            }
+            v[u] = value;
+        }
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int t = threadIdx.x + u * THREADS, row = t >> 2, h = t & 3;
+            const int kk = h < 2 ? h : TK + h;
+            if (t < HALO) s[row * LK + (kk & 1) * HALF + (kk >> 1)] = v[u];
+        }
+    }
+    __syncthreads();
+    // One update: red_black_sweep_kernel's arithmetic (sim/pressure.py:86-92), the neighbours added in the
+    // reference's order x-, x+, y-, y+, z-, z+. `base` indexes the cell's i-1 neighbour; an empty asm keeps it
+    // the one address register that every read offsets from.
+    auto update = [&](auto sigma_c, int base, float rhs_c) {
+        constexpr int SIGMA = decltype(sigma_c)::value;
+        asm volatile("" : "+v"(base));
+        float *b = s + base;
+        const float q_im = b[0], q_jm = b[(LJ - 1) * LK], q_c = b[LJ * LK];
+        const float q_jp = b[(LJ + 1) * LK], q_ip = b[2 * LJ * LK];
+        const float q_km = b[LJ * LK + (SIGMA == 0 ? HALF - 1 : -HALF)];
+        const float q_kp = b[LJ * LK + (SIGMA == 0 ? HALF : 1 - HALF)];
+        float t0 = 0.0f + q_im;
+        t0 = t0 + q_ip;
+        t0 = t0 + q_jm;
+        t0 = t0 + q_jp;
+        t0 = t0 + q_km;
+        t0 = t0 + q_kp;
+        const float target = divide_small(t0 - cm2 * rhs_c, t1[SIGMA], rcp[SIGMA]);
+        b[LJ * LK] = q_c + relaxation * (target - q_c);
+    };
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++) {
+        if (wave + u * WAVES < RED_PAIRS) {
+            const int row = red_row[u] * LK;
+            if (red_sigma[u] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[u]);
+            else update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[u]);
+        }
+    }
+    edge();
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++) {
+        if (wave + u * WAVES < BLACK_PAIRS) {
+            const int row = black_row[u] * LK;
+            if (black_sigma[u] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[u]);
+            else update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[u]);
+        }
+    }
+    __syncthreads();
+    constexpr int OUT_ROWS = TI * TJ / WAVES;
+    float v[OUT_ROWS];
+#pragma unroll
+    for (int u = 0; u < OUT_ROWS; u++) {
+        const int row = wave + u * WAVES;
+        v[u] = s[((row / TJ + TH) * LJ + (row % TJ + TH)) * LK + own_slot];
+    }
+#pragma unroll
+    for (int u = 0; u < OUT_ROWS; u++) {
+        const int row = wave + u * WAVES;
+        const uint32_t off = (uint32_t)(g_tile + (row / TJ) * n12 + (row % TJ) * n.n2) * B + lane_b;
+        *reinterpret_cast<T *>(reinterpret_cast<char *>(q_out) + off) = from_f32<T>(v[u]);
+    }
+}
+
 // One two-colour sweep over a TI x TJ x 64 tile (i, j, k) with THREADS threads, row by row: a row is the
 // tile's 64 cells along k plus a 2-cell halo each side, 68 floats in LDS stored colour-split (even k - k0 + 2
 // in the first 34 slots, odd in the last 34), so the cells of one colour and their k neighbours sit in
@@ -393,6 +597,22 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         k = k0 + 2 * m + ((i + j + 1) & 1);
         return i < n.n0 && j < n.n1 && k < n.n2;
     };
+    auto update = [&](int i, int j, int k, float rhs_c) {
+        float t0, t1;
+        neighbor_sum_padded<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
+        float &cell = at(i, j, k);
+        cell = gs_update(L, i, j, k, cell, rhs_c, t0, t1, relaxation);
+    };
+    if (UNIT && i0 >= TH && i0 + TI + TH <= n.n0 && j0 >= TH && j0 + TJ + TH <= n.n1 && k0 + TK <= n.n2) {
+        sweep_tile_interior<TI, TJ, THREADS, START, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0, k0,
+                                                       [&]() {
+                                                           int i, j, k;
+                                                           if (red_edge(threadIdx.x, i, j, k)) {
+                                                               update(i, j, k, to_f32(rhs[n.at(i, j, k)]));
+                                                           }
+                                                       });
+        return;
+    }
     float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS], rhs_edge = 0.0f;
 #pragma unroll
     for (int u = 0; u < RED_ITERS; u++) {
@@ -446,12 +666,6 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         }
     }
     __syncthreads();
-    auto update = [&](int i, int j, int k, float rhs_c) {
-        float t0, t1;
-        neighbor_sum_padded<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
-        float &cell = at(i, j, k);
-        cell = gs_update(L, i, j, k, cell, rhs_c, t0, t1, relaxation);
-    };
 #pragma unroll
     for (int u = 0; u < RED_ITERS; u++) {
         int i, j, k;

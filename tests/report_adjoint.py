@@ -7,8 +7,12 @@ source fields and alpha.
 
 Two phases, so that the FP64 rollouts (about a minute each on 8 cores) use no GPU time:
 
-    python tests/report_adjoint.py gpu  --npz OUT.npz    # on a GPU: the adjoint's directional derivatives
-    python tests/report_adjoint.py fp64 --npz OUT.npz    # anywhere: FP64 central differences, the report
+    python tests/report_adjoint.py gpu  --npz OUT.npz [--warm N]   # on a GPU: the adjoint's derivatives
+    python tests/report_adjoint.py fp64 --npz OUT.npz              # anywhere: FP64 central differences
+
+With --warm N the rollout starts from the state N plain substeps of the same sources reach (the same start for
+the target): from rest the clamped MacCormack meets exact ties (still air, zero velocity) where the rollout
+has no derivative along directions of mixed sign, while a developed plume has none in the sources' reach.
 
 The check differentiates <bar, images(sources, alpha)> on the FP64 oracle with bar = dloss/dimages of the
 FP32 images, the vector-Jacobian product that the adjoint computes. Directions: each source scaled by
@@ -59,7 +63,7 @@ def directions(sources64):
     return out
 
 
-def gpu_phase(npz: Path) -> None:
+def gpu_phase(npz: Path, warm: int) -> None:
     import torch
 
     from adjoint_oracle import Camera
@@ -69,14 +73,21 @@ def gpu_phase(npz: Path) -> None:
 
     device = init_device("cuda")
     recipe, derived, sim, sources = differentiable_puff(device, CELLS, PIXELS, MARCH)
+    initial = None
+    if warm > 0:
+        sim.reset()
+        with torch.no_grad():
+            for _ in range(warm):
+                sim.substep(sources, ALPHA)
+        initial = {k: v for k, v in sim.state().items() if k != "pressure"}
     with torch.no_grad():
         target = rollout(Sources(*(1.1 * t for t in sources.tensors())), torch.tensor(ALPHA, dtype=torch.float64),
-                         sim, SUBSTEPS, CHECKPOINT).clone()
+                         sim, SUBSTEPS, CHECKPOINT, initial).clone()
     params = [t.clone().requires_grad_(True) for t in sources.tensors()]
     alpha = torch.tensor(ALPHA, dtype=torch.float64, requires_grad=True)
     torch.cuda.synchronize()
     start = time.perf_counter()
-    images = rollout(Sources(*params), alpha, sim, SUBSTEPS, CHECKPOINT)
+    images = rollout(Sources(*params), alpha, sim, SUBSTEPS, CHECKPOINT, initial)
     loss = ((images - target) ** 2).sum()
     loss.backward()
     torch.cuda.synchronize()
@@ -88,11 +99,13 @@ def gpu_phase(npz: Path) -> None:
         value = sum(float((g * d).sum()) for g, d in zip(grads, dirs)) + float(alpha.grad) * dalpha
         adjoint.append(value)
         print(f"[gpu] {name}: adjoint {value:.9e}", flush=True)
+    start_fields = {f"initial_{k}": v.cpu().numpy() for k, v in (initial or {}).items()}
     np.savez_compressed(npz, smoke=sources64[0].astype(np.float32), temperature=sources64[1].astype(np.float32),
                         lift=sources64[2].astype(np.float32), images=images.detach().cpu().numpy(),
                         target=target.cpu().numpy(), camera=Camera.of(sim).as_array(), dt=sim.dt,
-                        adjoint=np.array(adjoint), loss=float(loss), seconds=seconds)
-    print(f"[gpu] loss {float(loss):.6e}, forward + backward {seconds:.3f} s; wrote {npz}", flush=True)
+                        adjoint=np.array(adjoint), loss=float(loss.detach()), seconds=seconds, warm=warm,
+                        **start_fields)
+    print(f"[gpu] loss {float(loss.detach()):.6e}, forward + backward {seconds:.3f} s; wrote {npz}", flush=True)
 
 
 def fp64_phase(npz: Path, steps: list[float], out: Path | None) -> None:
@@ -108,8 +121,11 @@ def fp64_phase(npz: Path, steps: list[float], out: Path | None) -> None:
     sources64 = [data[n].astype(np.float64) for n in ("smoke", "temperature", "lift")]
     bar64 = 2.0 * (data["images"].astype(np.float64) - data["target"].astype(np.float64))
 
+    initial = {k[len("initial_"):]: data[k] for k in data.files if k.startswith("initial_")} or None
+
     def images(srcs, a):
-        return rollout_images(o, recipe, derived, dt, camera, srcs, a, SUBSTEPS, CHECKPOINT, PIXELS, MARCH)
+        return rollout_images(o, recipe, derived, dt, camera, srcs, a, SUBSTEPS, CHECKPOINT, PIXELS, MARCH,
+                              initial)
 
     start = time.perf_counter()
     base = images(sources64, ALPHA)
@@ -126,7 +142,8 @@ def fp64_phase(npz: Path, steps: list[float], out: Path | None) -> None:
             row[f"rel_err_h{h:g}"] = abs(float(adj) - fd) / abs(fd)
         rows.append(row)
         print("[fp64] " + json.dumps(row), flush=True)
-    report = {"scene": f"differentiable puff {CELLS[0]}x{CELLS[1]}x{CELLS[2]}, {SUBSTEPS} substeps from rest, "
+    start_state = f"after {int(data['warm'])} plain substeps" if initial else "from rest"
+    report = {"scene": f"differentiable puff {CELLS[0]}x{CELLS[1]}x{CELLS[2]}, {SUBSTEPS} substeps {start_state}, "
                        f"images {PIXELS}x{PIXELS} every {CHECKPOINT} substeps, {MARCH} samples per ray",
               "loss": float(data["loss"]), "gpu_forward_backward_s": float(data["seconds"]),
               "fp32_vs_fp64_image_max_abs": float(np.abs(data["images"] - base).max()),
@@ -141,9 +158,10 @@ def main() -> None:
     parser.add_argument("--npz", type=Path, required=True)
     parser.add_argument("--steps", type=float, nargs="+", default=[1e-3, 1e-5])
     parser.add_argument("--json", type=Path, default=None)
+    parser.add_argument("--warm", type=int, default=0, help="plain substeps before the rollout (gpu phase)")
     args = parser.parse_args()
     if args.phase == "gpu":
-        gpu_phase(args.npz)
+        gpu_phase(args.npz, args.warm)
     else:
         fp64_phase(args.npz, args.steps, args.json)
 

@@ -280,17 +280,19 @@ def test_fused_sweep_on_large_grids_is_bit_exact(device, oracle32, cells):
         assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
-@pytest.mark.parametrize("level", [0, 1], ids=["top", "coarse"])
-def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level):
+@pytest.mark.parametrize("level, cells, faces", [(0, (45, 38, 70), DEFAULT_FACES), (1, (45, 38, 70), DEFAULT_FACES),
+                                                  (0, (40, 36, 128), ALL_CLOSED)],
+                         ids=["top", "coarse", "top-closed-k-full"])
+def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level, cells, faces):
     """The autotuner may pick any tile shape; each gives the reference's sweep, bit for bit (a coarse level
-    exercises the overhang couplings)."""
+    exercises the overhang couplings; 128 cells along k with closed faces give tiles whose 64 k cells all
+    exist, the unit-coupling fast path, with Σc = 5 at both k ends)."""
     from oracle.sim import level_infos
     from plume_amd._lib import LIB, call, int3, ptr, stream
     from plume_amd.sim.multigrid import build_levels
     from plume_amd.sim.pressure import red_black_sweeps
 
-    cells = (45, 38, 70)
-    gpu, orc = pair(device, oracle32, cells, DEFAULT_FACES)
+    gpu, orc = pair(device, oracle32, cells, faces)
     lv = build_levels(gpu)[level]
     info = level_infos(oracle32, cells, CELL_M, orc.origin_m, orc.open_lo, orc.open_hi)[level][1]
     r = rng(21)
