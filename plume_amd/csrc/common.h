@@ -119,36 +119,65 @@ template <> __device__ __forceinline__ float from_f32<float>(float x) { return x
 template <> __device__ __forceinline__ __hip_bfloat16 from_f32<__hip_bfloat16>(float x) { return __float2bfloat16(x); }
 template <> __device__ __forceinline__ __half from_f32<__half>(float x) { return __float2half(x); }
 
-// sample_index_space, sim/sample.py:10-29
-template <typename T>
-__device__ __forceinline__ float sample_index(const T *__restrict__ f, Dims n, float gx, float gy, float gz) {
-    float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
-    float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
-    float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
-    int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-    int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
-    float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
-    float c00 = lerp_(to_f32(f[n.at(i0, j0, k0)]), to_f32(f[n.at(i1, j0, k0)]), tx);
-    float c10 = lerp_(to_f32(f[n.at(i0, j1, k0)]), to_f32(f[n.at(i1, j1, k0)]), tx);
-    float c01 = lerp_(to_f32(f[n.at(i0, j0, k1)]), to_f32(f[n.at(i1, j0, k1)]), tx);
-    float c11 = lerp_(to_f32(f[n.at(i0, j1, k1)]), to_f32(f[n.at(i1, j1, k1)]), tx);
-    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+// Element offset (i n1 + j) n2 + k with full-rate 24-bit multiplies: validate_dims keeps (nx + 1)(ny + 1) below
+// 2^24 and every field below 2^31 / 3 elements, so the products and the byte offsets stay exact in 32 bits.
+__device__ __forceinline__ uint32_t offset_of(Dims n, int i, int j, int k) {
+    return __umul24(__umul24((uint32_t)i, (uint32_t)n.n1) + (uint32_t)j, (uint32_t)n.n2) + (uint32_t)k;
 }
 
-// range_index_space, sim/sample.py:32-50
+// f at a 32-bit byte offset from the field's base (the base stays scalar; one vector add per tap)
+template <typename T> __device__ __forceinline__ float load_bytes(const T *__restrict__ f, uint32_t bytes) {
+    return to_f32(*reinterpret_cast<const T *>(reinterpret_cast<const char *>(f) + bytes));
+}
+template <typename T> __device__ __forceinline__ float load_at(const T *__restrict__ f, uint32_t element) {
+    return load_bytes(f, element * (uint32_t)sizeof(T));
+}
+
+// The clamped corner of a trilinear sample over elements of B bytes: the first tap's byte offset, the byte
+// steps to the second tap on each axis (0 where the clamp pins both taps to the last index) and the weights.
+struct Corner {
+    uint32_t o, di, dj, dk;
+    float tx, ty, tz;
+};
+template <uint32_t B>
+__device__ __forceinline__ Corner corner_of(Dims n, float gx, float gy, float gz) {
+    const float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
+    const float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
+    const float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
+    const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
+    Corner c;
+    c.o = offset_of(n, i0, j0, k0) * B;
+    c.di = i0 + 1 < n.n0 ? (uint32_t)(n.n1 * n.n2) * B : 0u;
+    c.dj = j0 + 1 < n.n1 ? (uint32_t)n.n2 * B : 0u;
+    c.dk = k0 + 1 < n.n2 ? B : 0u;
+    c.tx = x - (float)i0;
+    c.ty = y - (float)j0;
+    c.tz = z - (float)k0;
+    return c;
+}
+
+// sample_index_space, sim/sample.py:10-29: i1 = min(i0 + 1, n - 1) etc., nested lerps over x, then y, then z
+template <typename T>
+__device__ __forceinline__ float sample_index(const T *__restrict__ f, Dims n, float gx, float gy, float gz) {
+    const Corner c = corner_of<sizeof(T)>(n, gx, gy, gz);
+    const uint32_t o1 = c.o + c.dk;
+    const float c00 = lerp_(load_bytes(f, c.o), load_bytes(f, c.o + c.di), c.tx);
+    const float c10 = lerp_(load_bytes(f, c.o + c.dj), load_bytes(f, c.o + c.di + c.dj), c.tx);
+    const float c01 = lerp_(load_bytes(f, o1), load_bytes(f, o1 + c.di), c.tx);
+    const float c11 = lerp_(load_bytes(f, o1 + c.dj), load_bytes(f, o1 + c.di + c.dj), c.tx);
+    return lerp_(lerp_(c00, c10, c.ty), lerp_(c01, c11, c.ty), c.tz);
+}
+
+// range_index_space, sim/sample.py:32-50: min and max of the 8 taps (exact, so only the set matters)
 __device__ __forceinline__ void range_index(const float *__restrict__ f, Dims n, float gx, float gy, float gz,
                                             float &lo, float &hi) {
-    float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
-    float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
-    float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
-    int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-    int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
-    float a = f[n.at(i0, j0, k0)];
-    lo = a;
-    hi = a;
-    // the reference visits (di, dj, dk) in this order; min/max are exact, so only the set matters
-    const float v[8] = {f[n.at(i0, j0, k0)], f[n.at(i0, j0, k1)], f[n.at(i0, j1, k0)], f[n.at(i0, j1, k1)],
-                        f[n.at(i1, j0, k0)], f[n.at(i1, j0, k1)], f[n.at(i1, j1, k0)], f[n.at(i1, j1, k1)]};
+    const Corner c = corner_of<4>(n, gx, gy, gz);
+    const uint32_t o1 = c.o + c.dk;
+    const float v[8] = {load_bytes(f, c.o),        load_bytes(f, o1),        load_bytes(f, c.o + c.dj),
+                        load_bytes(f, o1 + c.dj), load_bytes(f, c.o + c.di), load_bytes(f, o1 + c.di),
+                        load_bytes(f, c.o + c.di + c.dj), load_bytes(f, o1 + c.di + c.dj)};
+    lo = v[0];
+    hi = v[0];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         lo = fmin_(lo, v[t]);

@@ -41,21 +41,23 @@ __device__ __forceinline__ float sample_aligned(const float *__restrict__ f, Dim
     const float y = (MASK & 2) ? gy : clamp_(gy, 0.0f, (float)(n.n1 - 1));
     const float z = (MASK & 4) ? gz : clamp_(gz, 0.0f, (float)(n.n2 - 1));
     const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-    const int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
+    const uint32_t o = offset_of(n, i0, j0, k0) * 4u;
+    const uint32_t di = i0 + 1 < n.n0 ? (uint32_t)(n.n1 * n.n2) * 4u : 0u, dj = j0 + 1 < n.n1 ? (uint32_t)n.n2 * 4u : 0u;
+    const uint32_t dk = k0 + 1 < n.n2 ? 4u : 0u;
     const float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
-    auto xl = [&](int j, int k) -> float {
-        const float a = f[n.at(i0, j, k)];
-        if (MASK & 1) return lerp_zero(a, [&] { return f[n.at(i1, j, k)]; });
-        return lerp_(a, f[n.at(i1, j, k)], tx);
+    auto xl = [&](uint32_t base) -> float {
+        const float a = load_bytes(f, base);
+        if (MASK & 1) return lerp_zero(a, [&] { return load_bytes(f, base + di); });
+        return lerp_(a, load_bytes(f, base + di), tx);
     };
-    auto yl = [&](int k) -> float {
-        const float a = xl(j0, k);
-        if (MASK & 2) return lerp_zero(a, [&] { return xl(j1, k); });
-        return lerp_(a, xl(j1, k), ty);
+    auto yl = [&](uint32_t base) -> float {
+        const float a = xl(base);
+        if (MASK & 2) return lerp_zero(a, [&] { return xl(base + dj); });
+        return lerp_(a, xl(base + dj), ty);
     };
-    const float a = yl(k0);
-    if (MASK & 4) return lerp_zero(a, [&] { return yl(k1); });
-    return lerp_(a, yl(k1), tz);
+    const float a = yl(o);
+    if (MASK & 4) return lerp_zero(a, [&] { return yl(o + dk); });
+    return lerp_(a, yl(o + dk), tz);
 }
 
 // velocity_at (sim/sample.py:74-82) at the kind-K sample point of index (i, j, k)
@@ -88,20 +90,16 @@ __device__ __forceinline__ void trace_from_grid(const plm_grid_info &g, const fl
 // range_index_space at the same point share their clamped corner).
 __device__ __forceinline__ float sample_and_range(const float *__restrict__ f, Dims n, float gx, float gy, float gz,
                                                   float &lo, float &hi) {
-    const float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
-    const float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
-    const float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
-    const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-    const int i1 = min(i0 + 1, n.n0 - 1), j1 = min(j0 + 1, n.n1 - 1), k1 = min(k0 + 1, n.n2 - 1);
-    const float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
-    const float f000 = f[n.at(i0, j0, k0)], f100 = f[n.at(i1, j0, k0)], f010 = f[n.at(i0, j1, k0)],
-                f110 = f[n.at(i1, j1, k0)], f001 = f[n.at(i0, j0, k1)], f101 = f[n.at(i1, j0, k1)],
-                f011 = f[n.at(i0, j1, k1)], f111 = f[n.at(i1, j1, k1)];
+    const Corner c = corner_of<4>(n, gx, gy, gz);
+    const uint32_t o1 = c.o + c.dk;
+    const float f000 = load_bytes(f, c.o), f100 = load_bytes(f, c.o + c.di), f010 = load_bytes(f, c.o + c.dj),
+                f110 = load_bytes(f, c.o + c.di + c.dj), f001 = load_bytes(f, o1), f101 = load_bytes(f, o1 + c.di),
+                f011 = load_bytes(f, o1 + c.dj), f111 = load_bytes(f, o1 + c.di + c.dj);
     lo = fmin_(fmin_(fmin_(f000, f100), fmin_(f010, f110)), fmin_(fmin_(f001, f101), fmin_(f011, f111)));
     hi = fmax_(fmax_(fmax_(f000, f100), fmax_(f010, f110)), fmax_(fmax_(f001, f101), fmax_(f011, f111)));
-    const float c00 = lerp_(f000, f100, tx), c10 = lerp_(f010, f110, tx);
-    const float c01 = lerp_(f001, f101, tx), c11 = lerp_(f011, f111, tx);
-    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+    const float c00 = lerp_(f000, f100, c.tx), c10 = lerp_(f010, f110, c.tx);
+    const float c01 = lerp_(f001, f101, c.tx), c11 = lerp_(f011, f111, c.tx);
+    return lerp_(lerp_(c00, c10, c.ty), lerp_(c01, c11, c.ty), c.tz);
 }
 
 struct Field {

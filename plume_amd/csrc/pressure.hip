@@ -430,37 +430,42 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
             rhs_black[u] = ld(rhs, (uint32_t)g_row * B + (black_sigma[u] ? rhs_lane[1][1] : rhs_lane[1][0]));
     }
     // The tile's start values, row by row: row r = wave + u WAVES of the (TI + 4) x (TJ + 4) rows, its offset
-    // carried from row to row in scalar registers; then the k halo (4 cells a row) by a second loop.
+    // carried from row to row in scalar registers; the k halo (4 cells a row) by a second loop. All loads of a
+    // batch are in flight before the first LDS write (one batch of every row, unless the start adds the
+    // prolongation).
     const int own_slot = ((lane + TH) & 1) * HALF + ((lane + TH) >> 1);
     const uint32_t lane_b = (uint32_t)(k0 + lane) * B;
-    int jj = wave, g = g_tile - TH * n12 + (wave - TH) * n.n2;  // row (ii, jj) = (0, wave), at (i0 - 2, j0 - 2 + jj)
-    constexpr int ROWS = LI * LJ / WAVES, BATCH = START == FROM_PROLONG ? 2 : ROWS % 6 == 0 ? 6 : 8;
-    static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
-    for (int b = 0; b < ROWS; b += BATCH) {
-        float v[BATCH];
-#pragma unroll
-        for (int u = 0; u < BATCH; u++) {
-            if (START == FROM_ZERO) {
-                v[u] = 0.0f;
-            } else {
-                v[u] = ld(q_in, (uint32_t)g * B + lane_b);
-                if (START == FROM_PROLONG) {
-                    const int i = g / n12, j = (g - i * n12) / n.n2;
-                    v[u] = v[u] + prolongation(coarse, nc, i, j, k0 + lane);
-                }
-            }
-            // next row: WAVES rows on, wrapping into the next i row at most once (WAVES <= LJ)
-            jj += WAVES;
-            const bool wrap = jj >= LJ;
-            jj -= wrap ? LJ : 0;
-            g += wrap ? n12 + (WAVES - LJ) * n.n2 : WAVES * n.n2;
-        }
-#pragma unroll
-        for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + own_slot] = v[u];
+    // FROM_PROLONG: sample_index of the coarse correction at (i/2 - 1/4, j/2 - 1/4, k/2 - 1/4), its k part per
+    // lane (the same for every row), its i and j parts per row (uniform)
+    uint32_t pk0 = 0, pk1 = 0;
+    float tz = 0.0f;
+    if (START == FROM_PROLONG) {
+        const float z = clamp_((float)(k0 + lane) * 0.5f - 0.25f, 0.0f, (float)(nc.n2 - 1));
+        const int kc0 = (int)z, kc1 = min(kc0 + 1, nc.n2 - 1);
+        tz = z - (float)kc0;
+        pk0 = (uint32_t)kc0 * B;
+        pk1 = (uint32_t)kc1 * B;
     }
-    {
-        constexpr int HALO = LI * LJ * 4, HALO_ITERS = (HALO + THREADS - 1) / THREADS;
-        float v[HALO_ITERS];
+    auto prolong_row = [&](int i, int j) {
+        const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
+        const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
+        const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
+        const float tx = x - (float)ic0, ty = y - (float)jc0;
+        const uint32_t r00 = (uint32_t)((ic0 * nc.n1 + jc0) * nc.n2) * B, r10 = (uint32_t)((ic1 * nc.n1 + jc0) * nc.n2) * B;
+        const uint32_t r01 = (uint32_t)((ic0 * nc.n1 + jc1) * nc.n2) * B, r11 = (uint32_t)((ic1 * nc.n1 + jc1) * nc.n2) * B;
+        const float c00 = lerp_(ld(coarse, r00 + pk0), ld(coarse, r10 + pk0), tx);
+        const float c10 = lerp_(ld(coarse, r01 + pk0), ld(coarse, r11 + pk0), tx);
+        const float c01 = lerp_(ld(coarse, r00 + pk1), ld(coarse, r10 + pk1), tx);
+        const float c11 = lerp_(ld(coarse, r01 + pk1), ld(coarse, r11 + pk1), tx);
+        return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+    };
+    int ii = 0, jj = wave, g = g_tile - TH * n12 + (wave - TH) * n.n2;  // row (ii, jj) at (i0 - 2 + ii, j0 - 2 + jj)
+    constexpr int ROWS = LI * LJ / WAVES;
+    constexpr int BATCH = START == FROM_PROLONG ? (ROWS % 3 == 0 ? 3 : 2) : ROWS;
+    static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
+    constexpr int HALO = LI * LJ * 4, HALO_ITERS = (HALO + THREADS - 1) / THREADS;
+    float halo[HALO_ITERS];
+    auto load_halo = [&]() {
 #pragma unroll
         for (int u = 0; u < HALO_ITERS; u++) {
             const int t = min((int)threadIdx.x + u * THREADS, HALO - 1), row = t >> 2, h = t & 3;
@@ -471,14 +476,36 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
                 value = ld(q_in, (uint32_t)((i * n.n1 + j) * n.n2 + k) * B);
                 if (START == FROM_PROLONG) value = value + prolongation(coarse, nc, i, j, k);
             }
-            v[u] = value;
+            halo[u] = value;
+        }
+    };
+    if (START != FROM_PROLONG) load_halo();
+    for (int b = 0; b < ROWS; b += BATCH) {
+        float v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; u++) {
+            if (START == FROM_ZERO) {
+                v[u] = 0.0f;
+            } else {
+                v[u] = ld(q_in, (uint32_t)g * B + lane_b);
+                if (START == FROM_PROLONG) v[u] = v[u] + prolong_row(i0 - TH + ii, j0 - TH + jj);
+            }
+            // next row: WAVES rows on, wrapping into the next i row at most once (WAVES <= LJ)
+            jj += WAVES;
+            const bool wrap = jj >= LJ;
+            jj -= wrap ? LJ : 0;
+            ii += wrap ? 1 : 0;
+            g += wrap ? n12 + (WAVES - LJ) * n.n2 : WAVES * n.n2;
         }
 #pragma unroll
-        for (int u = 0; u < HALO_ITERS; u++) {
-            const int t = threadIdx.x + u * THREADS, row = t >> 2, h = t & 3;
-            const int kk = h < 2 ? h : TK + h;
-            if (t < HALO) s[row * LK + (kk & 1) * HALF + (kk >> 1)] = v[u];
-        }
+        for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + own_slot] = v[u];
+    }
+    if (START == FROM_PROLONG) load_halo();
+#pragma unroll
+    for (int u = 0; u < HALO_ITERS; u++) {
+        const int t = threadIdx.x + u * THREADS, row = t >> 2, h = t & 3;
+        const int kk = h < 2 ? h : TK + h;
+        if (t < HALO) s[row * LK + (kk & 1) * HALF + (kk >> 1)] = halo[u];
     }
     __syncthreads();
     // One update: red_black_sweep_kernel's arithmetic (sim/pressure.py:86-92), the neighbours added in the

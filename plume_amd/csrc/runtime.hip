@@ -34,6 +34,8 @@ int validate_dims(const int32_t *shape, const char *what) {
         count *= shape[a] + 1;
     }
     if (3 * count >= (int64_t(1) << 31)) return fail(PLM_EINVAL, "%s: grid too large for 32-bit indexing", what);
+    if (int64_t(shape[0] + 1) * (shape[1] + 1) >= (int64_t(1) << 24))
+        return fail(PLM_EINVAL, "%s: nx * ny = %d * %d too large for 24-bit row indexing", what, shape[0], shape[1]);
     if (shape[0] + 1 > 65535 / 3) return fail(PLM_EINVAL, "%s: nx = %d exceeds the launch limit", what, shape[0]);
     return 0;
 }
