@@ -833,18 +833,47 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
     int i, j, k;
     if (!thread_cell(nc, i, j, k)) return;
     float total = 0.0f, count = 0.0f;
-    for (int di = 0; di < 2; di++)
-        for (int dj = 0; dj < 2; dj++)
-            for (int dk = 0; dk < 2; dk++) {
-                const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
-                if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
-                    float t0, t1;
-                    neighbor_sum<UNIT>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
-                    const int c = nf.at(fi, fj, fk);
-                    total = total + residual_value(L, fi, fj, fk, to_f32(q[c]), to_f32(rhs[c]), t0, t1);
+    // Interior coarse cells: every child's six neighbours lie inside the level, none of them the last cell on
+    // an axis (on unit-coupling levels the last cell may be one), so every coupling is 1, Σc = 6 and the
+    // volume 1. The 32 distinct q values of the children and their neighbours are loaded once.
+    const int edge = UNIT ? 3 : 4;
+    if (i >= 1 && 2 * i + edge <= nf.n0 && j >= 1 && 2 * j + edge <= nf.n1 && k >= 1 && 2 * k + edge <= nf.n2) {
+        constexpr uint32_t B = sizeof(T);
+        const uint32_t si = (uint32_t)(nf.n1 * nf.n2) * B, sj = (uint32_t)nf.n2 * B;
+        const uint32_t o = offset_of(nf, 2 * i, 2 * j, 2 * k) * B;
+        // q at child-block coordinates (a, b, c) in [-1, 2]^3 (only the block and its face neighbours are read)
+        auto Q = [&](int a, int b, int c) { return load_bytes(q, o + a * si + b * sj + c * B); };
+        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    float t0 = 0.0f + Q(a - 1, b, c);
+                    t0 = t0 + Q(a + 1, b, c);
+                    t0 = t0 + Q(a, b - 1, c);
+                    t0 = t0 + Q(a, b + 1, c);
+                    t0 = t0 + Q(a, b, c - 1);
+                    t0 = t0 + Q(a, b, c + 1);
+                    const float q_c = Q(a, b, c), rhs_c = load_bytes(rhs, o + a * si + b * sj + c * B);
+                    total = total + (rhs_c - (t0 - 6.0f * q_c) / scale);
                     count = count + 1.0f;
                 }
-            }
+    } else {
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int dk = 0; dk < 2; dk++) {
+                    const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+                    if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
+                        float t0, t1;
+                        neighbor_sum<UNIT>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
+                        const int c = nf.at(fi, fj, fk);
+                        total = total + residual_value(L, fi, fj, fk, to_f32(q[c]), to_f32(rhs[c]), t0, t1);
+                        count = count + 1.0f;
+                    }
+                }
+    }
     const int c = nc.at(i, j, k);
     coarse_rhs[c] = from_f32<T>(total / count);
     if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
@@ -858,6 +887,33 @@ __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ c
     if (!thread_cell(nf, i, j, k)) return;
     const int c = nf.at(i, j, k);
     fine[c] = fine[c] + prolongation(coarse, nc, i, j, k);
+}
+
+// prolong_add_kernel with one wavefront per row of 64 cells along k: the row's i and j interpolation is
+// uniform and only the k part varies per lane (the operations of sample_index, so the same values)
+__global__ __launch_bounds__(256) void k_prolong_rows(const float *__restrict__ coarse, Dims nc,
+                                                      float *__restrict__ fine, Dims nf, const double *gate) {
+    if (gated_off(gate)) return;
+    const int k = blockIdx.x * BX + threadIdx.x;
+    const int j = uniform((int)(blockIdx.y * BY + threadIdx.y)), i = blockIdx.z;
+    if (j >= nf.n1 || k >= nf.n2) return;
+    const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(nc.n2 - 1));
+    const int kc0 = (int)z;
+    const uint32_t pk0 = (uint32_t)kc0 * 4u, pk1 = (kc0 + 1 < nc.n2 ? (uint32_t)(kc0 + 1) : (uint32_t)kc0) * 4u;
+    const float tz = z - (float)kc0;
+    const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
+    const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
+    const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
+    const float tx = x - (float)ic0, ty = y - (float)jc0;
+    const uint32_t r00 = (uint32_t)((ic0 * nc.n1 + jc0) * nc.n2) * 4u, r10 = (uint32_t)((ic1 * nc.n1 + jc0) * nc.n2) * 4u;
+    const uint32_t r01 = (uint32_t)((ic0 * nc.n1 + jc1) * nc.n2) * 4u, r11 = (uint32_t)((ic1 * nc.n1 + jc1) * nc.n2) * 4u;
+    const float c00 = lerp_(load_bytes(coarse, r00 + pk0), load_bytes(coarse, r10 + pk0), tx);
+    const float c10 = lerp_(load_bytes(coarse, r01 + pk0), load_bytes(coarse, r11 + pk0), tx);
+    const float c01 = lerp_(load_bytes(coarse, r00 + pk1), load_bytes(coarse, r10 + pk1), tx);
+    const float c11 = lerp_(load_bytes(coarse, r01 + pk1), load_bytes(coarse, r11 + pk1), tx);
+    const float p = lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+    float *cell = reinterpret_cast<float *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, k) * 4u);
+    *cell = *cell + p;
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t count, float *__restrict__ f, const double *gate) {
@@ -1272,47 +1328,77 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
 
 constexpr int NORM_THREADS = 256;
+constexpr int NORM_I = 4;  // cells along i per thread
 constexpr int FINISH_THREADS = 1024;
 
+inline dim3 norm_grid(const plm_grid_info &g) {
+    const Dims n = cell_dims(g);
+    return dim3((n.n2 + BX - 1) / BX, (n.n1 + BY - 1) / BY, (n.n0 + NORM_I - 1) / NORM_I);
+}
 inline int64_t norm_blocks(const plm_grid_info &g) {
-    const dim3 grid = grid3(cell_dims(g));
+    const dim3 grid = norm_grid(g);
     return (int64_t)grid.x * grid.y * grid.z;
 }
 
-// per block: sum of r^2 and rhs^2 over its 256 cells, in a fixed tree order
+// FP64 sum over the 64 lanes of a wavefront, in a fixed butterfly order
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int offset = 32; offset > 0; offset /= 2) v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+// per block: sum of r^2 and rhs^2 over its 64 x 4 x NORM_I cells, in a fixed order (per thread along i, then
+// a butterfly per wavefront, then the four wavefronts in order)
 template <bool UNIT>
 __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
                                                                 const float *__restrict__ rhs,
                                                                 double *__restrict__ partials, const double *gate) {
     if (gated_off(gate)) return;
     const Dims n = cell_dims(L.g);
-    int i, j, k;
+    const int k = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double rr = 0.0, bb = 0.0;
-    if (thread_cell(n, i, j, k)) {
-        float t0, t1;
-        neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
-        const int c = n.at(i, j, k);
-        const double r = (double)residual_value(L, i, j, k, q[c], rhs[c], t0, t1);
-        const double b = (double)rhs[c];
-        rr = r * r;
-        bb = b * b;
-    }
-    __shared__ double srr[NORM_THREADS], sbb[NORM_THREADS];
-    const int tid = threadIdx.y * BX + threadIdx.x;
-    srr[tid] = rr;
-    sbb[tid] = bb;
-    __syncthreads();
-    for (int half = NORM_THREADS / 2; half > 0; half /= 2) {
-        if (tid < half) {
-            srr[tid] += srr[tid + half];
-            sbb[tid] += sbb[tid + half];
+    if (k < n.n2 && j < n.n1) {
+        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on interior cells
+        const uint32_t si = (uint32_t)(n.n1 * n.n2) * 4u, sj = (uint32_t)n.n2 * 4u;
+        const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
+#pragma unroll
+        for (int t = 0; t < NORM_I; t++) {
+            const int i = blockIdx.z * NORM_I + t;
+            if (i >= n.n0) break;
+            const uint32_t o = offset_of(n, i, j, k) * 4u;
+            const float q_c = load_bytes(q, o), rhs_c = load_bytes(rhs, o);
+            float r;
+            if (UNIT && jk_interior && i >= 1 && i + 1 < n.n0) {
+                // every neighbour inside the level: couplings 1, Σc = 6, volume 1
+                float t0 = 0.0f + load_bytes(q, o - si);
+                t0 = t0 + load_bytes(q, o + si);
+                t0 = t0 + load_bytes(q, o - sj);
+                t0 = t0 + load_bytes(q, o + sj);
+                t0 = t0 + load_bytes(q, o - 4u);
+                t0 = t0 + load_bytes(q, o + 4u);
+                r = rhs_c - (t0 - 6.0f * q_c) / scale;
+            } else {
+                float t0, t1;
+                neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
+                r = residual_value(L, i, j, k, q_c, rhs_c, t0, t1);
+            }
+            rr += (double)r * (double)r;
+            bb += (double)rhs_c * (double)rhs_c;
         }
-        __syncthreads();
     }
-    if (tid == 0) {
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    __shared__ double srr[NORM_THREADS / 64], sbb[NORM_THREADS / 64];
+    const int wave = threadIdx.y;  // blocks are 64 x 4: one wavefront per row of j
+    if (threadIdx.x == 0) {
+        srr[wave] = rr;
+        sbb[wave] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
         const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[2 * block] = srr[0];
-        partials[2 * block + 1] = sbb[0];
+        partials[2 * block] = ((srr[0] + srr[1]) + srr[2]) + srr[3];
+        partials[2 * block + 1] = ((sbb[0] + sbb[1]) + sbb[2]) + sbb[3];
     }
 }
 
@@ -1359,9 +1445,9 @@ int launch_norms(const LevelArgs &L, const float *q, const float *rhs, double *s
                  double tolerance, int mode, int32_t *cycles_run, hipStream_t s) {
     const Dims n = cell_dims(L.g);
     if (L.unit)
-        k_norm_partials<true><<<grid3(n), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
+        k_norm_partials<true><<<norm_grid(L.g), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
     else
-        k_norm_partials<false><<<grid3(n), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
+        k_norm_partials<false><<<norm_grid(L.g), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
     PLM_TRY(check_launch("norm partials"));
     k_norm_finish<<<1, FINISH_THREADS, 0, s>>>(scratch, norm_blocks(L.g), tolerance, mode, cycles_run);
     return check_launch("norm finish");
@@ -1392,6 +1478,8 @@ int launch_tile(const LevelArgs &L, int start, const T *src, T *dst, const T *rh
     return check_launch("rb sweep tile");
 }
 
+constexpr int64_t PROLONG_ROWS_CELLS = int64_t(1) << 20;
+
 // `sweeps` sweeps on one level, the result in q. Before the first: FROM_Q uses q; FROM_ZERO treats q as 0
 // (q need not be cleared); FROM_PROLONG adds the prolongation of `coarse` to q. With a level-sized scratch the
 // sweeps ping-pong q -> scratch -> q in two-colour tiles (an odd count ends with a copy back); without one,
@@ -1417,6 +1505,12 @@ int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch
                 PLM_TRY(check_launch("rb sweep"));
             }
         return 0;
+    }
+    // On large levels the prolongation is cheaper as its own row kernel than sampled inside the tile load.
+    if (start == FROM_PROLONG && n.count() >= PROLONG_ROWS_CELLS) {
+        k_prolong_rows<<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
+        PLM_TRY(check_launch("prolong rows"));
+        start = FROM_Q;
     }
     for (int sw = 0; sw < sweeps; sw++) {
         const bool to_scratch = sw % 2 == 0;

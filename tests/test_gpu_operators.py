@@ -281,12 +281,15 @@ def test_fused_sweep_on_large_grids_is_bit_exact(device, oracle32, cells):
 
 
 @pytest.mark.parametrize("level, cells, faces", [(0, (45, 38, 70), DEFAULT_FACES), (1, (45, 38, 70), DEFAULT_FACES),
-                                                  (0, (40, 36, 128), ALL_CLOSED)],
-                         ids=["top", "coarse", "top-closed-k-full"])
+                                                  (0, (40, 36, 128), ALL_CLOSED), (1, (90, 90, 127), DEFAULT_FACES),
+                                                  (1, (90, 90, 127), ALL_CLOSED)],
+                         ids=["top", "coarse", "top-closed-k-full", "coarse-k-full-open", "coarse-k-full-closed"])
 def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level, cells, faces):
     """The autotuner may pick any tile shape; each gives the reference's sweep, bit for bit (a coarse level
     exercises the overhang couplings; 128 cells along k with closed faces give tiles whose 64 k cells all
-    exist, the unit-coupling fast path, with Σc = 5 at both k ends)."""
+    exist, the unit-coupling fast path, with Σc = 5 at both k ends; level 1 of 90 x 90 x 127 has 64 cells
+    along k with fill 0.5, so the general-coupling fast path meets the last cell, its neighbour, and a ghost or
+    closed face)."""
     from oracle.sim import level_infos
     from plume_amd._lib import LIB, call, int3, ptr, stream
     from plume_amd.sim.multigrid import build_levels
