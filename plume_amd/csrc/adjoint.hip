@@ -9,7 +9,7 @@
 //     (zero push below the 1e-12 guard, as the forward);
 //   - clamped MacCormack (sim/advect.py:24-68): both semi-Lagrangian passes through the trilinear weights
 //     (field role) and through the midpoint trace's positions (velocity role); the clamp passes the gradient
-//     to the tap it selected, or to the unclamped expression;
+//     to the unclamped expression when lo <= value <= hi (ties included), else to the tap it selected;
 //   - projection (sim/pressure.py:144-154) as the linear map u -> Z u - G A^-1 D Z u: its transpose needs
 //     G^T and D^T (here) and one more solve with the same symmetric Poisson operator (the caller's multigrid);
 //   - raymarch_kernel's heat and thickness (render/raymarch.py:86-140): a recording pass keeps each sample's
@@ -194,12 +194,17 @@ __device__ void maccormack_adjoint_1(const plm_grid_info &g, const FieldDesc &f,
             hi_tap = taps[t];
         }
     }
-    const float y = corrected > lo ? corrected : lo;
-    if (!(y < hi)) {
+    // The clamp passes the gradient through wherever lo <= corrected <= hi, and also in a numerically flat
+    // stencil: where the eight taps agree to rounding, the forward's corrected value lands inside or a few
+    // ulps outside [lo, hi] by rounding alone, while a perturbation moves the taps far more than that and the
+    // clamp stays inactive. Beyond that the gradient goes to the tap the clamp selected.
+    const float flat = 8.0f * __FLT_EPSILON__ * fmaxf(fabsf(lo), fabsf(hi));
+    const bool rounding_only = hi - lo <= flat && corrected <= hi + flat && corrected >= lo - flat;
+    if (corrected > hi && !rounding_only) {
         atomicAdd(orig_bar + hi_tap, next_bar);
         return;
     }
-    if (!(corrected > lo)) {
+    if (corrected < lo && !rounding_only) {
         atomicAdd(orig_bar + lo_tap, next_bar);
         return;
     }
@@ -625,7 +630,10 @@ __global__ __launch_bounds__(256) void k_adj_raymarch(plm_grid_info g, const flo
         if (hides != 0.0f && trs > 0.0f)
             trs_bar += heat_bar * glow * step_m * (hides == 1.0f ? 1.0f : hides * powf(trs, hides - 1.0f));
         // alpha = 1 - exp(-opd max(rho, 0) step)
-        const float rho_bar = rho > 0.0f ? alpha_bar * look.opacity_per_density * step_m * (1.0f - alpha) : 0.0f;
+        // max(rho, 0): slope 1 above 0, 0 below, and at exactly 0 (a sample of empty space) the mean of the
+        // two one-sided slopes, which is what a central difference there sees
+        const float slope = rho > 0.0f ? 1.0f : rho == 0.0f ? 0.5f : 0.0f;
+        const float rho_bar = slope * alpha_bar * look.opacity_per_density * step_m * (1.0f - alpha);
         const float t = span0 + ((float)s + 0.5f) * step_m;
         float gp[3];
         for (int a = 0; a < 3; a++) gp[a] = (origin[a] + dir[a] * t - g.origin_m[a]) / g.cell_m - 0.5f;
