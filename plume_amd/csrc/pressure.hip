@@ -324,6 +324,35 @@ __device__ __forceinline__ float prolongation(const T *__restrict__ coarse, Dims
     return sample_index(coarse, nc, (float)i * 0.5f - 0.25f, (float)j * 0.5f - 0.25f, (float)k * 0.5f - 0.25f);
 }
 
+// The k part of prolongation() at fine k (the same for every row): byte offsets of the two coarse k taps
+// and the weight
+template <typename T>
+__device__ __forceinline__ void prolong_k(Dims nc, int k, uint32_t &pk0, uint32_t &pk1, float &tz) {
+    const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(nc.n2 - 1));
+    const int kc0 = (int)z;
+    pk0 = (uint32_t)kc0 * (uint32_t)sizeof(T);
+    pk1 = (uint32_t)min(kc0 + 1, nc.n2 - 1) * (uint32_t)sizeof(T);
+    tz = z - (float)kc0;
+}
+
+// prolongation() at (i, j, k) for a wavefront's row: i and j uniform, the k part from prolong_k
+template <typename T>
+__device__ __forceinline__ float prolong_at_row(const T *__restrict__ coarse, Dims nc, int i, int j, uint32_t pk0,
+                                                uint32_t pk1, float tz) {
+    const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
+    const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
+    const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
+    const float tx = x - (float)ic0, ty = y - (float)jc0;
+    constexpr uint32_t B = sizeof(T);
+    const uint32_t r00 = (uint32_t)((ic0 * nc.n1 + jc0) * nc.n2) * B, r10 = (uint32_t)((ic1 * nc.n1 + jc0) * nc.n2) * B;
+    const uint32_t r01 = (uint32_t)((ic0 * nc.n1 + jc1) * nc.n2) * B, r11 = (uint32_t)((ic1 * nc.n1 + jc1) * nc.n2) * B;
+    const float c00 = lerp_(load_bytes(coarse, r00 + pk0), load_bytes(coarse, r10 + pk0), tx);
+    const float c10 = lerp_(load_bytes(coarse, r01 + pk0), load_bytes(coarse, r11 + pk0), tx);
+    const float c01 = lerp_(load_bytes(coarse, r00 + pk1), load_bytes(coarse, r10 + pk1), tx);
+    const float c11 = lerp_(load_bytes(coarse, r01 + pk1), load_bytes(coarse, r11 + pk1), tx);
+    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+}
+
 // Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD with its own L2. Renumbering so
 // that XCD x runs one contiguous range of tiles keeps a tile's halo neighbours in the same L2.
 constexpr int XCDS = 8;
@@ -435,30 +464,11 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
     // prolongation).
     const int own_slot = ((lane + TH) & 1) * HALF + ((lane + TH) >> 1);
     const uint32_t lane_b = (uint32_t)(k0 + lane) * B;
-    // FROM_PROLONG: sample_index of the coarse correction at (i/2 - 1/4, j/2 - 1/4, k/2 - 1/4), its k part per
-    // lane (the same for every row), its i and j parts per row (uniform)
+    // FROM_PROLONG: the prolongation's k part per lane (the same for every row), its i and j part per row
     uint32_t pk0 = 0, pk1 = 0;
     float tz = 0.0f;
-    if (START == FROM_PROLONG) {
-        const float z = clamp_((float)(k0 + lane) * 0.5f - 0.25f, 0.0f, (float)(nc.n2 - 1));
-        const int kc0 = (int)z, kc1 = min(kc0 + 1, nc.n2 - 1);
-        tz = z - (float)kc0;
-        pk0 = (uint32_t)kc0 * B;
-        pk1 = (uint32_t)kc1 * B;
-    }
-    auto prolong_row = [&](int i, int j) {
-        const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
-        const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
-        const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
-        const float tx = x - (float)ic0, ty = y - (float)jc0;
-        const uint32_t r00 = (uint32_t)((ic0 * nc.n1 + jc0) * nc.n2) * B, r10 = (uint32_t)((ic1 * nc.n1 + jc0) * nc.n2) * B;
-        const uint32_t r01 = (uint32_t)((ic0 * nc.n1 + jc1) * nc.n2) * B, r11 = (uint32_t)((ic1 * nc.n1 + jc1) * nc.n2) * B;
-        const float c00 = lerp_(ld(coarse, r00 + pk0), ld(coarse, r10 + pk0), tx);
-        const float c10 = lerp_(ld(coarse, r01 + pk0), ld(coarse, r11 + pk0), tx);
-        const float c01 = lerp_(ld(coarse, r00 + pk1), ld(coarse, r10 + pk1), tx);
-        const float c11 = lerp_(ld(coarse, r01 + pk1), ld(coarse, r11 + pk1), tx);
-        return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
-    };
+    if (START == FROM_PROLONG) prolong_k<T>(nc, k0 + lane, pk0, pk1, tz);
+    auto prolong_row = [&](int i, int j) { return prolong_at_row(coarse, nc, i, j, pk0, pk1, tz); };
     int ii = 0, jj = wave, g = g_tile - TH * n12 + (wave - TH) * n.n2;  // row (ii, jj) at (i0 - 2 + ii, j0 - 2 + jj)
     constexpr int ROWS = LI * LJ / WAVES;
     constexpr int BATCH = START == FROM_PROLONG ? (ROWS % 3 == 0 ? 3 : 2) : ROWS;
@@ -665,6 +675,18 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         if (START == FROM_PROLONG) value = value + prolongation(coarse, nc, ci, cj, ck);
         return in ? value : 0.0f;
     };
+    // rows (k = k0 + lane): a FROM_PROLONG start takes the prolongation's k part per lane at the clamped k
+    uint32_t pk0 = 0, pk1 = 0;
+    float tz = 0.0f;
+    if (START == FROM_PROLONG) prolong_k<T>(nc, min(max(k0 + lane, 0), n.n2 - 1), pk0, pk1, tz);
+    auto start_row = [&](int i, int j) {
+        if (START != FROM_PROLONG) return start_value(i, j, k0 + lane);
+        const int k = k0 + lane;
+        const bool in = inside(i, j, k);
+        const int ci = min(max(i, 0), n.n0 - 1), cj = min(max(j, 0), n.n1 - 1), ck = min(max(k, 0), n.n2 - 1);
+        const float value = to_f32(q_in[n.at(ci, cj, ck)]) + prolong_at_row(coarse, nc, ci, cj, pk0, pk1, tz);
+        return in ? value : 0.0f;
+    };
     constexpr int ROWS = LI * LJ / WAVES, BATCH = START == FROM_PROLONG ? 2 : ROWS % 6 == 0 ? 6 : 8;
     static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
     for (int b = 0; b < ROWS; b += BATCH) {
@@ -672,7 +694,7 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
 #pragma unroll
         for (int u = 0; u < BATCH; u++) {
             const int row = wave + (b + u) * WAVES;
-            v[u] = start_value(i0 - TH + row / LJ, j0 - TH + row % LJ, k0 + lane);
+            v[u] = start_row(i0 - TH + row / LJ, j0 - TH + row % LJ);
         }
 #pragma unroll
         for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + slot(k0 + lane)] = v[u];
@@ -897,21 +919,10 @@ __global__ __launch_bounds__(256) void k_prolong_rows(const float *__restrict__ 
     const int k = blockIdx.x * BX + threadIdx.x;
     const int j = uniform((int)(blockIdx.y * BY + threadIdx.y)), i = blockIdx.z;
     if (j >= nf.n1 || k >= nf.n2) return;
-    const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(nc.n2 - 1));
-    const int kc0 = (int)z;
-    const uint32_t pk0 = (uint32_t)kc0 * 4u, pk1 = (kc0 + 1 < nc.n2 ? (uint32_t)(kc0 + 1) : (uint32_t)kc0) * 4u;
-    const float tz = z - (float)kc0;
-    const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
-    const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
-    const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
-    const float tx = x - (float)ic0, ty = y - (float)jc0;
-    const uint32_t r00 = (uint32_t)((ic0 * nc.n1 + jc0) * nc.n2) * 4u, r10 = (uint32_t)((ic1 * nc.n1 + jc0) * nc.n2) * 4u;
-    const uint32_t r01 = (uint32_t)((ic0 * nc.n1 + jc1) * nc.n2) * 4u, r11 = (uint32_t)((ic1 * nc.n1 + jc1) * nc.n2) * 4u;
-    const float c00 = lerp_(load_bytes(coarse, r00 + pk0), load_bytes(coarse, r10 + pk0), tx);
-    const float c10 = lerp_(load_bytes(coarse, r01 + pk0), load_bytes(coarse, r11 + pk0), tx);
-    const float c01 = lerp_(load_bytes(coarse, r00 + pk1), load_bytes(coarse, r10 + pk1), tx);
-    const float c11 = lerp_(load_bytes(coarse, r01 + pk1), load_bytes(coarse, r11 + pk1), tx);
-    const float p = lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+    uint32_t pk0, pk1;
+    float tz;
+    prolong_k<float>(nc, k, pk0, pk1, tz);
+    const float p = prolong_at_row(coarse, nc, i, j, pk0, pk1, tz);
     float *cell = reinterpret_cast<float *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, k) * 4u);
     *cell = *cell + p;
 }
