@@ -1372,21 +1372,39 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
         const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on interior cells
         const uint32_t si = (uint32_t)(n.n1 * n.n2) * 4u, sj = (uint32_t)n.n2 * 4u;
         const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
+        const int ib = blockIdx.z * NORM_I;
+        // every load of the NORM_I cells first (rows past the level clamped to its last row, not counted)
+        float qc[NORM_I], bc[NORM_I], qn[NORM_I][6];
 #pragma unroll
         for (int t = 0; t < NORM_I; t++) {
-            const int i = blockIdx.z * NORM_I + t;
-            if (i >= n.n0) break;
+            const int i = min(ib + t, n.n0 - 1);
             const uint32_t o = offset_of(n, i, j, k) * 4u;
-            const float q_c = load_bytes(q, o), rhs_c = load_bytes(rhs, o);
+            qc[t] = load_bytes(q, o);
+            bc[t] = load_bytes(rhs, o);
+            if (UNIT && jk_interior) {
+                const uint32_t om = i >= 1 ? o - si : o, op = i + 1 < n.n0 ? o + si : o;
+                qn[t][0] = load_bytes(q, om);
+                qn[t][1] = load_bytes(q, op);
+                qn[t][2] = load_bytes(q, o - sj);
+                qn[t][3] = load_bytes(q, o + sj);
+                qn[t][4] = load_bytes(q, o - 4u);
+                qn[t][5] = load_bytes(q, o + 4u);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NORM_I; t++) {
+            const int i = ib + t;
+            if (i >= n.n0) break;
+            const float q_c = qc[t], rhs_c = bc[t];
             float r;
             if (UNIT && jk_interior && i >= 1 && i + 1 < n.n0) {
                 // every neighbour inside the level: couplings 1, Σc = 6, volume 1
-                float t0 = 0.0f + load_bytes(q, o - si);
-                t0 = t0 + load_bytes(q, o + si);
-                t0 = t0 + load_bytes(q, o - sj);
-                t0 = t0 + load_bytes(q, o + sj);
-                t0 = t0 + load_bytes(q, o - 4u);
-                t0 = t0 + load_bytes(q, o + 4u);
+                float t0 = 0.0f + qn[t][0];
+                t0 = t0 + qn[t][1];
+                t0 = t0 + qn[t][2];
+                t0 = t0 + qn[t][3];
+                t0 = t0 + qn[t][4];
+                t0 = t0 + qn[t][5];
                 r = rhs_c - (t0 - 6.0f * q_c) / scale;
             } else {
                 float t0, t1;

@@ -6,18 +6,35 @@
 namespace plm {
 namespace {
 
-// line_sums_kernel, sim/reduce.py:12-25: one thread per (j, k) line, sequential over i in FP32
-__global__ __launch_bounds__(256) void k_line_sums(Dims n, const float *__restrict__ a, const float *__restrict__ b,
-                                                   const float *__restrict__ c, float *__restrict__ lines) {
+// line_sums_kernel, sim/reduce.py:12-25: one thread per (j, k) line, sequential over i in FP32. There are
+// only n1 n2 lines, so each thread loads LINE_BURST values of i of all three fields at once (one memory
+// round trip) and then adds them in order.
+constexpr int LINE_THREADS = 64, LINE_BURST = 32;
+__global__ __launch_bounds__(LINE_THREADS) void k_line_sums(Dims n, const float *__restrict__ a,
+                                                            const float *__restrict__ b, const float *__restrict__ c,
+                                                            float *__restrict__ lines) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int plane = n.n1 * n.n2;
     if (t >= plane) return;
     float ta = 0.0f, tb = 0.0f, tc = 0.0f;
-    for (int i = 0; i < n.n0; i++) {
-        const int x = i * plane + t;
-        ta = ta + a[x];
-        tb = tb + b[x];
-        tc = tc + c[x];
+    for (int i0 = 0; i0 < n.n0; i0 += LINE_BURST) {
+        float va[LINE_BURST], vb[LINE_BURST], vc[LINE_BURST];
+#pragma unroll
+        for (int u = 0; u < LINE_BURST; u++) {
+            const int i = min(i0 + u, n.n0 - 1);  // in range; the values past n0 are not added
+            const uint32_t x = (uint32_t)(i * plane + t) * 4u;
+            va[u] = load_bytes(a, x);
+            vb[u] = load_bytes(b, x);
+            vc[u] = load_bytes(c, x);
+        }
+#pragma unroll
+        for (int u = 0; u < LINE_BURST; u++) {
+            if (i0 + u < n.n0) {
+                ta = ta + va[u];
+                tb = tb + vb[u];
+                tc = tc + vc[u];
+            }
+        }
     }
     lines[t] = ta;
     lines[plane + t] = tb;
@@ -110,7 +127,8 @@ int plm_line_sums(const int32_t shape[3], const float *a, const float *b, const 
     PLM_TRY(validate_dims(shape, "plm_line_sums"));
     PLM_REQUIRE(a && b && c && lines, "plm_line_sums: NULL array");
     const Dims n = dims_of(shape);
-    k_line_sums<<<(n.n1 * n.n2 + 255) / 256, 256, 0, as_stream(stream)>>>(n, a, b, c, lines);
+    k_line_sums<<<(n.n1 * n.n2 + LINE_THREADS - 1) / LINE_THREADS, LINE_THREADS, 0, as_stream(stream)>>>(n, a, b, c,
+                                                                                                     lines);
     return check_launch("plm_line_sums");
 }
 
@@ -121,7 +139,7 @@ int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat
     const Dims n = dims_of(shape);
     hipStream_t s = as_stream(stream);
     TimingScope t("budget", s);
-    k_line_sums<<<(n.n1 * n.n2 + 255) / 256, 256, 0, s>>>(n, data, hat, next, lines);
+    k_line_sums<<<(n.n1 * n.n2 + LINE_THREADS - 1) / LINE_THREADS, LINE_THREADS, 0, s>>>(n, data, hat, next, lines);
     PLM_TRY(check_launch("plm_smoke_budget line sums"));
     k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, totals);
     PLM_TRY(check_launch("plm_smoke_budget totals"));
