@@ -1632,7 +1632,7 @@ __device__ __forceinline__ double fill_f64(const plm_grid_info &g, int a, int in
     return index == g.dims[a] - 1 ? (double)g.fill_hi[a] : 1.0;
 }
 
-template <typename T>
+template <typename T, int CPT>
 __global__ __launch_bounds__(COARSE_F64_THREADS) void k_coarse_f64(plm_grid_info g, T *__restrict__ q,
                                                                    const T *__restrict__ rhs, int sweeps,
                                                                    int from_zero) {
@@ -1646,32 +1646,54 @@ __global__ __launch_bounds__(COARSE_F64_THREADS) void k_coarse_f64(plm_grid_info
     }
     __syncthreads();
     const double cm = (double)g.cell_m, gap = (double)g.ghost_gap;
+    // Each thread's CPT cells (CPT x COARSE_F64_THREADS >= cells), planned once: the six neighbours in the order
+    // x-, x+, y-, y+, z-, z+ (an absent one points at the cell itself with coupling 0, which adds +0 to a sum
+    // that starts at +0), the sum of couplings with open-face ghosts, and vol cm² rhs.
+    int nb[CPT][6], colour[CPT];
+    double cp[CPT][6], t1[CPT], src[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+        const int c = min((int)threadIdx.x + u * COARSE_F64_THREADS, cells - 1);
+        const int k = c % n.n2, r = c / n.n2, j = r % n.n1, i = r / n.n1;
+        const int cell[3] = {i, j, k};
+        colour[u] = (int)threadIdx.x + u * COARSE_F64_THREADS < cells ? (i + j + k) & 1 : 2;
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int b = a == 0 ? 1 : 0, cc = a == 2 ? 1 : 2;
+            const double area = 1.0 * fill_f64(g, b, cell[b]) * fill_f64(g, cc, cell[cc]);
+            const double own = fill_f64(g, a, cell[a]);
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                int other[3] = {i, j, k};
+                other[a] += side == 0 ? -1 : 1;
+                nb[u][2 * a + side] = c;
+                cp[u][2 * a + side] = 0.0;
+                if (other[a] >= 0 && other[a] < g.dims[a]) {
+                    const double coupling = area / (0.5 * (own + fill_f64(g, a, other[a])));
+                    nb[u][2 * a + side] = n.at(other[0], other[1], other[2]);
+                    cp[u][2 * a + side] = coupling;
+                    sum += coupling;
+                } else if (side == 0 ? g.open_lo[a] == 1 : g.open_hi[a] == 1) {
+                    sum += area / (0.5 * own + gap);
+                }
+            }
+        }
+        t1[u] = sum;
+        const double vol = fill_f64(g, 0, i) * fill_f64(g, 1, j) * fill_f64(g, 2, k);
+        src[u] = vol * cm * cm * drhs[c];
+        if (sum == 0.0) colour[u] = 2;  // nothing couples: q stays
+    }
     for (int s = 0; s < sweeps; s++)
         for (int color = 0; color < 2; color++) {
-            for (int c = threadIdx.x; c < cells; c += COARSE_F64_THREADS) {
-                const int k = c % n.n2, r = c / n.n2, j = r % n.n1, i = r / n.n1;
-                if (((i + j + k) & 1) != color) continue;
-                const int cell[3] = {i, j, k};
-                double t0 = 0.0, t1 = 0.0;
-                for (int a = 0; a < 3; a++) {
-                    const int b = a == 0 ? 1 : 0, cc = a == 2 ? 1 : 2;
-                    const double area = 1.0 * fill_f64(g, b, cell[b]) * fill_f64(g, cc, cell[cc]);
-                    const double own = fill_f64(g, a, cell[a]);
-                    for (int side = 0; side < 2; side++) {
-                        int nb[3] = {i, j, k};
-                        nb[a] += side == 0 ? -1 : 1;
-                        if (nb[a] >= 0 && nb[a] < g.dims[a]) {
-                            const double coupling = area / (0.5 * (own + fill_f64(g, a, nb[a])));
-                            t0 += coupling * dq[n.at(nb[0], nb[1], nb[2])];
-                            t1 += coupling;
-                        } else if (side == 0 ? g.open_lo[a] == 1 : g.open_hi[a] == 1) {
-                            t1 += area / (0.5 * own + gap);
-                        }
-                    }
-                }
-                if (t1 == 0.0) continue;
-                const double vol = fill_f64(g, 0, i) * fill_f64(g, 1, j) * fill_f64(g, 2, k);
-                const double target = (t0 - vol * cm * cm * drhs[c]) / t1;
+#pragma unroll
+            for (int u = 0; u < CPT; u++) {
+                if (colour[u] != color) continue;
+                const int c = (int)threadIdx.x + u * COARSE_F64_THREADS;
+                double t0 = 0.0;
+#pragma unroll
+                for (int m = 0; m < 6; m++) t0 += cp[u][m] * dq[nb[u][m]];
+                const double target = (t0 - src[u]) / t1[u];
                 dq[c] = dq[c] + 1.5 * (target - dq[c]);
             }
             __syncthreads();
@@ -1740,8 +1762,14 @@ int v_cycle_lp(const LpHierarchy<T> &h, int index, bool cleared, hipStream_t s) 
     const Dims n = cell_dims(L.g);
     if (index == h.n_levels - 1) {
         const int longest = max(n.n0, max(n.n1, n.n2));
-        k_coarse_f64<T><<<1, COARSE_F64_THREADS, 2 * sizeof(double) * (size_t)n.count(), s>>>(
-            L.g, h.q[index], h.rhs[index], max(20, 4 * longest), cleared ? 1 : 0);
+        const size_t bytes = 2 * sizeof(double) * (size_t)n.count();
+        const int sweeps = max(20, 4 * longest), zero = cleared ? 1 : 0;
+        if (n.count() <= COARSE_F64_THREADS)
+            k_coarse_f64<T, 1><<<1, COARSE_F64_THREADS, bytes, s>>>(L.g, h.q[index], h.rhs[index], sweeps, zero);
+        else if (n.count() <= 2 * COARSE_F64_THREADS)
+            k_coarse_f64<T, 2><<<1, COARSE_F64_THREADS, bytes, s>>>(L.g, h.q[index], h.rhs[index], sweeps, zero);
+        else
+            k_coarse_f64<T, 4><<<1, COARSE_F64_THREADS, bytes, s>>>(L.g, h.q[index], h.rhs[index], sweeps, zero);
         return check_launch("coarsest FP64 solve");
     }
     const Dims none{1, 1, 1};
@@ -1771,10 +1799,14 @@ int vcycle_lp(int32_t n_levels, const plm_grid_info *infos, void *const *q, void
     h.sweeps = sweeps;
     const Dims nb = cell_dims(infos[n_levels - 1]);
     const size_t coarse_bytes = 2 * sizeof(double) * (size_t)nb.count();
-    if (coarse_bytes > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_f64<T>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarse_bytes) != hipSuccess)
-        return fail(PLM_EINVAL, "plm_vcycle_lp: %zu bytes of LDS refused", coarse_bytes);
+    if (coarse_bytes > 65536) {
+        const void *kernels[3] = {reinterpret_cast<const void *>(&k_coarse_f64<T, 1>),
+                                  reinterpret_cast<const void *>(&k_coarse_f64<T, 2>),
+                                  reinterpret_cast<const void *>(&k_coarse_f64<T, 4>)};
+        for (const void *kernel : kernels)
+            if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)coarse_bytes) != hipSuccess)
+                return fail(PLM_EINVAL, "plm_vcycle_lp: %zu bytes of LDS refused", coarse_bytes);
+    }
     TimingScope t("multigrid_lp", s);
     for (int c = 0; c < cycles; c++) PLM_TRY(v_cycle_lp<T>(h, 0, c == 0 && zero_start, s));
     return 0;
