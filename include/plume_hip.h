@@ -279,6 +279,14 @@ int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage,
 int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage, void *dst, void *stream);
 /* x = x + float(e) * inv_scale. */
 int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, float inv_scale, void *stream);
+/* One refinement step, fused (no reference counterpart; it replaces plm_add_storage, plm_residual,
+   plm_to_storage and plm_residual_norms in a row): x_out = x + float(e) * inv_scale (x alone when e is NULL),
+   r = rhs - A x_out in FP32 (residual_kernel, sim/multigrid.py:36-43), r_out = storage(r * r_scale) when r_out
+   is not NULL, out[0..1] = |r|^2 and |rhs|^2 in FP64 (device, deterministic). x and x_out must not overlap;
+   x_out may be NULL when e is (nothing written there). scratch holds plm_norm_scratch_doubles(info) doubles. */
+int plm_refine_residual(const plm_grid_info *info, const float *x, const void *e, float inv_scale, int32_t storage,
+                        const float *rhs, float *x_out, void *r_out, float r_scale, double *scratch, double *out,
+                        void *stream);
 
 /* ------------------------------------------------------------------ adjoints (no reference counterpart: the
    reference runs no backward pass, sim/device.py:28). Reverse-mode derivatives of the differentiable substep

@@ -105,6 +105,7 @@ SIGNATURES = {
     "plm_vcycle_lp": [_I, _P, _I, _P, _P, _P, _I, _I, _I, _P],
     "plm_to_storage": [_L, _P, _F, _I, _P, _P],
     "plm_add_storage": [_L, _P, _P, _I, _F, _P],
+    "plm_refine_residual": [_INFO, _P, _P, _F, _I, _P, _P, _P, _F, _P, _P, _P],
     "plm_adj_buoyancy": [_INFO, _P, _P, _F, _F, _F, _F, _F, _P, _P, _P, _P],
     "plm_adj_confinement": [_INFO, _P, _P, _P, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "plm_adj_advect_velocity": [_INFO] + [_P] * 15 + [_F, _P],

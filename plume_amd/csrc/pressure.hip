@@ -912,19 +912,21 @@ __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ c
 }
 
 // prolong_add_kernel with one wavefront per row of 64 cells along k: the row's i and j interpolation is
-// uniform and only the k part varies per lane (the operations of sample_index, so the same values)
-__global__ __launch_bounds__(256) void k_prolong_rows(const float *__restrict__ coarse, Dims nc,
-                                                      float *__restrict__ fine, Dims nf, const double *gate) {
+// uniform and only the k part varies per lane (the operations of sample_index, so the same values). T is the
+// stored type (the sum is formed in FP32 and rounded once into T).
+template <typename T = float>
+__global__ __launch_bounds__(256) void k_prolong_rows(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine,
+                                                      Dims nf, const double *gate) {
     if (gated_off(gate)) return;
     const int k = blockIdx.x * BX + threadIdx.x;
     const int j = uniform((int)(blockIdx.y * BY + threadIdx.y)), i = blockIdx.z;
     if (j >= nf.n1 || k >= nf.n2) return;
     uint32_t pk0, pk1;
     float tz;
-    prolong_k<float>(nc, k, pk0, pk1, tz);
+    prolong_k<T>(nc, k, pk0, pk1, tz);
     const float p = prolong_at_row(coarse, nc, i, j, pk0, pk1, tz);
-    float *cell = reinterpret_cast<float *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, k) * 4u);
-    *cell = *cell + p;
+    T *cell = reinterpret_cast<T *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, k) * (uint32_t)sizeof(T));
+    *cell = from_f32<T>(to_f32(*cell) + p);
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t count, float *__restrict__ f, const double *gate) {
@@ -1373,22 +1375,26 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
         const uint32_t si = (uint32_t)(n.n1 * n.n2) * 4u, sj = (uint32_t)n.n2 * 4u;
         const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
         const int ib = blockIdx.z * NORM_I;
-        // every load of the NORM_I cells first (rows past the level clamped to its last row, not counted)
+        // every load of the NORM_I cells first (rows past the level clamped to its last row, not counted),
+        // unconditionally: a neighbour outside the level reads the cell itself and goes unused, so no load
+        // sits in a branch (the wait for a branch's loads would come at its end, one batch at a time)
         float qc[NORM_I], bc[NORM_I], qn[NORM_I][6];
+        const uint32_t dj_lo = j >= 1 ? sj : 0u, dj_hi = j + 1 < n.n1 ? sj : 0u;
+        const uint32_t dk_lo = k >= 1 ? 4u : 0u, dk_hi = k + 1 < n.n2 ? 4u : 0u;
 #pragma unroll
         for (int t = 0; t < NORM_I; t++) {
             const int i = min(ib + t, n.n0 - 1);
             const uint32_t o = offset_of(n, i, j, k) * 4u;
             qc[t] = load_bytes(q, o);
             bc[t] = load_bytes(rhs, o);
-            if (UNIT && jk_interior) {
+            if (UNIT) {
                 const uint32_t om = i >= 1 ? o - si : o, op = i + 1 < n.n0 ? o + si : o;
                 qn[t][0] = load_bytes(q, om);
                 qn[t][1] = load_bytes(q, op);
-                qn[t][2] = load_bytes(q, o - sj);
-                qn[t][3] = load_bytes(q, o + sj);
-                qn[t][4] = load_bytes(q, o - 4u);
-                qn[t][5] = load_bytes(q, o + 4u);
+                qn[t][2] = load_bytes(q, o - dj_lo);
+                qn[t][3] = load_bytes(q, o + dj_hi);
+                qn[t][4] = load_bytes(q, o - dk_lo);
+                qn[t][5] = load_bytes(q, o + dk_hi);
             }
         }
 #pragma unroll
@@ -1537,7 +1543,7 @@ int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch
     }
     // On large levels the prolongation is cheaper as its own row kernel than sampled inside the tile load.
     if (start == FROM_PROLONG && n.count() >= PROLONG_ROWS_CELLS) {
-        k_prolong_rows<<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
+        k_prolong_rows<float><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
         PLM_TRY(check_launch("prolong rows"));
         start = FROM_Q;
     }
@@ -1723,6 +1729,125 @@ __global__ __launch_bounds__(256) void k_add_storage(int64_t count, float *__res
         x[c] = x[c] + to_f32(e[c]) * inv_scale;
 }
 
+// x + float(e) * inv_scale at a cell: the FP32 pressure with the last round's correction added (x alone
+// without CORRECT), the value k_add_storage would store there
+template <bool CORRECT, typename T>
+struct CorrectedQ {
+    const float *x;
+    const T *e;
+    float inv_scale;
+    Dims n;
+    __device__ __forceinline__ float at(int c) const {
+        const float v = load_at(x, (uint32_t)c);
+        return CORRECT ? v + load_at(e, (uint32_t)c) * inv_scale : v;
+    }
+    __device__ float operator()(int i, int j, int k) const { return at(n.at(i, j, k)); }
+};
+
+// One refinement step, fused: x_out = x + float(e) * inv_scale, r = rhs - A x_out in FP32 (residual_kernel,
+// sim/multigrid.py:36-43, the same values as k_residual on the corrected pressure), r_out = T(r * r_scale),
+// and per block the FP64 sums of r^2 and rhs^2 in k_norm_partials' order. Blocks of 64 x 4 (k, j) threads,
+// NORM_I cells along i per thread; the corrected column i - 1 ... i + NORM_I of a thread is formed once.
+// x and x_out must not overlap (the neighbours read x while x_out is written); x_out may be null when e is.
+template <bool UNIT, bool CORRECT, typename T>
+__global__ __launch_bounds__(NORM_THREADS) void k_refine_residual(LevelArgs L, const float *__restrict__ x,
+                                                                  const T *__restrict__ e, float inv_scale,
+                                                                  const float *__restrict__ rhs,
+                                                                  float *__restrict__ x_out, T *__restrict__ r_out,
+                                                                  float r_scale, double *__restrict__ partials) {
+    const Dims n = cell_dims(L.g);
+    const int k = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double rr = 0.0, bb = 0.0;
+    if (k < n.n2 && j < n.n1) {
+        const CorrectedQ<CORRECT, T> q{x, e, inv_scale, n};
+        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on interior cells
+        const int si = n.n1 * n.n2, sj = n.n2;
+        const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
+        const int ib = blockIdx.z * NORM_I;
+        // column[t + 1] is the corrected q at i = ib + t (rows past the level clamped to its last row). Every
+        // load is unconditional (a neighbour outside the level reads the cell itself and goes unused), so the
+        // loads issue as one batch.
+        float column[NORM_I + 2], bc[NORM_I], side[NORM_I][4];
+        const int dj_lo = j >= 1 ? sj : 0, dj_hi = j + 1 < n.n1 ? sj : 0;
+        const int dk_lo = k >= 1 ? 1 : 0, dk_hi = k + 1 < n.n2 ? 1 : 0;
+#pragma unroll
+        for (int t = -1; t <= NORM_I; t++) {
+            const int i = max(0, min(ib + t, n.n0 - 1));
+            column[t + 1] = q.at(n.at(i, j, k));
+        }
+#pragma unroll
+        for (int t = 0; t < NORM_I; t++) {
+            const int c = n.at(min(ib + t, n.n0 - 1), j, k);
+            bc[t] = load_at(rhs, (uint32_t)c);
+            if (UNIT) {
+                side[t][0] = q.at(c - dj_lo);
+                side[t][1] = q.at(c + dj_hi);
+                side[t][2] = q.at(c - dk_lo);
+                side[t][3] = q.at(c + dk_hi);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NORM_I; t++) {
+            const int i = ib + t;
+            if (i >= n.n0) break;
+            const float q_c = column[t + 1], rhs_c = bc[t];
+            float r;
+            if (UNIT && jk_interior && i >= 1 && i + 1 < n.n0) {
+                // every neighbour inside the level: couplings 1, sum 6, volume 1; the reference's order
+                float t0 = 0.0f + column[t];
+                t0 = t0 + column[t + 2];
+                t0 = t0 + side[t][0];
+                t0 = t0 + side[t][1];
+                t0 = t0 + side[t][2];
+                t0 = t0 + side[t][3];
+                r = rhs_c - (t0 - 6.0f * q_c) / scale;
+            } else {
+                float t0, t1;
+                neighbor_sum<UNIT>(L, i, j, k, q, t0, t1);
+                r = residual_value(L, i, j, k, q_c, rhs_c, t0, t1);
+            }
+            const uint32_t c = (uint32_t)(i * si + j * sj + k);
+            if (x_out != nullptr) x_out[c] = q_c;
+            if (r_out != nullptr) r_out[c] = from_f32<T>(r * r_scale);
+            rr += (double)r * (double)r;
+            bb += (double)rhs_c * (double)rhs_c;
+        }
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    __shared__ double srr[NORM_THREADS / 64], sbb[NORM_THREADS / 64];
+    const int wave = threadIdx.y;
+    if (threadIdx.x == 0) {
+        srr[wave] = rr;
+        sbb[wave] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * block] = ((srr[0] + srr[1]) + srr[2]) + srr[3];
+        partials[2 * block + 1] = ((sbb[0] + sbb[1]) + sbb[2]) + sbb[3];
+    }
+}
+
+template <typename T>
+int launch_refine(const LevelArgs &L, const float *x, const void *e, float inv_scale, const float *rhs, float *x_out,
+                  void *r_out, float r_scale, double *scratch, hipStream_t s) {
+    const T *et = static_cast<const T *>(e);
+    T *rt = static_cast<T *>(r_out);
+    const dim3 grid = norm_grid(L.g), block = block3();
+    if (L.unit && et)
+        k_refine_residual<true, true, T><<<grid, block, 0, s>>>(L, x, et, inv_scale, rhs, x_out, rt, r_scale, scratch + 4);
+    else if (L.unit)
+        k_refine_residual<true, false, T><<<grid, block, 0, s>>>(L, x, et, inv_scale, rhs, x_out, rt, r_scale, scratch + 4);
+    else if (et)
+        k_refine_residual<false, true, T><<<grid, block, 0, s>>>(L, x, et, inv_scale, rhs, x_out, rt, r_scale, scratch + 4);
+    else
+        k_refine_residual<false, false, T><<<grid, block, 0, s>>>(L, x, et, inv_scale, rhs, x_out, rt, r_scale, scratch + 4);
+    PLM_TRY(check_launch("refine residual"));
+    k_norm_finish<<<1, FINISH_THREADS, 0, s>>>(scratch, norm_blocks(L.g), 0.0, 0, nullptr);
+    return check_launch("norm finish");
+}
+
 template <typename T>
 struct LpHierarchy {
     int n_levels;
@@ -1737,6 +1862,13 @@ struct LpHierarchy {
 template <typename T>
 int lp_sweeps(const LevelArgs &L, T *q, const T *rhs, T *scratch, int sweeps, int start, const T *coarse, Dims nc,
               hipStream_t s) {
+    // large levels: the prolongation as its own row kernel (q + p rounded into T), as in launch_sweeps
+    if (start == FROM_PROLONG && cell_dims(L.g).count() >= PROLONG_ROWS_CELLS) {
+        const Dims n = cell_dims(L.g);
+        k_prolong_rows<T><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, nullptr);
+        PLM_TRY(check_launch("prolong rows"));
+        start = FROM_Q;
+    }
     for (int sw = 0; sw < sweeps; sw++) {
         const bool to_scratch = sw % 2 == 0;
         const T *src = to_scratch ? q : scratch;
@@ -2126,6 +2258,32 @@ int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, flo
     else
         return fail(PLM_EINVAL, "plm_add_storage: storage %d unknown", storage);
     return check_launch("add storage");
+}
+
+int plm_refine_residual(const plm_grid_info *info, const float *x, const void *e, float inv_scale, int32_t storage,
+                        const float *rhs, float *x_out, void *r_out, float r_scale, double *scratch, double *out,
+                        void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(x && rhs && scratch && out, "plm_refine_residual: NULL array");
+    PLM_REQUIRE(x_out || !e, "plm_refine_residual: a correction e needs x_out");
+    const int64_t count = cell_dims(*info).count();
+    const uintptr_t xa = reinterpret_cast<uintptr_t>(x), xb = reinterpret_cast<uintptr_t>(x_out);
+    const uintptr_t bytes = (uintptr_t)count * sizeof(float);
+    PLM_REQUIRE(!x_out || xa + bytes <= xb || xb + bytes <= xa, "plm_refine_residual: x and x_out overlap");
+    hipStream_t s = as_stream(stream);
+    const LevelArgs L = level_args(*info);
+    TimingScope t("refine_residual", s);
+    if (storage == PLM_STORAGE_BF16)
+        PLM_TRY(launch_refine<__hip_bfloat16>(L, x, e, inv_scale, rhs, x_out, r_out, r_scale, scratch, s));
+    else if (storage == PLM_STORAGE_F16)
+        PLM_TRY(launch_refine<__half>(L, x, e, inv_scale, rhs, x_out, r_out, r_scale, scratch, s));
+    else if (storage == PLM_STORAGE_F32)
+        PLM_TRY(launch_refine<float>(L, x, e, inv_scale, rhs, x_out, r_out, r_scale, scratch, s));
+    else
+        return fail(PLM_EINVAL, "plm_refine_residual: storage %d unknown", storage);
+    PLM_REQUIRE(hipMemcpyAsync(out, scratch + 2, 2 * sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess,
+                "plm_refine_residual: copy failed");
+    return 0;
 }
 
 }  // extern "C"

@@ -12,6 +12,11 @@ until |b - A x| / |b| <= tolerance, the norms reduced in FP64 on the device. Eac
    residual + restrict, and the coarsest level solved in FP64 (plm_vcycle_lp);
 4. adds e back into x in FP32.
 
+Steps 4, 1 and 2 of consecutive rounds and the FP64 norms run as one kernel (plm_refine_residual), which
+writes the corrected x to the other of two FP32 buffers (the neighbours of a cell still read the old x). Its
+power-of-two scale is predicted from the previous rounds' residuals; a prediction more than 2^4 off the
+measured RMS stores r again with the measured scale.
+
 The FP32 pressure itself bounds the reachable residual: even the exact solution rounded to FP32 leaves a
 relative residual of order 1e-6 on grids of 10^5 cells and more. A round that does not halve the residual
 therefore ends the solve (`log.stalled`) instead of spinning at that floor.
@@ -79,7 +84,7 @@ class MixedPrecisionMultigrid:
         self.q = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells]
         self.rhs = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells]
         self.scratch = [torch.zeros(c, dtype=dtype, device=device) for c in self.cells[:-1]]
-        self.residual = torch.zeros(self.cells[0], dtype=torch.float32, device=device)
+        self.x_alt = torch.zeros(self.cells[0], dtype=torch.float32, device=device)
         self.norm_scratch = torch.zeros(LIB.plm_norm_scratch_doubles(ctypes.byref(self.infos[0])),
                                         dtype=torch.float64, device=device)
         self.norms = torch.zeros(2, dtype=torch.float64, device=device)
@@ -100,27 +105,56 @@ class MixedPrecisionMultigrid:
         rel = math.sqrt(r2 / b2) if b2 > 0 else (0.0 if r2 == 0 else math.inf)
         return rel, math.sqrt(r2 / count)
 
+    def _refine(self, grid, x, e, inv_scale, x_out, r_scale) -> tuple[float, float]:
+        """plm_refine_residual on the simulation level: x_out = x + e * inv_scale (e None: no correction, x_out
+        None: not written), rhs[0] = storage(r * r_scale) for r = b - A x_out; returns (|r| / |b|, RMS of r)
+        from the FP64 norms (synchronises)."""
+        call("plm_refine_residual", ctypes.byref(self.infos[0]), ptr(x), None if e is None else _vptr(e),
+             inv_scale, STORAGE[self.storage], ptr(grid.divergence), None if x_out is None else ptr(x_out),
+             _vptr(self.rhs[0]), r_scale, ptr(self.norm_scratch, torch.float64), ptr(self.norms, torch.float64),
+             stream())
+        r2, b2 = self.norms.tolist()
+        rel = math.sqrt(r2 / b2) if b2 > 0 else (0.0 if r2 == 0 else math.inf)
+        return rel, math.sqrt(r2 / grid.pressure.numel())
+
     def __call__(self, grid) -> None:
-        x, b = grid.pressure, grid.divergence
-        count = x.numel()
+        x = grid.pressure
         code = STORAGE[self.storage]
         q, rhs, scratch = self._arrays()
         self.log = RefinementLog()
-        for _ in range(self.max_rounds + 1):
+        buffers = (x, self.x_alt)
+        current = 0  # buffers[current] holds the latest x
+        stored = None  # the scale rhs[0] was stored with, when it was the latest r's
+        try:
             rel, rms = self.relative_residual(grid)
-            self.log.residuals.append(rel)
-            if rel <= self.tolerance or rms == 0.0 or len(self.log.residuals) > self.max_rounds:
-                return
-            if len(self.log.residuals) >= 2 and rel > STALL * self.log.residuals[-2]:
-                self.log.stalled = True
-                return
-            call("plm_residual", ctypes.byref(self.infos[0]), ptr(x), ptr(b), ptr(self.residual), stream())
-            scale = 2.0 ** (-round(math.log2(rms)))
-            call("plm_to_storage", count, ptr(self.residual), scale, code, _vptr(self.rhs[0]), stream())
-            call("plm_vcycle_lp", len(self.cells), self.infos, code, q, rhs, scratch, self.inner_cycles, self.sweeps,
-                 1, stream())
-            call("plm_add_storage", count, ptr(x), _vptr(self.q[0]), code, 1.0 / scale, stream())
+            while True:
+                self.log.residuals.append(rel)
+                if rel <= self.tolerance or rms == 0.0 or len(self.log.residuals) > self.max_rounds:
+                    return
+                if len(self.log.residuals) >= 2 and rel > STALL * self.log.residuals[-2]:
+                    self.log.stalled = True
+                    return
+                scale = _pow2_scale(rms)
+                if stored is None or abs(math.log2(stored / scale)) > 4:
+                    self._refine(grid, buffers[current], None, 0.0, None, scale)
+                    stored = scale
+                call("plm_vcycle_lp", len(self.cells), self.infos, code, q, rhs, scratch, self.inner_cycles,
+                     self.sweeps, 1, stream())
+                # the next round's r, stored with the scale its predicted RMS asks for
+                ratio = rel / self.log.residuals[-2] if len(self.log.residuals) >= 2 else 0.1
+                predicted = _pow2_scale(rms * min(max(ratio, 1e-3), 1.0))
+                rel, rms = self._refine(grid, buffers[current], self.q[0], 1.0 / stored, buffers[1 - current],
+                                        predicted)
+                current = 1 - current
+                stored = predicted
+        finally:
+            if current == 1:
+                x.copy_(self.x_alt)
 
+
+def _pow2_scale(rms: float) -> float:
+    """The power of two that brings `rms` nearest 1 (exact to apply in any floating-point type)."""
+    return 2.0 ** (-round(math.log2(rms)))
 
 def divergent_velocity(grid, seed: int, blobs: int = 64, emitter_radius_cells: float = 12.0,
                        emitter_speed: float = 1.0) -> None:

@@ -40,6 +40,63 @@ def test_storage_conversion_rounds_to_nearest_even(device, storage):
     assert torch.equal(y, expected)
 
 
+@pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("level", [0, 1])
+def test_fused_refinement_step_equals_the_unfused_sequence(device, oracle64, storage, level):
+    """plm_refine_residual gives, bit for bit, x + e * inv_scale (plm_add_storage), the FP32 residual of that x
+    (plm_residual), its scaled copy in the storage type (plm_to_storage) and the FP64 norms
+    (plm_residual_norms): on the unit-coupling simulation level and on a coarser level (couplings other than
+    1, ghost gaps). Without a correction it writes no x; x_out overlapping x is refused."""
+    import ctypes
+
+    import torch
+
+    from plume_amd._lib import LIB, PlumeHipError, call, ptr, stream
+    from plume_amd.sim.mixed import DTYPES, STORAGE
+    from plume_amd.sim.multigrid import build_levels
+
+    gpu, _ = divergent_pair(device, oracle64)
+    info = build_levels(gpu, 8)[level].info
+    cells = tuple(info.dims)
+    code, dtype = STORAGE[storage], DTYPES[storage]
+    bits = torch.int32 if storage == "f32" else torch.int16
+    g = torch.Generator().manual_seed(11 + level)
+    x = torch.randn(cells, generator=g).to(device)
+    b = (torch.randn(cells, generator=g) * 100.0).to(device)
+    e = (torch.randn(cells, generator=g) * 4.0).to(dtype).to(device)
+    scratch = torch.zeros(LIB.plm_norm_scratch_doubles(ctypes.byref(info)), dtype=torch.float64, device=device)
+    norms, expected_norms = (torch.zeros(2, dtype=torch.float64, device=device) for _ in range(2))
+    x_out = torch.full_like(x, float("nan"))
+    r_out = torch.empty(cells, dtype=dtype, device=device)
+    call("plm_refine_residual", ctypes.byref(info), ptr(x), e.data_ptr(), 0.25, code, ptr(b), ptr(x_out),
+         r_out.data_ptr(), 8.0, ptr(scratch, torch.float64), ptr(norms, torch.float64), stream())
+
+    y = x.clone()
+    call("plm_add_storage", y.numel(), ptr(y), e.data_ptr(), code, 0.25, stream())
+    r = torch.empty_like(x)
+    call("plm_residual", ctypes.byref(info), ptr(y), ptr(b), ptr(r), stream())
+    stored = torch.empty_like(r_out)
+    call("plm_to_storage", r.numel(), ptr(r), 8.0, code, stored.data_ptr(), stream())
+    call("plm_residual_norms", ctypes.byref(info), ptr(y), ptr(b), ptr(scratch, torch.float64),
+         ptr(expected_norms, torch.float64), stream())
+    assert torch.equal(x_out.view(torch.int32), y.view(torch.int32))
+    assert torch.equal(r_out.view(bits), stored.view(bits))
+    assert torch.equal(norms, expected_norms)
+    # the norms themselves against FP64 sums of the same FP32 residual
+    r64, b64 = r.double(), b.double()
+    want = torch.stack([(r64 * r64).sum(), (b64 * b64).sum()])
+    assert torch.allclose(norms, want, rtol=1e-12, atol=0.0)
+
+    # no correction: the residual of x itself, nothing written to x
+    call("plm_refine_residual", ctypes.byref(info), ptr(y), None, 0.0, code, ptr(b), None, r_out.data_ptr(), 8.0,
+         ptr(scratch, torch.float64), ptr(norms, torch.float64), stream())
+    assert torch.equal(r_out.view(bits), stored.view(bits))
+    assert torch.equal(norms, expected_norms)
+    with pytest.raises(PlumeHipError, match="overlap"):
+        call("plm_refine_residual", ctypes.byref(info), ptr(y), e.data_ptr(), 1.0, code, ptr(b), ptr(y), None,
+             1.0, ptr(scratch, torch.float64), ptr(norms, torch.float64), stream())
+
+
 def divergent_pair(device, oracle64):
     from oracle.sim import compute_divergence as oracle_divergence
     from plume_amd.sim.mixed import divergent_velocity
