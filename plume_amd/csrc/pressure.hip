@@ -1360,8 +1360,28 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// The residual of one cell of a unit-coupling level (every coupling and cell volume 1, so the scale is
+// (1 cell_m) cell_m) from its six neighbours in the reference's order x-, x+, y-, y+, z-, z+, each passed as
+// (value, inside): an outside neighbour adds +0.0, what neighbor_sum adds for a ghost or a closed face.
+// `couples` is the sum of couplings, the faces with an inside neighbour or open: a sum of ones, exact in any
+// order. The same values as residual_value(neighbor_sum).
+__device__ __forceinline__ float unit_residual(float q_c, float rhs_c, float scale, const float (&qn)[6],
+                                               const bool (&in)[6], int couples) {
+    float t0 = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 6; m++) t0 = t0 + (in[m] ? qn[m] : 0.0f);
+    return rhs_c - (t0 - (float)couples * q_c) / scale;
+}
+
+// the faces of a cell along one axis that couple on a unit level: an inside neighbour, or an open face
+__device__ __forceinline__ int unit_couples(const plm_grid_info &g, int a, bool lo_in, bool hi_in) {
+    return ((lo_in || uniform(g.open_lo[a]) == 1) ? 1 : 0) + ((hi_in || uniform(g.open_hi[a]) == 1) ? 1 : 0);
+}
+
 // per block: sum of r^2 and rhs^2 over its 64 x 4 x NORM_I cells, in a fixed order (per thread along i, then
-// a butterfly per wavefront, then the four wavefronts in order)
+// a butterfly per wavefront, then the four wavefronts in order). Every load comes first and unconditionally
+// (a neighbour outside the level reads the cell itself and is masked), so they issue as one batch; unit levels
+// then need no branch per cell.
 template <bool UNIT>
 __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
                                                                 const float *__restrict__ rhs,
@@ -1371,16 +1391,14 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
     const int k = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double rr = 0.0, bb = 0.0;
     if (k < n.n2 && j < n.n1) {
-        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on interior cells
+        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on unit levels
         const uint32_t si = (uint32_t)(n.n1 * n.n2) * 4u, sj = (uint32_t)n.n2 * 4u;
-        const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
         const int ib = blockIdx.z * NORM_I;
-        // every load of the NORM_I cells first (rows past the level clamped to its last row, not counted),
-        // unconditionally: a neighbour outside the level reads the cell itself and goes unused, so no load
-        // sits in a branch (the wait for a branch's loads would come at its end, one batch at a time)
+        const bool jm = j >= 1, jp = j + 1 < n.n1, km = k >= 1, kp = k + 1 < n.n2;
+        const int couples_jk = unit_couples(L.g, 1, jm, jp) + unit_couples(L.g, 2, km, kp);
+        const uint32_t dj_lo = jm ? sj : 0u, dj_hi = jp ? sj : 0u, dk_lo = km ? 4u : 0u, dk_hi = kp ? 4u : 0u;
+        // rows past the level clamped to its last row, not counted
         float qc[NORM_I], bc[NORM_I], qn[NORM_I][6];
-        const uint32_t dj_lo = j >= 1 ? sj : 0u, dj_hi = j + 1 < n.n1 ? sj : 0u;
-        const uint32_t dk_lo = k >= 1 ? 4u : 0u, dk_hi = k + 1 < n.n2 ? 4u : 0u;
 #pragma unroll
         for (int t = 0; t < NORM_I; t++) {
             const int i = min(ib + t, n.n0 - 1);
@@ -1388,9 +1406,8 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
             qc[t] = load_bytes(q, o);
             bc[t] = load_bytes(rhs, o);
             if (UNIT) {
-                const uint32_t om = i >= 1 ? o - si : o, op = i + 1 < n.n0 ? o + si : o;
-                qn[t][0] = load_bytes(q, om);
-                qn[t][1] = load_bytes(q, op);
+                qn[t][0] = load_bytes(q, i >= 1 ? o - si : o);
+                qn[t][1] = load_bytes(q, i + 1 < n.n0 ? o + si : o);
                 qn[t][2] = load_bytes(q, o - dj_lo);
                 qn[t][3] = load_bytes(q, o + dj_hi);
                 qn[t][4] = load_bytes(q, o - dk_lo);
@@ -1403,15 +1420,9 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
             if (i >= n.n0) break;
             const float q_c = qc[t], rhs_c = bc[t];
             float r;
-            if (UNIT && jk_interior && i >= 1 && i + 1 < n.n0) {
-                // every neighbour inside the level: couplings 1, Σc = 6, volume 1
-                float t0 = 0.0f + qn[t][0];
-                t0 = t0 + qn[t][1];
-                t0 = t0 + qn[t][2];
-                t0 = t0 + qn[t][3];
-                t0 = t0 + qn[t][4];
-                t0 = t0 + qn[t][5];
-                r = rhs_c - (t0 - 6.0f * q_c) / scale;
+            if (UNIT) {
+                const bool in[6] = {i >= 1, i + 1 < n.n0, jm, jp, km, kp};
+                r = unit_residual(q_c, rhs_c, scale, qn[t], in, couples_jk + unit_couples(L.g, 0, in[0], in[1]));
             } else {
                 float t0, t1;
                 neighbor_sum<UNIT>(L, i, j, k, GlobalQ{q, n}, t0, t1);
@@ -1760,16 +1771,16 @@ __global__ __launch_bounds__(NORM_THREADS) void k_refine_residual(LevelArgs L, c
     double rr = 0.0, bb = 0.0;
     if (k < n.n2 && j < n.n1) {
         const CorrectedQ<CORRECT, T> q{x, e, inv_scale, n};
-        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on interior cells
+        const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on unit levels
         const int si = n.n1 * n.n2, sj = n.n2;
-        const bool jk_interior = j >= 1 && j + 1 < n.n1 && k >= 1 && k + 1 < n.n2;
         const int ib = blockIdx.z * NORM_I;
+        const bool jm = j >= 1, jp = j + 1 < n.n1, km = k >= 1, kp = k + 1 < n.n2;
+        const int couples_jk = unit_couples(L.g, 1, jm, jp) + unit_couples(L.g, 2, km, kp);
+        const int dj_lo = jm ? sj : 0, dj_hi = jp ? sj : 0, dk_lo = km ? 1 : 0, dk_hi = kp ? 1 : 0;
         // column[t + 1] is the corrected q at i = ib + t (rows past the level clamped to its last row). Every
-        // load is unconditional (a neighbour outside the level reads the cell itself and goes unused), so the
+        // load is unconditional (a neighbour outside the level reads the cell itself and is masked), so the
         // loads issue as one batch.
         float column[NORM_I + 2], bc[NORM_I], side[NORM_I][4];
-        const int dj_lo = j >= 1 ? sj : 0, dj_hi = j + 1 < n.n1 ? sj : 0;
-        const int dk_lo = k >= 1 ? 1 : 0, dk_hi = k + 1 < n.n2 ? 1 : 0;
 #pragma unroll
         for (int t = -1; t <= NORM_I; t++) {
             const int i = max(0, min(ib + t, n.n0 - 1));
@@ -1792,15 +1803,10 @@ __global__ __launch_bounds__(NORM_THREADS) void k_refine_residual(LevelArgs L, c
             if (i >= n.n0) break;
             const float q_c = column[t + 1], rhs_c = bc[t];
             float r;
-            if (UNIT && jk_interior && i >= 1 && i + 1 < n.n0) {
-                // every neighbour inside the level: couplings 1, sum 6, volume 1; the reference's order
-                float t0 = 0.0f + column[t];
-                t0 = t0 + column[t + 2];
-                t0 = t0 + side[t][0];
-                t0 = t0 + side[t][1];
-                t0 = t0 + side[t][2];
-                t0 = t0 + side[t][3];
-                r = rhs_c - (t0 - 6.0f * q_c) / scale;
+            if (UNIT) {
+                const float qn[6] = {column[t], column[t + 2], side[t][0], side[t][1], side[t][2], side[t][3]};
+                const bool in[6] = {i >= 1, i + 1 < n.n0, jm, jp, km, kp};
+                r = unit_residual(q_c, rhs_c, scale, qn, in, couples_jk + unit_couples(L.g, 0, in[0], in[1]));
             } else {
                 float t0, t1;
                 neighbor_sum<UNIT>(L, i, j, k, q, t0, t1);
