@@ -283,6 +283,26 @@ __device__ __forceinline__ float residual_value(const LevelArgs &L, int i, int j
     return rhs_c - (t0 - t1 * q_c) / scale;
 }
 
+// The residual of one cell of a unit-coupling level (every coupling and cell volume 1, so the scale is
+// (1 cell_m) cell_m) from its six neighbours in the reference's order x-, x+, y-, y+, z-, z+, each passed as
+// (value, inside): an outside neighbour adds +0.0, what neighbor_sum adds for a ghost or a closed face.
+// `couples` is the sum of couplings, the faces with an inside neighbour or open: a sum of ones, exact in any
+// order. The same values as residual_value(neighbor_sum).
+__device__ __forceinline__ float unit_residual(float q_c, float rhs_c, float scale, const float (&qn)[6],
+                                               const bool (&in)[6], int couples) {
+    float t0 = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 6; m++) t0 = t0 + (in[m] ? qn[m] : 0.0f);
+    return rhs_c - (t0 - (float)couples * q_c) / scale;
+}
+
+// the faces of a cell along one axis that couple on a unit level: an inside neighbour, or an open face
+__device__ __forceinline__ int unit_couples(const plm_grid_info &g, int a, bool lo_in, bool hi_in) {
+    // bitwise, not ||: a short-circuit on the uniform flag would become a branch
+    const int open_lo = uniform(g.open_lo[a]) == 1 ? 1 : 0, open_hi = uniform(g.open_hi[a]) == 1 ? 1 : 0;
+    return ((int)lo_in | open_lo) + ((int)hi_in | open_hi);
+}
+
 struct GlobalQ {
     const float *q;
     Dims n;
@@ -854,50 +874,91 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
     const Dims nf = cell_dims(L.g);
     int i, j, k;
     if (!thread_cell(nc, i, j, k)) return;
-    float total = 0.0f, count = 0.0f;
-    // Interior coarse cells: every child's six neighbours lie inside the level, none of them the last cell on
-    // an axis (on unit-coupling levels the last cell may be one), so every coupling is 1, Σc = 6 and the
-    // volume 1. The 32 distinct q values of the children and their neighbours are loaded once.
-    const int edge = UNIT ? 3 : 4;
-    if (i >= 1 && 2 * i + edge <= nf.n0 && j >= 1 && 2 * j + edge <= nf.n1 && k >= 1 && 2 * k + edge <= nf.n2) {
+    float total = 0.0f;
+    int count = 0;
+    if constexpr (UNIT) {
+        // Unit-coupling level: the children's 4 x 4 x 4 neighbourhood (fine coordinates 2i - 1 ... 2i + 2 per
+        // axis) read at indices clamped into the level and masked (+0.0 outside, what neighbor_sum adds for a
+        // ghost or a closed face); a child past the level (an overhang child) adds +0.0 to a total that starts
+        // at +0.0 and is not counted. No load sits in a branch and no wavefront runs two paths; the same
+        // values as neighbor_sum + residual_value over the in-range children.
         constexpr uint32_t B = sizeof(T);
         const uint32_t si = (uint32_t)(nf.n1 * nf.n2) * B, sj = (uint32_t)nf.n2 * B;
-        const uint32_t o = offset_of(nf, 2 * i, 2 * j, 2 * k) * B;
-        // q at child-block coordinates (a, b, c) in [-1, 2]^3 (only the block and its face neighbours are read)
-        auto Q = [&](int a, int b, int c) { return load_bytes(q, o + a * si + b * sj + c * B); };
+        uint32_t oi[4], oj[4], ok[4];
+        bool in_i[4], in_j[4], in_k[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int x = 2 * i - 1 + m, y = 2 * j - 1 + m, z = 2 * k - 1 + m;
+            in_i[m] = x >= 0 && x < nf.n0;
+            in_j[m] = y >= 0 && y < nf.n1;
+            in_k[m] = z >= 0 && z < nf.n2;
+            oi[m] = (uint32_t)max(0, min(x, nf.n0 - 1)) * si;
+            oj[m] = (uint32_t)max(0, min(y, nf.n1 - 1)) * sj;
+            ok[m] = (uint32_t)max(0, min(z, nf.n2 - 1)) * B;
+        }
+        auto Q = [&](int a, int b, int c) { return load_bytes(q, oi[a] + oj[b] + ok[c]); };
         const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 1; a <= 2; a++)
 #pragma unroll
-            for (int b = 0; b < 2; b++)
+            for (int b = 1; b <= 2; b++)
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    float t0 = 0.0f + Q(a - 1, b, c);
-                    t0 = t0 + Q(a + 1, b, c);
-                    t0 = t0 + Q(a, b - 1, c);
-                    t0 = t0 + Q(a, b + 1, c);
-                    t0 = t0 + Q(a, b, c - 1);
-                    t0 = t0 + Q(a, b, c + 1);
-                    const float q_c = Q(a, b, c), rhs_c = load_bytes(rhs, o + a * si + b * sj + c * B);
-                    total = total + (rhs_c - (t0 - 6.0f * q_c) / scale);
-                    count = count + 1.0f;
+                for (int c = 1; c <= 2; c++) {
+                    const float qn[6] = {Q(a - 1, b, c), Q(a + 1, b, c), Q(a, b - 1, c),
+                                         Q(a, b + 1, c), Q(a, b, c - 1), Q(a, b, c + 1)};
+                    const bool in[6] = {in_i[a - 1], in_i[a + 1], in_j[b - 1], in_j[b + 1], in_k[c - 1], in_k[c + 1]};
+                    const int couples = unit_couples(L.g, 0, in[0], in[1]) + unit_couples(L.g, 1, in[2], in[3]) +
+                                        unit_couples(L.g, 2, in[4], in[5]);
+                    const float r = unit_residual(Q(a, b, c), load_bytes(rhs, oi[a] + oj[b] + ok[c]), scale, qn, in,
+                                                  couples);
+                    const bool exists = in_i[a] && in_j[b] && in_k[c];
+                    total = total + (exists ? r : 0.0f);
+                    count += exists ? 1 : 0;
                 }
     } else {
-        for (int di = 0; di < 2; di++)
-            for (int dj = 0; dj < 2; dj++)
-                for (int dk = 0; dk < 2; dk++) {
-                    const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
-                    if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
-                        float t0, t1;
-                        neighbor_sum<UNIT>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
-                        const int c = nf.at(fi, fj, fk);
-                        total = total + residual_value(L, fi, fj, fk, to_f32(q[c]), to_f32(rhs[c]), t0, t1);
-                        count = count + 1.0f;
+        // Interior coarse cells: every child's six neighbours lie inside the level and none is the last cell
+        // on an axis, so every coupling is 1, the sum of couplings 6 and the volume 1; the 32 distinct q
+        // values of the children and their neighbours are loaded once. The rest take the general update.
+        if (i >= 1 && 2 * i + 4 <= nf.n0 && j >= 1 && 2 * j + 4 <= nf.n1 && k >= 1 && 2 * k + 4 <= nf.n2) {
+            constexpr uint32_t B = sizeof(T);
+            const uint32_t si = (uint32_t)(nf.n1 * nf.n2) * B, sj = (uint32_t)nf.n2 * B;
+            const uint32_t o = offset_of(nf, 2 * i, 2 * j, 2 * k) * B;
+            // q at child-block coordinates (a, b, c) in [-1, 2]^3 (only the block and its face neighbours)
+            auto Q = [&](int a, int b, int c) { return load_bytes(q, o + a * si + b * sj + c * B); };
+            const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        float t0 = 0.0f + Q(a - 1, b, c);
+                        t0 = t0 + Q(a + 1, b, c);
+                        t0 = t0 + Q(a, b - 1, c);
+                        t0 = t0 + Q(a, b + 1, c);
+                        t0 = t0 + Q(a, b, c - 1);
+                        t0 = t0 + Q(a, b, c + 1);
+                        const float q_c = Q(a, b, c), rhs_c = load_bytes(rhs, o + a * si + b * sj + c * B);
+                        total = total + (rhs_c - (t0 - 6.0f * q_c) / scale);
+                        count += 1;
                     }
-                }
+        } else {
+            for (int di = 0; di < 2; di++)
+                for (int dj = 0; dj < 2; dj++)
+                    for (int dk = 0; dk < 2; dk++) {
+                        const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+                        if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
+                            float t0, t1;
+                            neighbor_sum<UNIT>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
+                            const int c = nf.at(fi, fj, fk);
+                            total = total + residual_value(L, fi, fj, fk, to_f32(q[c]), to_f32(rhs[c]), t0, t1);
+                            count += 1;
+                        }
+                    }
+        }
     }
     const int c = nc.at(i, j, k);
-    coarse_rhs[c] = from_f32<T>(total / count);
+    coarse_rhs[c] = from_f32<T>(total / (float)count);
     if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
 }
 
@@ -1358,24 +1419,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int offset = 32; offset > 0; offset /= 2) v += __shfl_xor(v, offset, 64);
     return v;
-}
-
-// The residual of one cell of a unit-coupling level (every coupling and cell volume 1, so the scale is
-// (1 cell_m) cell_m) from its six neighbours in the reference's order x-, x+, y-, y+, z-, z+, each passed as
-// (value, inside): an outside neighbour adds +0.0, what neighbor_sum adds for a ghost or a closed face.
-// `couples` is the sum of couplings, the faces with an inside neighbour or open: a sum of ones, exact in any
-// order. The same values as residual_value(neighbor_sum).
-__device__ __forceinline__ float unit_residual(float q_c, float rhs_c, float scale, const float (&qn)[6],
-                                               const bool (&in)[6], int couples) {
-    float t0 = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 6; m++) t0 = t0 + (in[m] ? qn[m] : 0.0f);
-    return rhs_c - (t0 - (float)couples * q_c) / scale;
-}
-
-// the faces of a cell along one axis that couple on a unit level: an inside neighbour, or an open face
-__device__ __forceinline__ int unit_couples(const plm_grid_info &g, int a, bool lo_in, bool hi_in) {
-    return ((lo_in || uniform(g.open_lo[a]) == 1) ? 1 : 0) + ((hi_in || uniform(g.open_hi[a]) == 1) ? 1 : 0);
 }
 
 // per block: sum of r^2 and rhs^2 over its 64 x 4 x NORM_I cells, in a fixed order (per thread along i, then
