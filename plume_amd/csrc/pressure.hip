@@ -962,6 +962,58 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
     if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
 }
 
+// k_residual_restrict for levels with couplings other than 1, with one thread per child: lanes 8 m + 4 di +
+// 2 dj + dk hold the children of coarse cell m of the wavefront. A child whose six neighbours are inside the
+// level and none of them (nor the child) the last cell on an axis has couplings 1, a sum of couplings of 6
+// and volume 1 (the interior form); the others take the general update. Lane 8 m then adds its cell's eight
+// residuals in the reference's child order (an absent, overhang child adds +0.0 to a total that starts at
+// +0.0, which changes no bit) and divides by the count. Eight short threads instead of one thread walking
+// eight children keep the small levels' latency hidden.
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L, const T *__restrict__ q,
+                                                                    const T *__restrict__ rhs,
+                                                                    T *__restrict__ coarse_rhs, Dims nc,
+                                                                    const double *gate) {
+    if (gated_off(gate)) return;
+    const Dims nf = cell_dims(L.g);
+    const int child = threadIdx.x & 7;
+    const int cell = (int)(blockIdx.x * 32u + (threadIdx.x >> 3));
+    const bool valid = cell < nc.count();
+    const int row = cell / nc.n2, k = cell - row * nc.n2, i = row / nc.n1, j = row - i * nc.n1;
+    const int fi = 2 * i + (child >> 2), fj = 2 * j + ((child >> 1) & 1), fk = 2 * k + (child & 1);
+    const bool exists = valid && fi < nf.n0 && fj < nf.n1 && fk < nf.n2;
+    float r = 0.0f;
+    if (exists) {
+        const float q_c = to_f32(q[nf.at(fi, fj, fk)]), rhs_c = to_f32(rhs[nf.at(fi, fj, fk)]);
+        if (fi >= 1 && fi + 3 <= nf.n0 && fj >= 1 && fj + 3 <= nf.n1 && fk >= 1 && fk + 3 <= nf.n2) {
+            const StoredQ<T> Q{q, nf};
+            float t0 = 0.0f + Q(fi - 1, fj, fk);
+            t0 = t0 + Q(fi + 1, fj, fk);
+            t0 = t0 + Q(fi, fj - 1, fk);
+            t0 = t0 + Q(fi, fj + 1, fk);
+            t0 = t0 + Q(fi, fj, fk - 1);
+            t0 = t0 + Q(fi, fj, fk + 1);
+            const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m
+            r = rhs_c - (t0 - 6.0f * q_c) / scale;
+        } else {
+            float t0, t1;
+            neighbor_sum<false>(L, fi, fj, fk, StoredQ<T>{q, nf}, t0, t1);
+            r = residual_value(L, fi, fj, fk, q_c, rhs_c, t0, t1);
+        }
+    }
+    const int lead = (int)(threadIdx.x & 63u) & ~7;
+    float total = 0.0f;
+    int count = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        total = total + __shfl(r, lead + c, 64);
+        count += __shfl(exists ? 1 : 0, lead + c, 64);
+    }
+    if (child == 0 && valid) coarse_rhs[cell] = from_f32<T>(total / (float)count);
+}
+
+inline dim3 children_grid(const Dims &nc) { return dim3((unsigned)((nc.count() + 31) / 32)); }
+
 // prolong_add_kernel, sim/multigrid.py:63-68
 __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ coarse, Dims nc, float *__restrict__ fine,
                                                      Dims nf, const double *gate) {
@@ -1672,8 +1724,8 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
         k_residual_restrict<true, float><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
                                                                 nullptr, nc, h.gate);
     else
-        k_residual_restrict<false, float><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
-                                                                 nullptr, nc, h.gate);
+        k_residual_restrict_children<float><<<children_grid(nc), 256, 0, s>>>(L, h.q[index], h.rhs[index],
+                                                                             h.rhs[index + 1], nc, h.gate);
     PLM_TRY(check_launch("residual restrict"));
     PLM_TRY(v_cycle(h, index + 1, true, s));
     return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, FROM_PROLONG,
@@ -1961,8 +2013,8 @@ int v_cycle_lp(const LpHierarchy<T> &h, int index, bool cleared, hipStream_t s) 
         k_residual_restrict<true, T><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
                                                                    nullptr, nc, nullptr);
     else
-        k_residual_restrict<false, T><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
-                                                                    nullptr, nc, nullptr);
+        k_residual_restrict_children<T><<<children_grid(nc), 256, 0, s>>>(L, h.q[index], h.rhs[index],
+                                                                         h.rhs[index + 1], nc, nullptr);
     PLM_TRY(check_launch("residual restrict"));
     PLM_TRY(v_cycle_lp(h, index + 1, true, s));
     return lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, FROM_PROLONG, h.q[index + 1], nc, s);
