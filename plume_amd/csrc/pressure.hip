@@ -259,6 +259,49 @@ __device__ __forceinline__ void neighbor_sum_padded(const LevelArgs &L, int i, i
     }
 }
 
+// The coupling table of a level with couplings other than 1 as an LDS array: Coupling.in (3 x 4 x 2 floats),
+// Coupling.ghost (the same) and +0.0 at CP_ZERO. Selecting among uniform values costs a move and a select
+// per value (one scalar operand per vector instruction); an indexed LDS read is one instruction.
+constexpr int CP_GHOST = 24, CP_ZERO = 48, CP_WORDS = 49;
+
+// neighbor_sum_padded for a level with couplings other than 1, every coupling read from the LDS table:
+// per neighbour one read, of the in-domain coupling, the open-face ghost coupling or +0.0 as the neighbour
+// is inside, beyond an open face or beyond a closed one. An outside neighbour's q is +0.0, so its t0 term
+// is +0.0 whichever of these it is multiplied by, as in neighbor_sum. The same values in the same order.
+template <typename Q>
+__device__ __forceinline__ void neighbor_sum_lds(const LevelArgs &L, const float *cps, int i, int j, int k, Q q,
+                                                 float &t0, float &t1) {
+    const plm_grid_info &g = L.g;
+    const int cell[3] = {i, j, k};
+    const bool last[3] = {i == g.dims[0] - 1, j == g.dims[1] - 1, k == g.dims[2] - 1};
+    t0 = 0.0f;
+    t1 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const bool open_lo = uniform(g.open_lo[a]) == 1, open_hi = uniform(g.open_hi[a]) == 1;
+        const int row = a * 8 + (last[b] ? 2 : 0) + (last[c] ? 4 : 0);  // Coupling.in[a][last_b + 2 last_c]
+        {
+            int nb[3] = {i, j, k};
+            nb[a] -= 1;
+            const int pair_last = last[a] ? 1 : 0;
+            const int index = cell[a] > 0 ? row + pair_last : open_lo ? CP_GHOST + row + pair_last : CP_ZERO;
+            const float cp = cps[index];
+            t0 = t0 + cp * q(nb[0], nb[1], nb[2]);
+            t1 = t1 + cp;
+        }
+        {
+            int nb[3] = {i, j, k};
+            nb[a] += 1;
+            const int pair_last = cell[a] + 1 == g.dims[a] - 1 ? 1 : 0;
+            const int index = !last[a] ? row + pair_last : open_hi ? CP_GHOST + row + 1 : CP_ZERO;
+            const float cp = cps[index];
+            t0 = t0 + cp * q(nb[0], nb[1], nb[2]);
+            t1 = t1 + cp;
+        }
+    }
+}
+
 // cell_volume, sim/pressure.py:47-49: (F_0 F_1) F_2 with F = fill_hi on the last cell of an axis, else 1
 __device__ __forceinline__ float cell_volume(const LevelArgs &L, int i, int j, int k) {
     const plm_grid_info &g = L.g;
@@ -617,6 +660,17 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
     static_assert(RI * RJ <= THREADS, "one thread per ring row for the k-halo red cells");
     if (gated_off(gate)) return;
     __shared__ float s[LI * LJ * LK];
+    __shared__ float cps[UNIT ? 1 : CP_WORDS];  // the coupling table (levels with couplings other than 1)
+    if (!UNIT && threadIdx.x == 0) {
+        // constant indices: read straight from the kernel arguments
+        const float *table = &L.cp.in[0][0][0];
+#pragma unroll
+        for (int m = 0; m < 24; m++) cps[m] = table[m];
+        const float *ghost = &L.cp.ghost[0][0][0];
+#pragma unroll
+        for (int m = 0; m < 24; m++) cps[CP_GHOST + m] = ghost[m];
+        cps[CP_ZERO] = 0.0f;
+    }
     const Dims n = cell_dims(L.g);
     const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
     const int tk = tile % tiles_k, tr = tile / tiles_k, tj = tr % tiles_j, ti = tr / tiles_j;
@@ -656,7 +710,10 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
     };
     auto update = [&](int i, int j, int k, float rhs_c) {
         float t0, t1;
-        neighbor_sum_padded<UNIT>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
+        if (UNIT)
+            neighbor_sum_padded<true>(L, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
+        else
+            neighbor_sum_lds(L, cps, i, j, k, [&](int a, int b, int c) { return at(a, b, c); }, t0, t1);
         float &cell = at(i, j, k);
         cell = gs_update(L, i, j, k, cell, rhs_c, t0, t1, relaxation);
     };
