@@ -1184,10 +1184,10 @@ struct TailLevel {
     float cell_m;
     int unit;
     FastDiv div_n1, div_n2, div_half;
-    // Coupling.in[a][area][1] and Coupling.ghost[a][area][*]: the couplings that are not the area itself,
-    // selected per lane instead of divided out per neighbour
-    float in1[3][4];
-    float ghost[3][2][4];  // [axis][this cell is the last][area]
+    // The level's coupling table in the staged LDS copy: Coupling.in[a][area][1] and Coupling.ghost, the
+    // couplings that are not the area itself, read per lane with one indexed LDS load (a select among
+    // uniform values costs a move and a select per value)
+    const Coupling *cp;
 };
 
 
@@ -1215,13 +1215,8 @@ __device__ __forceinline__ TailLevel load_level(const TailLevels &t, int m) {
         r.open_lo[a] = uniform(L.g.open_lo[a]);
         r.open_hi[a] = uniform(L.g.open_hi[a]);
         r.fill[a] = uniform(L.g.fill_hi[a]);
-#pragma unroll
-        for (int area = 0; area < 4; area++) {
-            r.in1[a][area] = uniform(L.cp.in[a][area][1]);
-            r.ghost[a][0][area] = uniform(L.cp.ghost[a][area][0]);
-            r.ghost[a][1][area] = uniform(L.cp.ghost[a][area][1]);
-        }
     }
+    r.cp = &L.cp;
     r.cell_m = uniform(L.g.cell_m);
     r.unit = uniform(L.unit);
     r.div_n1 = FastDiv{uniform(t.div_n1[m].d), uniform(t.div_n1[m].m)};
@@ -1233,10 +1228,6 @@ __device__ __forceinline__ TailLevel load_level(const TailLevels &t, int m) {
 // cell_fill, sim/pressure.py:39-44
 __device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
     return index == L.n[a] - 1 ? L.fill[a] : 1.0f;
-}
-
-__device__ __forceinline__ float pick4(const float (&v)[4], bool last_b, bool last_c) {
-    return sel4(v[0], v[1], v[2], v[3], last_b, last_c);
 }
 
 // LDS word 0 holds +0.0: the q a pass reads for a neighbour outside the level (a ghost or a closed face,
@@ -1269,19 +1260,20 @@ __device__ __forceinline__ PlanCell<ONES> plan_cell(const TailLevel &L, int i, i
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const int area_index = (last[b] ? 1 : 0) + (last[c] ? 2 : 0);
         float area = 1.0f, in1 = 1.0f;
         if (!ONES) {
             area = area * (last[b] ? L.fill[b] : 1.0f);
             area = area * (last[c] ? L.fill[c] : 1.0f);
-            in1 = pick4(L.in1[a], last[b], last[c]);
+            in1 = L.cp->in[a][area_index][1];
         }
         const bool in_lo = cell[a] > 0, in_hi = !last[a];
         mask |= (in_lo ? 1 : 0) << (2 * a);
         mask |= (in_hi ? 1 : 0) << (2 * a + 1);
         const float cp_lo = ONES ? 1.0f : (last[a] ? in1 : area);
         const float cp_hi = ONES ? 1.0f : (cell[a] + 1 == L.n[a] - 1 ? in1 : area);
-        const float ghost_lo = last[a] ? pick4(L.ghost[a][1], last[b], last[c]) : pick4(L.ghost[a][0], last[b], last[c]);
-        const float ghost_hi = pick4(L.ghost[a][1], last[b], last[c]);
+        const float ghost_lo = L.cp->ghost[a][area_index][last[a] ? 1 : 0];
+        const float ghost_hi = L.cp->ghost[a][area_index][1];
         t1 = t1 + (in_lo ? cp_lo : L.open_lo[a] == 1 ? ghost_lo : 0.0f);
         t1 = t1 + (in_hi ? cp_hi : L.open_hi[a] == 1 ? ghost_hi : 0.0f);
         if (!ONES) {
