@@ -764,7 +764,7 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         const float value = to_f32(q_in[n.at(ci, cj, ck)]) + prolong_at_row(coarse, nc, ci, cj, pk0, pk1, tz);
         return in ? value : 0.0f;
     };
-    constexpr int ROWS = LI * LJ / WAVES, BATCH = START == FROM_PROLONG ? 2 : ROWS % 6 == 0 ? 6 : 8;
+    constexpr int ROWS = LI * LJ / WAVES, BATCH = ROWS % 6 == 0 ? 6 : START == FROM_PROLONG ? 4 : 8;
     static_assert((LI * LJ) % WAVES == 0 && ROWS % BATCH == 0, "halo rows split evenly over wavefronts");
     for (int b = 0; b < ROWS; b += BATCH) {
         float v[BATCH];
