@@ -168,10 +168,70 @@ struct Faces3 {
     Field f[3];
 };
 
-__global__ __launch_bounds__(256) void k_velocity_forward(plm_grid_info g, Faces3 faces, float dto) {
+// t / d for any 32-bit t: the multiply-high by m = ceil(2^32 / d) is floor(t / d) or one more (its error
+// t (m d - 2^32) / (d 2^32) is below 1), corrected once
+struct ExtentDiv {
+    uint32_t d, m;
+    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
+        if (d == 1) return t;
+        const uint32_t q = __umulhi(t, m);
+        return q * d > t ? q - 1 : q;
+    }
+};
+
+ExtentDiv extent_div(int d) {
+    return ExtentDiv{(uint32_t)d, d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d)};
+}
+
+// The three face arrays of a grid in one 1D launch, each enumerated in its own C order (one thread per face,
+// 256 per block, a block never spanning two arrays): every wavefront holds 64 faces, where the (k, j) blocks
+// of grid_faces leave most lanes of a 129-face row's third wavefront idle
+constexpr int FACE_BLOCK = 256;
+struct FaceLaunch {
+    uint32_t block_end[3];  // cumulative blocks: u in [0, block_end[0]), v, then w
+    uint32_t count[3];
+    ExtentDiv div_n2[3], div_n1[3];
+};
+
+FaceLaunch face_launch(const plm_grid_info &g) {
+    FaceLaunch f{};
+    uint32_t blocks = 0;
+    for (int a = 0; a < 3; a++) {
+        const int n0 = g.dims[0] + (a == 0), n1 = g.dims[1] + (a == 1), n2 = g.dims[2] + (a == 2);
+        f.count[a] = (uint32_t)n0 * (uint32_t)n1 * (uint32_t)n2;
+        blocks += (f.count[a] + FACE_BLOCK - 1) / FACE_BLOCK;
+        f.block_end[a] = blocks;
+        f.div_n2[a] = extent_div(n2);
+        f.div_n1[a] = extent_div(n1);
+    }
+    return f;
+}
+
+// The block's face array (uniform) and the thread's face in it; false past the array's end. The per-array
+// values are picked with constant indices (a runtime index into the kernel arguments would copy them to
+// scratch).
+__device__ __forceinline__ bool packed_face(const plm_grid_info &g, const FaceLaunch &f, int &axis, int &i, int &j,
+                                            int &k) {
+    const uint32_t b = blockIdx.x;
+    axis = b < f.block_end[0] ? 0 : b < f.block_end[1] ? 1 : 2;
+    const uint32_t first = axis == 0 ? 0u : axis == 1 ? f.block_end[0] : f.block_end[1];
+    const uint32_t count = axis == 0 ? f.count[0] : axis == 1 ? f.count[1] : f.count[2];
+    const ExtentDiv dn2 = axis == 0 ? f.div_n2[0] : axis == 1 ? f.div_n2[1] : f.div_n2[2];
+    const ExtentDiv dn1 = axis == 0 ? f.div_n1[0] : axis == 1 ? f.div_n1[1] : f.div_n1[2];
+    const uint32_t t = (b - first) * FACE_BLOCK + threadIdx.x;
+    if (t >= count) return false;
+    const uint32_t r = dn2(t), c = dn1(r);
+    k = (int)(t - r * dn2.d);
+    i = (int)c;
+    j = (int)(r - c * dn1.d);
+    (void)g;
+    return true;
+}
+
+__global__ __launch_bounds__(FACE_BLOCK) void k_velocity_forward(plm_grid_info g, FaceLaunch fl, Faces3 faces,
+                                                                 float dto) {
     int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
     const float *u = faces.f[0].data, *v = faces.f[1].data, *w = faces.f[2].data;
     if (axis == 0)
         forward_sample<1, 1>(g, &faces.f[0], u, v, w, i, j, k, dto);
@@ -181,10 +241,10 @@ __global__ __launch_bounds__(256) void k_velocity_forward(plm_grid_info g, Faces
         forward_sample<3, 1>(g, &faces.f[2], u, v, w, i, j, k, dto);
 }
 
-__global__ __launch_bounds__(256) void k_velocity_correct2(plm_grid_info g, Faces3 faces, float dto) {
+__global__ __launch_bounds__(FACE_BLOCK) void k_velocity_correct2(plm_grid_info g, FaceLaunch fl, Faces3 faces,
+                                                                  float dto) {
     int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
     const float *u = faces.f[0].data, *v = faces.f[1].data, *w = faces.f[2].data;
     if (axis == 0)
         correct_sample<1, 1>(g, &faces.f[0], u, v, w, i, j, k, dto);
@@ -235,9 +295,10 @@ int plm_advect_velocity_bounded(const plm_grid_info *info, const float *u, const
     Faces3 faces{{Field{u, u_hat, u_bounds, u_next, 0.0f, 0}, Field{v, v_hat, v_bounds, v_next, 0.0f, 0},
                   Field{w, w_hat, w_bounds, w_next, 0.0f, 0}}};
     TimingScope t("advect", s);
-    k_velocity_forward<<<grid_faces(*info), block3(), 0, s>>>(*info, faces, dt_over_cell);
+    const FaceLaunch fl = face_launch(*info);
+    k_velocity_forward<<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, faces, dt_over_cell);
     PLM_TRY(check_launch("plm_advect_velocity_bounded forward"));
-    k_velocity_correct2<<<grid_faces(*info), block3(), 0, s>>>(*info, faces, dt_over_cell);
+    k_velocity_correct2<<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, faces, dt_over_cell);
     return check_launch("plm_advect_velocity_bounded correct");
 }
 
