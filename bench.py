@@ -1,6 +1,6 @@
 """Benchmark: cell-substeps/s of the full simulation substep at 128 x 192 x 128 on one MI355X.
 
-A step is one substep of the buoyant smoke puff (config 2 of the benchmark plan): emit (while active),
+A step is one substep of the buoyant smoke puff on a 128 x 192 x 128 grid: emit (while active),
 buoyancy, vorticity confinement, MacCormack advection of u, v, w, projection with multigrid V(2,2) cycles
 to a 1e-5 relative residual (FP64 norm on the device), MacCormack advection of smoke (with the smoke
 budget) and temperature. Inputs live in HBM before the timed region; W warmup substeps (the 12 emitting
@@ -33,7 +33,7 @@ CELL_M = 0.01
 
 
 def scene_table(cells=GRID, frames=96, radius_cells=12, tolerance=1e-5, cycles=2):
-    """Config 2: the smoke puff (plume_amd.sim.scenes.puff_table) with alpha 0.1, beta 1, confinement 0.3."""
+    """The smoke puff (plume_amd.sim.scenes.puff_table) with alpha 0.1, beta 1, confinement 0.3."""
     from plume_amd.sim.scenes import puff_table
 
     return puff_table(cells, frames=frames, radius_cells=radius_cells, tolerance=tolerance, cycles=cycles)
@@ -133,7 +133,7 @@ MIXED_GRID = (256, 384, 256)
 
 
 def mixed_bench(args) -> dict:
-    """Mixed-precision pressure projection at 256x384x256 (benchmark config 5): the seeded divergent field
+    """Mixed-precision pressure projection at 256x384x256: the seeded divergent field
     (64 Gaussian vortex blobs plus a radial emitter source), solved from q = 0 to a common relative residual
     by the plain FP32 multigrid (tolerance mode) and by FP32 iterative refinement around BF16 and FP16
     multigrid with an FP64 coarsest 8x12x8 level. The common target is max(tolerance, 2 x the FP32 floor),
@@ -166,7 +166,7 @@ def mixed_bench(args) -> dict:
 
 
 def ensemble_bench(args) -> dict:
-    """Batched ensemble (benchmark config 4): 32 smoke puffs with seeded sizes over 48^3, 64x96x64, 96x128x96
+    """Batched ensemble: 32 smoke puffs with seeded sizes over 48^3, 64x96x64, 96x128x96
     and 128x192x128, emitter radii 4-12 cells, alpha in [0.05, 0.2] and epsilon in [0, 0.5]. The 12 emitting
     frames and one more run eagerly; then the same number of frames is timed twice from the state reached:
     batched (one captured HIP graph launch per two substeps of all 32 members, members on 4 streams) and plain
@@ -216,7 +216,7 @@ def ensemble_bench(args) -> dict:
 
 
 def adjoint_bench(args) -> dict:
-    """Differentiable rollout (benchmark config 3): the puff at 64x96x64 from rest, 32 substeps, the (heat,
+    """Differentiable rollout: the puff at 64x96x64 from rest, 32 substeps, the (heat,
     thickness) images of 128x128 pixels after every 8 substeps (the state is stored only there), the L2 loss
     against the images of the same rollout with smoke, heat and lift sources 10% stronger; gradients with
     respect to the three per-cell source fields and alpha. Times the forward rollout alone and forward plus
@@ -313,7 +313,7 @@ def main() -> int:
                         help="only run the streaming-copy probe over MIB MiB (HBM counter calibration) and exit")
     args = parser.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU; there is no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
 
@@ -401,7 +401,7 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "cell-substeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded smoke-puff scene, emitter on for frames 0-11)",
-        "config": {"workload": "buoyant smoke puff substep, 128x192x128 FP32 MAC grid (benchmark config 2)",
+        "config": {"workload": "buoyant smoke puff substep, 128x192x128 FP32 MAC grid",
                    "grid": list(GRID), "cell_m": CELL_M, "dt_s": dt,
                    "pressure": f"multigrid V(2,2), relative residual <= {args.tolerance:g} (FP64 norm), "
                                f"mean {mean_cycles:.2f} cycles",

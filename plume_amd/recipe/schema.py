@@ -175,7 +175,7 @@ class LookSettings:
 
 @dataclass(frozen=True, kw_only=True)
 class Recipe:
-    extends: str = setting("", "", "Parent recipe (resolved before this build sees the table).")
+    extends: str = setting("", "", "Parent recipe (resolved before the table is read).")
     seed: int = setting(0, "", f"Seed for every random choice, 0 to {SEED_MAX}.")
     domain: DomainSettings = section("The simulation box.")
     time: TimeSettings = section("Effect time and frames.")

@@ -1,6 +1,6 @@
 """Choosing the device (sim/device.py:12-30 of the reference).
 
-The reference runs its Warp kernels on "cuda:0" or "cpu". This build has one compute path, the HIP kernels on
+The reference runs its Warp kernels on "cuda:0" or "cpu". This package has one compute path, the HIP kernels on
 the ROCm GPU (which torch names "cuda"), so `--device cpu` and a machine without a GPU are errors.
 """
 

@@ -52,7 +52,7 @@ class Member:
 
 def ensemble_tables(count: int, seed: int, frames: int = 48, sizes=ENSEMBLE_SIZES) -> list[dict]:
     """`count` puff recipes drawn from `seed`: a grid size from `sizes`, an emitter radius of 4-12 cells,
-    buoyancy alpha in [0.05, 0.2] and confinement epsilon in [0, 0.5] each (benchmark config 4)."""
+    buoyancy alpha in [0.05, 0.2] and confinement epsilon in [0, 0.5] each."""
     rng = np.random.default_rng(seed)
     tables = []
     for index in range(count):

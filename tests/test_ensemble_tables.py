@@ -1,4 +1,4 @@
-"""The ensemble's member recipes (benchmark config 4): sizes, emitter radii, buoyancy and confinement drawn
+"""The ensemble's member recipes: sizes, emitter radii, buoyancy and confinement drawn
 from a seed within their ranges. No GPU calls."""
 
 
