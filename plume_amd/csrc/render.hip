@@ -72,11 +72,22 @@ __global__ __launch_bounds__(256) void k_light_volume(Dims n, const float *__res
     float column = 0.0f;
     if (steps > 0) {
         const float step = length / (float)steps;
-        for (int s = 0; s < steps; s++) {
+        auto sample = [&](int s) {
             const float t = ((float)s + 0.5f) * step;
             const float px = start[0] + lx * t, py = start[1] + ly * t, pz = start[2] + lz * t;
-            column = column + fmax_(sample_index(smoke, n, px - 0.5f, py - 0.5f, pz - 0.5f), 0.0f) * step;
+            return sample_index(smoke, n, px - 0.5f, py - 0.5f, pz - 0.5f);
+        };
+        // four samples' taps in flight at once, then added to the column in step order
+        constexpr int UNROLL = 4;
+        int s = 0;
+        for (; s + UNROLL <= steps; s += UNROLL) {
+            float v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) v[u] = sample(s + u);
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) column = column + fmax_(v[u], 0.0f) * step;
         }
+        for (; s < steps; s++) column = column + fmax_(sample(s), 0.0f) * step;
     }
     light[n.at(i, j, k)] = expf(-opacity_per_density * column * cell_m);
 }
