@@ -156,6 +156,7 @@ def _pow2_scale(rms: float) -> float:
     """The power of two that brings `rms` nearest 1 (exact to apply in any floating-point type)."""
     return 2.0 ** (-round(math.log2(rms)))
 
+
 def divergent_velocity(grid, seed: int, blobs: int = 64, emitter_radius_cells: float = 12.0,
                        emitter_speed: float = 1.0) -> None:
     """Fills grid.u/v/w with a seeded divergent field: `blobs` Gaussian vortex blobs (random centre, axis,
