@@ -136,7 +136,8 @@ def mixed_bench(args) -> dict:
     """Mixed-precision pressure projection at 256x384x256: the seeded divergent field
     (64 Gaussian vortex blobs plus a radial emitter source), solved from q = 0 to a common relative residual
     by the plain FP32 multigrid (tolerance mode) and by FP32 iterative refinement around BF16 and FP16
-    multigrid with an FP64 coarsest 8x12x8 level. The common target is max(tolerance, 2 x the FP32 floor),
+    multigrid with an FP64 coarsest 8x12x8 level, every form with the same V(sweeps, sweeps) cycle. The
+    common target is max(tolerance, 2 x the FP32 floor),
     the floor being where plain FP32 V-cycles stall (below it no FP32 pressure gets; the FP64 oracle's
     solution rounded to FP32 sits at the same level). Solve time, FP64 relative residual and
     post-projection divergence per form; the errors against an FP64 oracle solution are
@@ -150,7 +151,7 @@ def mixed_bench(args) -> dict:
     target = max(args.mixed_tolerance, 2.0 * floor)
     print(f"[bench] mixed: FP32 floor {floor:.3g}, target {target:.3g}", file=sys.stderr, flush=True)
     results = {}
-    for name, (entry, _) in solve_forms(problem, target).items():
+    for name, (entry, _) in solve_forms(problem, target, sweeps=args.mixed_sweeps).items():
         results[name] = entry
         print(f"[bench] mixed {name}: {entry['seconds'] * 1e3:.2f} ms, residual {entry['relative_residual']:.3g}, "
               f"post div {entry['post_projection_divergence']:.3g}", file=sys.stderr, flush=True)
@@ -161,6 +162,7 @@ def mixed_bench(args) -> dict:
                       "multigrid with FP32 iterative refinement)", "grid": list(MIXED_GRID),
             "tolerance": args.mixed_tolerance, "fp32_floor": floor, "target": target,
             "target_rule": "max(tolerance, 2 x FP32 floor); floor = where plain FP32 V-cycles stall",
+            "sweeps": args.mixed_sweeps,
             "data": "synthetic (seeded divergent field: 64 Gaussian vortex blobs + radial emitter source)",
             "forms": results}
 
@@ -306,6 +308,9 @@ def main() -> int:
     parser.add_argument("--mixed", action="store_true",
                         help="run the 256x384x256 mixed-precision projection benchmark instead and exit")
     parser.add_argument("--mixed-tolerance", type=float, default=1e-6)
+    parser.add_argument("--mixed-sweeps", type=int, default=2,
+                        help="smoothing sweeps per level and direction, the same for every form (2: the puff "
+                             "recipe's V(2,2))")
     parser.add_argument("--adjoint", action="store_true",
                         help="run the 64x96x64 differentiable-rollout benchmark instead and exit")
     parser.add_argument("--adjoint-repeats", type=int, default=3)

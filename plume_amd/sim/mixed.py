@@ -243,9 +243,10 @@ def divergent_problem(cells, device: str, seed: int = 20261208) -> Problem:
 
 
 def solve_forms(problem: Problem, target: float, forms=("f32", "bf16", "f16"), max_levels: int = 6,
-                repeats: int = 2) -> dict:
+                repeats: int = 2, sweeps: int = 2) -> dict:
     """Solves the problem from q = 0 to `target` with each form: "f32" the plain FP32 multigrid in tolerance
-    mode (autotuned tiles), "bf16" / "f16" iterative refinement around that storage. The last of `repeats`
+    mode (autotuned tiles), "bf16" / "f16" iterative refinement around that storage, every form with
+    V(sweeps, sweeps) cycles. The last of `repeats`
     solves is timed (the first warms the code paths). Returns {form: (entry, pressure)} where entry holds the
     seconds, cells per second, FP64 relative residual and post-projection divergence |div u| / |div u0|."""
     from plume_amd.sim.multigrid import Multigrid, relative_residual
@@ -257,10 +258,10 @@ def solve_forms(problem: Problem, target: float, forms=("f32", "bf16", "f16"), m
     out = {}
     for name in forms:
         if name == "f32":
-            solve = Multigrid(grid, 2, 2, tolerance=target, max_cycles=80)
+            solve = Multigrid(grid, 2, sweeps, tolerance=target, max_cycles=80)
             solve.autotune()
         else:
-            solve = MixedPrecisionMultigrid(grid, name, tolerance=target, max_levels=max_levels)
+            solve = MixedPrecisionMultigrid(grid, name, tolerance=target, sweeps=sweeps, max_levels=max_levels)
         for _ in range(repeats):
             grid.pressure.zero_()
             torch.cuda.synchronize()
