@@ -19,7 +19,9 @@ measured RMS stores r again with the measured scale.
 
 The FP32 pressure itself bounds the reachable residual: even the exact solution rounded to FP32 leaves a
 relative residual of order 1e-6 on grids of 10^5 cells and more. A round that does not halve the residual
-therefore ends the solve (`log.stalled`) instead of spinning at that floor.
+therefore ends the solve (`log.stalled`) instead of spinning at that floor; when the first round itself
+reduces less (a weak inner cycle, such as V(1,1) on 16-bit storage), the bar is 1.5 times its ratio, at most
+0.95 (stall_threshold).
 
 The level hierarchy is the reference's (Multigrid._build_levels, sim/multigrid.py:106-122), optionally cut
 at `max_levels` so that the coarsest level is the one the FP64 solve handles (8 x 12 x 8 at 256 x 384 x 256).
@@ -36,7 +38,7 @@ from plume_amd._lib import LIB, GridInfo, call, ptr, stream
 from plume_amd.sim.multigrid import MAX_LEVELS, build_levels
 
 STORAGE = {"f32": 0, "bf16": 1, "f16": 2}  # PLM_STORAGE_* in include/plume_hip.h
-STALL = 0.5  # a round that does not at least halve the residual ends the solve
+STALL = 0.5  # a round that does not at least halve the residual ends the solve (V(2,2) and stronger cycles)
 DTYPES = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}
 
 
@@ -125,15 +127,19 @@ class MixedPrecisionMultigrid:
         buffers = (x, self.x_alt)
         current = 0  # buffers[current] holds the latest x
         stored = None  # the scale rhs[0] was stored with, when it was the latest r's
+        bar = STALL  # stall threshold, set from the first round's ratio
         try:
             rel, rms = self.relative_residual(grid)
             while True:
                 self.log.residuals.append(rel)
                 if rel <= self.tolerance or rms == 0.0 or len(self.log.residuals) > self.max_rounds:
                     return
-                if len(self.log.residuals) >= 2 and rel > STALL * self.log.residuals[-2]:
-                    self.log.stalled = True
-                    return
+                if len(self.log.residuals) >= 2:
+                    if len(self.log.residuals) == 2:
+                        bar = stall_threshold(rel / self.log.residuals[0])
+                    if rel > bar * self.log.residuals[-2]:
+                        self.log.stalled = True
+                        return
                 scale = _pow2_scale(rms)
                 if stored is None or abs(math.log2(stored / scale)) > 4:
                     self._refine(grid, buffers[current], None, 0.0, None, scale)
@@ -150,6 +156,12 @@ class MixedPrecisionMultigrid:
         finally:
             if current == 1:
                 x.copy_(self.x_alt)
+
+
+def stall_threshold(first_ratio: float) -> float:
+    """The residual ratio above which a refinement round counts as stalled, from the first round's ratio:
+    STALL for cycles that cut the residual well, up to 0.95 for weak ones (1.5 times the first ratio)."""
+    return max(STALL, min(0.95, 1.5 * first_ratio))
 
 
 def _pow2_scale(rms: float) -> float:

@@ -387,6 +387,20 @@ def test_projection_with_multigrid_is_bit_exact(device, oracle32, cells, faces):
         assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
 
 
+@pytest.mark.parametrize("sweeps", [1, 3])
+def test_multigrid_with_odd_sweep_counts_is_bit_exact(device, oracle32, sweeps):
+    """An odd number of sweeps per level ends each level's ping-pong with a copy back (the tile sweeps, the LDS
+    tail and the large levels' row prolongation)."""
+    from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
+    from plume_amd.sim.multigrid import Multigrid
+    from plume_amd.sim.pressure import project
+
+    gpu, orc = pair(device, oracle32, (64, 96, 64), DEFAULT_FACES)
+    project(gpu, Multigrid(gpu, 2, sweeps))
+    oracle_project(orc, OracleMultigrid(orc, 2, sweeps))
+    assert bit_equal(to_np(gpu.pressure), orc.pressure)
+
+
 def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
     from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
     from plume_amd.sim.multigrid import Multigrid
