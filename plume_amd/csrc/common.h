@@ -19,10 +19,18 @@ namespace plm {
 int fail(int code, const char *fmt, ...);
 int check_launch(const char *what);
 
+// Event timing of one launch family, for benchmarks. A plain scope records an event pair around what it
+// covers. A `bound` scope records nothing itself: the single kernel it covers is launched with
+// hipExtLaunchKernelGGL(..., start(), stop(), ...), so the pair carries that dispatch's own begin and end
+// timestamps and leaves out the gap before the dispatch starts. start() and stop() are null when the scope is
+// inactive, and the launch then takes the ordinary path.
 struct TimingScope {
-    TimingScope(const char *family, hipStream_t stream);
+    TimingScope(const char *family, hipStream_t stream, bool bound = false);
     ~TimingScope();
+    hipEvent_t start() const;
+    hipEvent_t stop() const;
     bool active;
+    bool bound;
     hipStream_t stream;
     int slot;
 };

@@ -19,6 +19,8 @@
 #include <mutex>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "common.h"
 
 namespace plm {
@@ -831,17 +833,36 @@ constexpr TileShape TILE_SHAPES[] = {{4, 4, 256}, {2, 8, 256}, {8, 4, 256}, {4, 
 constexpr int N_TILE_SHAPES = 6;
 constexpr int DEFAULT_TILE_SHAPE = 0;
 
+// One launch of a tile shape. With timing events (a bound TimingScope) the launch goes through
+// hipExtLaunchKernelGGL so that the events take this dispatch's own begin and end timestamps.
+template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T>
+void launch_tile_kernel(dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const LevelArgs &L, const T *src,
+                        T *dst, const T *rhs, const T *coarse, Dims nc, float relaxation, const double *gate,
+                        int tiles_k, int tiles_j) {
+    if (e0 != nullptr)
+        hipExtLaunchKernelGGL(k_rb_sweep_tile<TI, TJ, THREADS, UNIT, START, T>, grid, dim3(THREADS), 0, s, e0, e1, 0u,
+                              L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j);
+    else
+        k_rb_sweep_tile<TI, TJ, THREADS, UNIT, START, T><<<grid, THREADS, 0, s>>>(L, src, dst, rhs, coarse, nc,
+                                                                                relaxation, gate, tiles_k, tiles_j);
+}
+
 template <bool UNIT, int START, typename T>
-void launch_tile_shape(int shape, dim3 grid, hipStream_t s, const LevelArgs &L, const T *src, T *dst, const T *rhs,
-                       const T *coarse, Dims nc, float relaxation, const double *gate, int tiles_k, int tiles_j) {
+void launch_tile_shape(int shape, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const LevelArgs &L,
+                       const T *src, T *dst, const T *rhs, const T *coarse, Dims nc, float relaxation,
+                       const double *gate, int tiles_k, int tiles_j) {
+#define PLM_TILE(TI, TJ, THREADS)                                                                                    \
+    launch_tile_kernel<TI, TJ, THREADS, UNIT, START, T>(grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation,  \
+                                                        gate, tiles_k, tiles_j)
     switch (shape) {
-    case 1: k_rb_sweep_tile<2, 8, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 2: k_rb_sweep_tile<8, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 3: k_rb_sweep_tile<4, 8, 512, UNIT, START, T><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 4: k_rb_sweep_tile<8, 8, 512, UNIT, START, T><<<grid, 512, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    case 5: k_rb_sweep_tile<2, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
-    default: k_rb_sweep_tile<4, 4, 256, UNIT, START, T><<<grid, 256, 0, s>>>(L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j); break;
+    case 1: PLM_TILE(2, 8, 256); break;
+    case 2: PLM_TILE(8, 4, 256); break;
+    case 3: PLM_TILE(4, 8, 512); break;
+    case 4: PLM_TILE(8, 8, 512); break;
+    case 5: PLM_TILE(2, 4, 256); break;
+    default: PLM_TILE(4, 4, 256); break;
     }
+#undef PLM_TILE
 }
 
 // The shape pinned for a level size (by plm_autotune_sweep or plm_set_sweep_tile), else the default.
@@ -1655,16 +1676,17 @@ int launch_tile(const LevelArgs &L, int start, const T *src, T *dst, const T *rh
     TimingScope launch_timing(!UNIT || sizeof(T) != 4 ? "rb_sweep_tile"
                               : start == FROM_Q      ? "rb_sweep_tile_top"
                                                      : "rb_sweep_tile_top_start",
-                              s);
+                              s, true);
+    const hipEvent_t e0 = launch_timing.start(), e1 = launch_timing.stop();
     if (start == FROM_ZERO)
-        launch_tile_shape<UNIT, FROM_ZERO, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
-                                               tiles_j);
+        launch_tile_shape<UNIT, FROM_ZERO, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
+                                             tiles_k, tiles_j);
     else if (start == FROM_PROLONG)
-        launch_tile_shape<UNIT, FROM_PROLONG, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
-                                               tiles_j);
+        launch_tile_shape<UNIT, FROM_PROLONG, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
+                                             tiles_k, tiles_j);
     else
-        launch_tile_shape<UNIT, FROM_Q, T>(shape, grid, s, L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k,
-                                               tiles_j);
+        launch_tile_shape<UNIT, FROM_Q, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
+                                             tiles_k, tiles_j);
     return check_launch("rb sweep tile");
 }
 

@@ -66,7 +66,8 @@ Timing &timing() {
 }
 }  // namespace
 
-TimingScope::TimingScope(const char *family, hipStream_t s) : active(false), stream(s), slot(-1) {
+TimingScope::TimingScope(const char *family, hipStream_t s, bool bind)
+    : active(false), bound(bind), stream(s), slot(-1) {
     Timing &t = timing();
     std::lock_guard<std::mutex> lock(t.mutex);
     if (!t.enabled || t.family != family) return;
@@ -81,11 +82,25 @@ TimingScope::TimingScope(const char *family, hipStream_t s) : active(false), str
         t.stops.push_back(b);
     }
     slot = (int)t.used++;
-    active = hipEventRecord(t.starts[slot], stream) == hipSuccess;
+    active = bound || hipEventRecord(t.starts[slot], stream) == hipSuccess;
+}
+
+hipEvent_t TimingScope::start() const {
+    if (!active || !bound) return nullptr;
+    Timing &t = timing();
+    std::lock_guard<std::mutex> lock(t.mutex);
+    return t.starts[slot];
+}
+
+hipEvent_t TimingScope::stop() const {
+    if (!active || !bound) return nullptr;
+    Timing &t = timing();
+    std::lock_guard<std::mutex> lock(t.mutex);
+    return t.stops[slot];
 }
 
 TimingScope::~TimingScope() {
-    if (!active) return;
+    if (!active || bound) return;
     Timing &t = timing();
     std::lock_guard<std::mutex> lock(t.mutex);
     (void)hipEventRecord(t.stops[slot], stream);

@@ -401,6 +401,31 @@ def test_multigrid_with_odd_sweep_counts_is_bit_exact(device, oracle32, sweeps):
     assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
+def test_timed_sweeps_are_bit_exact_and_report_each_launch(device, oracle32):
+    """While a launch family is timed, its sweeps launch with the event pair bound to the dispatch
+    (hipExtLaunchKernelGGL). The solve must stay bit-identical to the oracle, and the events must give one
+    positive duration per launch."""
+    from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
+    from plume_amd._lib import LIB
+    from plume_amd.sim.multigrid import Multigrid
+    from plume_amd.sim.pressure import project
+
+    gpu, orc = pair(device, oracle32, (64, 96, 64), DEFAULT_FACES)
+    solve = Multigrid(gpu, 2, 2)
+    total, launches = ctypes.c_double(), ctypes.c_int64()
+    assert LIB.plm_timing_enable(b"rb_sweep_tile_top") == 0
+    try:
+        project(gpu, solve)
+        torch.cuda.synchronize()
+        assert LIB.plm_timing_read(ctypes.byref(total), ctypes.byref(launches)) == 0, LIB.plm_last_error()
+    finally:
+        LIB.plm_timing_disable()
+    oracle_project(orc, OracleMultigrid(orc, 2, 2))
+    assert bit_equal(to_np(gpu.pressure), orc.pressure)
+    assert launches.value > 0
+    assert 0.0 < total.value / launches.value < 50.0  # ms per launch of a 64x96x64 sweep
+
+
 def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
     from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
     from plume_amd.sim.multigrid import Multigrid
