@@ -11,6 +11,8 @@ One JSON line on stdout. Besides the contract's keys it carries:
                  launch / average launch time from HIP events, against 8 TB/s HBM;
   substep_roofline  B_alg = 4 N (36 + 18.43 C) bytes per substep (C = V-cycles run) x substeps/s / 8 TB/s;
   raymarch       256 x 256 px, 192 samples per ray, light volume with 64 steps per cell;
+  recipe_run     the smoke-puff recipe run end to end (80 x 100 x 80 cells, 203 substeps, 64 frames,
+                 sheet assembly): stage wall times beside the reference's published RTX 2080 Ti ones;
   cpu_baseline   the FP32 CPU oracle (a scalar C port of the same operators, OpenMP) on a bounded sample.
 """
 
@@ -127,6 +129,31 @@ def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
     elapsed = time.perf_counter() - start
     assert np.isfinite(grid.smoke.data).all()
     return elapsed
+
+
+PUFF_RECIPE = ROOT / "tests" / "golden" / "smoke_puff.toml"  # the reference's recipes/smoke_puff.toml
+# The reference's stage timings of this run on an RTX 2080 Ti (Warp CUDA): simulate 2.30 s (render excluded),
+# render 0.16 s (light volume, march and copy to the host, 64 frames)
+PUFF_2080TI_S = {"simulate": 2.30, "render": 0.16}
+
+
+def recipe_run() -> dict:
+    """`plume run recipes/smoke_puff.toml` end to end on the HIP path (80 x 100 x 80 cells, 64 frames of a
+    160 x 200 px view, sheet assembly and numbers): stage wall times of the second of two runs."""
+    from plume_amd.pipeline import run_recipe
+
+    run_recipe(PUFF_RECIPE, preview=False, device="cuda")
+    run = run_recipe(PUFF_RECIPE, preview=False, device="cuda")
+    cells = run.derived.cells[0] * run.derived.cells[1] * run.derived.cells[2]
+    substeps = run.steps.total_steps
+    print(f"[bench] recipe run: {substeps} substeps, simulate {run.seconds['simulate']:.3f} s, render "
+          f"{run.seconds['render']:.3f} s, assemble {run.seconds['assemble']:.3f} s", file=sys.stderr, flush=True)
+    return {"recipe": "smoke_puff.toml (the reference's own scene)", "cells": list(run.derived.cells),
+            "frames": len(run.raw_frames), "substeps": substeps, "simulate_s": run.seconds["simulate"],
+            "render_s": run.seconds["render"], "assemble_s": run.seconds["assemble"],
+            "cell_substeps_per_s": cells * substeps / run.seconds["simulate"],
+            "reference_rtx2080ti_s": PUFF_2080TI_S,
+            "speedup_vs_reference_rtx2080ti": {k: PUFF_2080TI_S[k] / run.seconds[k] for k in PUFF_2080TI_S}}
 
 
 MIXED_GRID = (256, 384, 256)
@@ -298,6 +325,8 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=48, help="untimed substeps first (W); 48 = 12 frames")
     parser.add_argument("--cpu-steps", type=int, default=32, help="substeps of the CPU baseline sample (0 = skip)")
     parser.add_argument("--no-render", action="store_true")
+    parser.add_argument("--no-recipe-run", action="store_true",
+                        help="skip timing the smoke-puff recipe run end to end (80 x 100 x 80, 64 frames)")
     parser.add_argument("--tolerance", type=float, default=1e-5)
     parser.add_argument("--top-tile", type=int, default=-1, metavar="SHAPE",
                         help="pin the sweep tile shape of the 128x192x128 level instead of autotuning it")
@@ -458,6 +487,9 @@ def main() -> int:
             "light_volume_ms": light_s * 1e3, "light_cell_samples_per_s": cells * 64 / light_s,
             "compulsory_hbm_frac": (12.0 * cells + 20.0 * width * height) / march_s / 1e9 / HBM_PEAK_GBPS,
         }
+
+    if not args.no_recipe_run:
+        result["recipe_run"] = recipe_run()
 
     if args.cpu_steps > 0:
         import dataclasses

@@ -1,0 +1,84 @@
+"""One recipe run through the simulation and render path to the sheet frames and the numbers file
+(cli/run.py:43-130 of the reference, without its PNG, JSON, contact-sheet and settings outputs): load the
+recipe (optionally its --preview form), simulate on the HIP path capturing a raw frame after each frame's
+substeps, assemble the sheet frames, and measure them."""
+
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from plume_amd.recipe.derived import Derived, derive
+from plume_amd.recipe.load import load_recipe
+from plume_amd.recipe.schema import Recipe
+from plume_amd.render.camera import sheet_domain_view
+from plume_amd.render.frame_renderer import FrameRenderer
+from plume_amd.report.numbers import RunFacts, measure, numbers_text
+from plume_amd.sheet.assemble import AssembledFrames, assemble, seat_point
+from plume_amd.sim.device import init_device
+from plume_amd.sim.run import StepLog, simulate
+
+
+@dataclass(frozen=True)
+class RunResult:
+    recipe: Recipe
+    derived: Derived
+    device: str
+    raw_frames: list[np.ndarray]
+    assembled: AssembledFrames
+    numbers: str
+    steps: StepLog
+    seconds: dict[str, float]  # wall time of "simulate" (render excluded), "render" and "assemble"
+
+
+def run_facts(recipe: Recipe, derived: Derived, backend: str, assembled: AssembledFrames, source: str) -> RunFacts:
+    """The numbers file's summary facts (cli/run.py:122-125)."""
+    nx, ny, nz = derived.cells
+    return RunFacts(recipe=source, backend=backend, grid=f"{nx} x {ny} x {nz} cells of {recipe.domain.cell_m:.6f} m",
+                    fps=derived.fps, frame_m=assembled.frame_m, heat_divisor=assembled.heat_divisor,
+                    channels=recipe.output.channels, shade_divisor=assembled.shade_divisor)
+
+
+def sheet_numbers(raw_frames: list[np.ndarray], recipe: Recipe, derived: Derived, backend: str,
+                  source: str) -> tuple[AssembledFrames, str]:
+    """The assembled sheet frames of a run's raw frames and its numbers file text."""
+    view = sheet_domain_view(recipe, derived)
+    assembled = assemble(raw_frames, view, seat_point(recipe), recipe.output.frame_px, recipe.render,
+                         recipe.output.channels)
+    return assembled, numbers_text(measure(assembled.frames, derived.fps), run_facts(recipe, derived, backend,
+                                                                                      assembled, source))
+
+
+def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", source: str | None = None,
+               out: Path | None = None) -> RunResult:
+    """Runs a recipe on the HIP path and returns its sheet frames and numbers; with `out`, also writes
+    `numbers.txt` there. `source` is the recipe path the numbers file names (default: as given)."""
+    import torch
+
+    recipe = load_recipe(Path(recipe_path), preview=preview)
+    derived = derive(recipe)
+    device_name = init_device(device)
+    view = sheet_domain_view(recipe, derived)
+    renderer = FrameRenderer(view, recipe.render, derived.cells, recipe.output.channels, device_name)
+    raw_frames = []
+    seconds = {"simulate": 0.0, "render": 0.0}
+
+    def capture(frame: int, grid) -> None:
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        raw_frames.append(renderer.render(grid))
+        seconds["render"] += time.perf_counter() - start
+
+    start = time.perf_counter()
+    steps = simulate(recipe, derived, device_name, capture)
+    torch.cuda.synchronize()
+    seconds["simulate"] = time.perf_counter() - start - seconds["render"]
+    start = time.perf_counter()
+    assembled, numbers = sheet_numbers(raw_frames, recipe, derived, device_name,
+                                       source if source is not None else str(recipe_path))
+    seconds["assemble"] = time.perf_counter() - start
+    if out is not None:
+        Path(out).mkdir(parents=True, exist_ok=True)
+        (Path(out) / "numbers.txt").write_text(numbers, newline="\n")
+    return RunResult(recipe, derived, device_name, raw_frames, assembled, numbers, steps, seconds)
