@@ -1,5 +1,6 @@
 """The reference's approved preview of the smoke puff and the bounds its CLI test allows
-(tests/cli/test_golden_numbers.py:21-63 of the reference).
+(tests/cli/test_golden_numbers.py:21-63 of the reference), the bounds of its CPU-against-GPU sheet test
+(tests/cli/test_cpu_gpu.py:21-33), and a recipe run on the oracle for either comparison.
 
 `tests/golden/smoke_puff_preview.numbers.txt` is the reference's golden numbers file (a Warp CPU run of
 `plume run recipes/smoke_puff.toml --preview`: 38 x 48 x 38 cells, 16 frames of 128 px) and
@@ -8,6 +9,7 @@
 
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 GOLDEN_DIR = Path(__file__).resolve().parent / "golden"
@@ -69,4 +71,49 @@ def largest_differences(actual_text: str, golden_text: str) -> dict[str, float]:
         for key in BOX_KEYS + AMOUNT_KEYS + HEAT_KEYS:
             if want[key] != "-" and row[key] != "-":
                 worst[key] = max(worst.get(key, 0.0), abs(float(row[key]) - float(want[key])))
+    return worst
+
+
+SHEET_G_BYTES = 3  # thickness bytes of the two runs' sheets within 3
+SHEET_BA_BYTES = 6  # shade / normal bytes within 6
+SHEET_RELATIVE = 0.02  # coverage, mean and total thickness within 2 percent
+
+
+def oracle_run(recipe_path, preview: bool, source: str):
+    """(AssembledFrames, numbers text) of a recipe run on the FP32 oracle (simulation and frame render), then the
+    package's sheet assembly and measurement."""
+    from oracle import Oracle
+    from oracle.render import render_oracle
+    from oracle.sim import simulate_frames
+    from plume_amd.pipeline import sheet_numbers
+    from plume_amd.recipe.derived import derive
+    from plume_amd.recipe.load import load_recipe
+    from plume_amd.render.camera import sheet_domain_view
+
+    recipe = load_recipe(Path(recipe_path), preview=preview)
+    derived = derive(recipe)
+    o = Oracle("f32")
+    view = sheet_domain_view(recipe, derived)
+    raws = [render_oracle(o, grid, view, recipe.render, recipe.output.channels)
+            for _, grid in simulate_frames(o, recipe, derived)]
+    return sheet_numbers(raws, recipe, derived, "cpu", source)
+
+
+def assert_sheets_agree(gpu_frames, cpu_frames, gpu_text: str, cpu_text: str) -> dict[str, int]:
+    """The CPU-against-GPU bounds on two runs' sheet frames and numbers; returns the largest byte differences
+    per channel."""
+    from plume_amd.report.numbers import read_numbers
+
+    assert len(gpu_frames) == len(cpu_frames)
+    gpu = np.stack(gpu_frames).astype(int)
+    cpu = np.stack(cpu_frames).astype(int)
+    assert gpu.shape == cpu.shape
+    worst = {name: int(np.abs(gpu[..., c] - cpu[..., c]).max()) for c, name in enumerate("RGBA")}
+    assert worst["G"] <= SHEET_G_BYTES, worst
+    assert max(worst["B"], worst["A"]) <= SHEET_BA_BYTES, worst
+    _, gpu_rows = read_numbers(gpu_text)
+    _, cpu_rows = read_numbers(cpu_text)
+    for gpu_row, cpu_row in zip(gpu_rows, cpu_rows):
+        for key in ("coverage", "mean_thick", "total_thick"):
+            assert float(gpu_row[key]) == pytest.approx(float(cpu_row[key]), rel=SHEET_RELATIVE), key
     return worst

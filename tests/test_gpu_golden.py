@@ -1,16 +1,26 @@
-"""`plume run recipes/smoke_puff.toml --preview` on the HIP path (38 x 48 x 38 cells, 16 frames of 128 px)
-against the reference's golden numbers file, with the bounds of the reference's own golden-numbers test
-(tests/cli/test_golden_numbers.py:21-63): the HIP simulation and ray march, then the package's sheet assembly
-and measurement, must give the approved preview's coverage, boxes, thickness and heat."""
+"""Recipe runs on the HIP path through the sheet frames and the numbers file.
 
+- `plume run recipes/smoke_puff.toml --preview` (38 x 48 x 38 cells, 16 frames of 128 px) against the
+  reference's golden numbers file, with the bounds of the reference's own golden-numbers test
+  (tests/cli/test_golden_numbers.py:21-63), and its sheet frames against the FP32 oracle's;
+- the tiny recipe in both channel layouts against the oracle's run with the bounds of the reference's
+  CPU-against-GPU test (tests/cli/test_cpu_gpu.py:21-33: thickness bytes within 3, shade / normal bytes
+  within 6, coverage and thickness within 2%), the layout checks of tests/cli/test_run.py:43-61, and
+  identical frames from identical runs;
+- the tiny recipe with a 900 K emitter, so that heat reaches the sheet, against the oracle's run.
+"""
+
+import numpy as np
 import pytest
 
-from golden_numbers import GOLDEN, RECIPE, SOURCE, assert_matches_golden, largest_differences
+from golden_numbers import (GOLDEN, RECIPE, SOURCE, assert_matches_golden, assert_sheets_agree, largest_differences,
+                            oracle_run)
+from support import TINY
 
 pytestmark = pytest.mark.gpu
 
 
-def test_preview_run_matches_the_golden_numbers(device):
+def test_preview_run_matches_the_golden_numbers_and_the_oracle_sheet(device):
     from plume_amd.pipeline import run_recipe
     from plume_amd.report.numbers import read_numbers
 
@@ -22,3 +32,52 @@ def test_preview_run_matches_the_golden_numbers(device):
     assert_matches_golden(result.numbers, golden)
     summary, golden_summary = read_numbers(result.numbers)[0], read_numbers(golden)[0]
     assert float(summary["shade_divisor"]) == pytest.approx(float(golden_summary["shade_divisor"]), rel=1e-3)
+    cpu, cpu_text = oracle_run(RECIPE, True, SOURCE)
+    worst = assert_sheets_agree(result.assembled.frames, cpu.frames, result.numbers, cpu_text)
+    print("largest sheet byte differences against the oracle:", worst, flush=True)
+    assert result.assembled.crop == cpu.crop
+
+
+@pytest.mark.parametrize("channels", ["heat,thickness,shade,unused", "heat,thickness,normal_x,normal_y"])
+def test_tiny_recipe_sheets_agree_with_the_oracle(device, tmp_path, channels):
+    from plume_amd.pipeline import run_recipe
+
+    path = tmp_path / "tiny.toml"
+    path.write_text(TINY + f'channels = "{channels}"\n')
+    first = run_recipe(path, device="cuda", source="tiny.toml")
+    second = run_recipe(path, device="cuda", source="tiny.toml")
+    assert all(np.array_equal(a, b) for a, b in zip(first.assembled.frames, second.assembled.frames))
+    assert first.numbers == second.numbers
+    frames = np.stack(first.assembled.frames)
+    assert frames.shape == (4, 64, 64, 4)
+    if channels.endswith("unused"):
+        assert np.all(frames[..., 3] == 255)
+        covered = frames[..., 1] >= 128
+        assert covered.any() and frames[covered, 2].max() > 0
+    else:
+        empty = (frames[..., 0] == 0) & (frames[..., 1] == 0)
+        assert np.all(frames[empty, 2:] == 128)
+        assert "shade_divisor" not in first.numbers
+    cpu, cpu_text = oracle_run(path, False, "tiny.toml")
+    worst = assert_sheets_agree(first.assembled.frames, cpu.frames, first.numbers, cpu_text)
+    print(channels, "largest sheet byte differences against the oracle:", worst, flush=True)
+
+
+def test_a_hot_tiny_recipe_sheet_agrees_with_the_oracle_in_heat(device, tmp_path):
+    """The tiny recipe with a 900 K emitter (over the 600 K glow start), so the sheet's R channel carries heat:
+    heat bytes held to the thickness bound (3)."""
+    from plume_amd.pipeline import run_recipe
+    from plume_amd.report.numbers import read_numbers
+
+    path = tmp_path / "tiny.toml"
+    assert "temperature_k = 500.0" in TINY
+    path.write_text(TINY.replace("temperature_k = 500.0", "temperature_k = 900.0"))
+    gpu = run_recipe(path, device="cuda", source="tiny.toml")
+    cpu, cpu_text = oracle_run(path, False, "tiny.toml")
+    assert gpu.assembled.heat_divisor > 0.0
+    assert gpu.assembled.heat_divisor == pytest.approx(cpu.heat_divisor, rel=1e-4)
+    worst = assert_sheets_agree(gpu.assembled.frames, cpu.frames, gpu.numbers, cpu_text)
+    print("hot tiny: largest sheet byte differences against the oracle:", worst, flush=True)
+    assert worst["R"] <= 3
+    _, rows = read_numbers(gpu.numbers)
+    assert max(float(row["peak_heat"]) for row in rows) == pytest.approx(1.0, abs=1.0 / 255)

@@ -10,7 +10,7 @@ resampler. The golden check runs `plume run recipes/smoke_puff.toml --preview` o
 import numpy as np
 import pytest
 
-from golden_numbers import GOLDEN, RECIPE, SOURCE, assert_matches_golden, largest_differences
+from golden_numbers import GOLDEN, RECIPE, SOURCE, assert_matches_golden, largest_differences, oracle_run
 from plume_amd.recipe.schema import NORMAL_LAYOUT, SHADE_LAYOUT, RenderSettings
 from plume_amd.render.camera import make_camera, make_domain_view
 from plume_amd.render.raw_frame import CHANNELS, HEAT, NORMAL_X, NORMAL_Y, NORMAL_Z, SHADE, THICKNESS
@@ -288,21 +288,7 @@ def test_numbers_text_round_trips_and_keeps_the_golden_layout():
 
 @pytest.fixture(scope="module")
 def oracle_preview():
-    from oracle import Oracle
-    from oracle.render import render_oracle
-    from oracle.sim import simulate_frames
-    from plume_amd.pipeline import sheet_numbers
-    from plume_amd.recipe.derived import derive
-    from plume_amd.recipe.load import load_recipe
-    from plume_amd.render.camera import sheet_domain_view
-
-    recipe = load_recipe(RECIPE, preview=True)
-    derived = derive(recipe)
-    o = Oracle("f32")
-    view = sheet_domain_view(recipe, derived)
-    raws = [render_oracle(o, grid, view, recipe.render, recipe.output.channels)
-            for _, grid in simulate_frames(o, recipe, derived)]
-    return sheet_numbers(raws, recipe, derived, "cpu", SOURCE)
+    return oracle_run(RECIPE, True, SOURCE)
 
 
 def test_oracle_preview_matches_the_golden_numbers(oracle_preview):
