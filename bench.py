@@ -11,6 +11,7 @@ One JSON line on stdout. Besides the contract's keys it carries:
                  launch / average launch time from HIP events, against 8 TB/s HBM;
   substep_roofline  B_alg = 4 N (36 + 18.43 C) bytes per substep (C = V-cycles run) x substeps/s / 8 TB/s;
   raymarch       256 x 256 px, 192 samples per ray, light volume with 64 steps per cell;
+  frame_cache    the viewer's binary16 frame-cache kernels at 128 x 192 x 128 (GB/s, HIP events);
   recipe_run     the smoke-puff recipe run end to end (80 x 100 x 80 cells, 203 substeps, 64 frames,
                  sheet assembly): stage wall times beside the reference's published RTX 2080 Ti ones;
   cpu_baseline   the FP32 CPU oracle (a scalar C port of the same operators, OpenMP) on a bounded sample.
@@ -129,6 +130,45 @@ def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
     elapsed = time.perf_counter() - start
     assert np.isfinite(grid.smoke.data).all()
     return elapsed
+
+
+def frame_cache_bench(grid, repeats: int = 20) -> dict:
+    """The viewer's frame-cache kernels on the bench's warm 128 x 192 x 128 fields, each timed with HIP events
+    over `repeats` launches: narrowing a scalar field to binary16 (4 B read + 2 B written per cell), the fused
+    cell-centre velocity to binary16 (three face arrays read, 6 B written per cell) and widening back (2 B read
+    + 4 B written per value)."""
+    import torch
+
+    from plume_amd._lib import call, ptr, stream
+
+    cells = grid.smoke.data.numel()
+    half = torch.empty(cells, dtype=torch.float16, device="cuda")
+    wide = torch.empty(cells, dtype=torch.float32, device="cuda")
+    centre = torch.empty(3 * cells, dtype=torch.float16, device="cuda")
+    u, v, w = (f.data for f in (grid.u, grid.v, grid.w))
+    kernels = {
+        "narrow_f16": (lambda: call("plm_narrow_f16", cells, ptr(grid.smoke.data), ptr(half, torch.float16),
+                                    stream()), 6.0 * cells),
+        "widen_f16": (lambda: call("plm_widen_f16", cells, ptr(half, torch.float16), ptr(wide), stream()),
+                      6.0 * cells),
+        "narrow_center_velocity": (lambda: call("plm_narrow_center_velocity", ctypes.byref(grid.info), ptr(u), ptr(v), ptr(w),
+                                                ptr(centre, torch.float16), stream()),
+                                   4.0 * (u.numel() + v.numel() + w.numel()) + 6.0 * cells),
+    }
+    out = {}
+    for name, (launch, nbytes) in kernels.items():
+        launch()
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(repeats):
+            launch()
+        end.record()
+        end.synchronize()
+        seconds = start.elapsed_time(end) * 1e-3 / repeats
+        out[name] = {"us": seconds * 1e6, "bytes": nbytes, "GBps": nbytes / seconds / 1e9,
+                     "hbm_frac": nbytes / seconds / 1e9 / HBM_PEAK_GBPS}
+    return out
 
 
 PUFF_RECIPE = ROOT / "tests" / "golden" / "smoke_puff.toml"  # the reference's recipes/smoke_puff.toml
@@ -488,6 +528,7 @@ def main() -> int:
             "compulsory_hbm_frac": (12.0 * cells + 20.0 * width * height) / march_s / 1e9 / HBM_PEAK_GBPS,
         }
 
+    result["frame_cache"] = frame_cache_bench(grid)
     if not args.no_recipe_run:
         result["recipe_run"] = recipe_run()
 

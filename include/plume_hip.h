@@ -375,11 +375,32 @@ int plm_raymarch(const plm_grid_info *info, const float *smoke, const float *tem
                  const plm_view *view, const plm_look *look, float *image, int32_t height, int32_t width,
                  int32_t max_steps, void *stream);
 
+/* ------------------------------------------------------------------ viewer: view/ */
+
+/* narrow_kernel, view/frame_cache.py:26-29: count FP32 values rounded to IEEE binary16 (round to nearest
+   even, subnormals kept) into dst (count halves). src 16-byte and dst 8-byte aligned. */
+int plm_narrow_f16(int64_t count, const float *src, void *dst, void *stream);
+/* widen_kernel and widen_vector_kernel, view/frame_cache.py:32-35,45-49: count binary16 values to FP32 (a vec3h
+   array is 3N halves). src 8-byte and dst 16-byte aligned. */
+int plm_widen_f16(int64_t count, const void *src, float *dst, void *stream);
+/* center_velocity_kernel (sim/vorticity.py:17-22) then narrow_vector_kernel (view/frame_cache.py:38-42), in one
+   kernel: per cell (0.5 (u_i + u_i+1), 0.5 (v_j + v_j+1), 0.5 (w_k + w_k+1)) in FP32, rounded to binary16, into
+   center as 3 halves per cell (nx, ny, nz, 3). */
+int plm_narrow_center_velocity(const plm_grid_info *info, const float *u, const float *v, const float *w,
+                               void *center, void *stream);
+/* boundary_layer_kernel, view/live_run.py:41-57: layers (6, extent_a, extent_b), face = 2 axis + side in
+   left, right, bottom, top, back, front order, a and b the face's other two axes in x, y, z order; entries
+   past a face's size are 0. */
+int plm_boundary_layers(const int32_t shape[3], const float *smoke, float *layers, int32_t extent_a,
+                        int32_t extent_b, void *stream);
+
 /* ------------------------------------------------------------------ timing */
 
 /* Kernel timing for the benchmark: while enabled, the library brackets every launch of the named kernel
    family ("rb_sweeps", "advect", ...) with HIP events on the launch stream and accumulates their elapsed
-   time. plm_timing_read synchronises those events and returns total milliseconds and launch count. */
+   time; a single-kernel family ("rb_sweep_tile_top", "rb_sweep_tile") binds the events to the kernel's own
+   dispatch (hipExtLaunchKernelGGL), so they time the kernel alone. plm_timing_read synchronises those
+   events and returns total milliseconds and launch count. No counterpart in the reference. */
 int plm_timing_enable(const char *family);
 int plm_timing_disable(void);
 int plm_timing_read(double *total_ms, int64_t *launches);

@@ -122,6 +122,10 @@ SIGNATURES = {
     "plm_blur_axis": [_P, _P, _P, _P, _I, _I, _P],
     "plm_gradient": [_P, _P, _P, _P],
     "plm_raymarch": [_INFO, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P],
+    "plm_narrow_f16": [_L, _P, _P, _P],
+    "plm_widen_f16": [_L, _P, _P, _P],
+    "plm_narrow_center_velocity": [_INFO, _P, _P, _P, _P, _P],
+    "plm_boundary_layers": [_P, _P, _P, _I, _I, _P],
     "plm_timing_enable": [ctypes.c_char_p],
     "plm_timing_disable": [],
     "plm_timing_read": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)],
