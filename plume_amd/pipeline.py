@@ -1,7 +1,7 @@
-"""One recipe run through the simulation and render path to the sheet frames and the numbers file
-(cli/run.py:43-130 of the reference, without its PNG, JSON, contact-sheet and settings outputs): load the
-recipe (optionally its --preview form), simulate on the HIP path capturing a raw frame after each frame's
-substeps, assemble the sheet frames, and measure them."""
+"""One recipe run through the simulation and render path to the sheet frames, the numbers file and the
+settings log (cli/run.py:43-130 of the reference, without its PNG, JSON and contact-sheet outputs): load the
+recipe (its extends chain, --set items and optionally its --preview form), simulate on the HIP path capturing a
+raw frame after each frame's substeps, assemble the sheet frames, and measure them."""
 
 import time
 from dataclasses import dataclass
@@ -10,8 +10,9 @@ from pathlib import Path
 import numpy as np
 
 from plume_amd.recipe.derived import Derived, derive
-from plume_amd.recipe.load import load_recipe
+from plume_amd.recipe.load import load_recipe_with_origin
 from plume_amd.recipe.schema import Recipe
+from plume_amd.recipe.serialize import recipe_to_toml
 from plume_amd.render.camera import sheet_domain_view
 from plume_amd.render.frame_renderer import FrameRenderer
 from plume_amd.report.numbers import RunFacts, measure, numbers_text
@@ -51,12 +52,13 @@ def sheet_numbers(raw_frames: list[np.ndarray], recipe: Recipe, derived: Derived
 
 
 def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", source: str | None = None,
-               out: Path | None = None) -> RunResult:
-    """Runs a recipe on the HIP path and returns its sheet frames and numbers; with `out`, also writes
-    `numbers.txt` there. `source` is the recipe path the numbers file names (default: as given)."""
+               out: Path | None = None, sets: tuple[str, ...] = ()) -> RunResult:
+    """Runs a recipe (with its extends chain and `--set` items) on the HIP path and returns its sheet frames and
+    numbers; with `out`, also writes `numbers.txt` and the settings log `settings.toml` there. `source` is the
+    recipe path the numbers file names (default: as given)."""
     import torch
 
-    recipe = load_recipe(Path(recipe_path), preview=preview)
+    recipe, origin = load_recipe_with_origin(Path(recipe_path), preview=preview, sets=tuple(sets))
     derived = derive(recipe)
     device_name = init_device(device)
     view = sheet_domain_view(recipe, derived)
@@ -81,4 +83,8 @@ def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", s
     if out is not None:
         Path(out).mkdir(parents=True, exist_ok=True)
         (Path(out) / "numbers.txt").write_text(numbers, newline="\n")
+        facts = {"warmup_steps_run": steps.warmup_steps, "steps_per_frame": list(steps.steps_per_frame),
+                 "total_steps": steps.total_steps}
+        (Path(out) / "settings.toml").write_text(recipe_to_toml(recipe, derived, device_name, origin, facts),
+                                                 newline="\n")
     return RunResult(recipe, derived, device_name, raw_frames, assembled, numbers, steps, seconds)

@@ -1,6 +1,7 @@
 """Building and loading recipes: strict table -> dataclass build (recipe/build.py:16-113 of the reference),
-the value rules the simulation relies on (recipe/rules.py:21-112), --preview (recipe/load.py:46-69) and the
-dependent defaults (recipe/load.py:72-87)."""
+the value rules the simulation relies on (recipe/rules.py:21-112), --preview (recipe/load.py:46-69), the
+dependent defaults (recipe/load.py:72-87), and loading a file with its extends chain and --set items
+(recipe/load.py:18-29; the variations themselves are in plume_amd.recipe.variation)."""
 
 import dataclasses
 import difflib
@@ -8,8 +9,6 @@ import math
 import typing
 from pathlib import Path
 from typing import Any
-
-import tomli
 
 from plume_amd.recipe.errors import RecipeError
 from plume_amd.recipe.schema import SEED_MAX, Recipe
@@ -222,13 +221,19 @@ def resolve_table(table: dict, recipe_stem: str, preview: bool = False) -> Recip
     return fill_dependent_defaults(recipe, recipe_stem)
 
 
-def load_recipe(path: Path, preview: bool = False) -> Recipe:
-    """Reads the TOML recipe at `path` and returns it fully resolved."""
+def load_recipe_with_origin(path: Path, preview: bool = False,
+                            sets: tuple[str, ...] = ()) -> tuple[Recipe, "RecipeOrigin"]:
+    """The fully resolved recipe at `path` and the parents and --set items that shaped it
+    (recipe/load.py:24-29): parse, merge the extends chain, apply the sets, build and check, preview, then the
+    dependent defaults."""
+    from plume_amd.recipe.variation import RecipeOrigin, apply_sets, read_toml, resolve_extends
+
     path = Path(path)
-    try:
-        table = tomli.loads(path.read_text())
-    except tomli.TOMLDecodeError as error:
-        raise RecipeError(f"{path.name} is not valid TOML: {error}") from error
-    if table.get("extends"):
-        raise RecipeError("extends is not supported by this loader; resolve the parent recipe first")
-    return resolve_table(table, path.stem, preview)
+    table, chain = resolve_extends(read_toml(path), path)
+    recipe = resolve_table(apply_sets(table, tuple(sets)), path.stem, preview)
+    return recipe, RecipeOrigin(extends_chain=chain, sets=tuple(sets))
+
+
+def load_recipe(path: Path, preview: bool = False, sets: tuple[str, ...] = ()) -> Recipe:
+    """Reads the TOML recipe at `path` (with its extends chain and `sets`) and returns it fully resolved."""
+    return load_recipe_with_origin(path, preview, sets)[0]

@@ -5,8 +5,8 @@
   (tests/cli/test_golden_numbers.py:21-63), and its sheet frames against the FP32 oracle's;
 - the tiny recipe in both channel layouts against the oracle's run with the bounds of the reference's
   CPU-against-GPU test (tests/cli/test_cpu_gpu.py:21-33: thickness bytes within 3, shade / normal bytes
-  within 6, coverage and thickness within 2%), the layout checks of tests/cli/test_run.py:43-61, and
-  identical frames from identical runs;
+  within 6, coverage and thickness within 2%), the layout checks of tests/cli/test_run.py:43-61, identical
+  frames from identical runs, and the run's numbers.txt and settings log (which loads back to the recipe);
 - the tiny recipe with a 900 K emitter, so that heat reaches the sheet, against the oracle's run.
 """
 
@@ -44,8 +44,18 @@ def test_tiny_recipe_sheets_agree_with_the_oracle(device, tmp_path, channels):
 
     path = tmp_path / "tiny.toml"
     path.write_text(TINY + f'channels = "{channels}"\n')
-    first = run_recipe(path, device="cuda", source="tiny.toml")
+    first = run_recipe(path, device="cuda", source="tiny.toml", out=tmp_path / "out")
     second = run_recipe(path, device="cuda", source="tiny.toml")
+    # the run's files: numbers.txt as returned, and a settings log that loads back to the same recipe
+    import tomli
+
+    from plume_amd.recipe.load import parse_recipe
+
+    assert (tmp_path / "out" / "numbers.txt").read_text() == first.numbers
+    settings = tomli.loads((tmp_path / "out" / "settings.toml").read_text())
+    assert parse_recipe(settings) == first.recipe
+    resolved = settings["resolved"]
+    assert resolved["total_steps"] == first.steps.total_steps == sum(resolved["steps_per_frame"])
     assert all(np.array_equal(a, b) for a, b in zip(first.assembled.frames, second.assembled.frames))
     assert first.numbers == second.numbers
     frames = np.stack(first.assembled.frames)
