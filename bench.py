@@ -133,10 +133,11 @@ def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
 
 
 def frame_cache_bench(grid, repeats: int = 20) -> dict:
-    """The viewer's frame-cache kernels on the bench's warm 128 x 192 x 128 fields, each timed with HIP events
-    over `repeats` launches: narrowing a scalar field to binary16 (4 B read + 2 B written per cell), the fused
-    cell-centre velocity to binary16 (three face arrays read, 6 B written per cell) and widening back (2 B read
-    + 4 B written per value)."""
+    """The viewer's frame-cache kernels on the bench's warm 128 x 192 x 128 fields, each timed over `repeats`
+    launches with HIP events bound to each dispatch (the library's timing families, as for the sweep):
+    narrowing a scalar field to binary16 (4 B read + 2 B written per cell), the fused cell-centre velocity to
+    binary16 (three face arrays read, 6 B written per cell) and widening back (2 B read + 4 B written per
+    value)."""
     import torch
 
     from plume_amd._lib import call, ptr, stream
@@ -159,14 +160,9 @@ def frame_cache_bench(grid, repeats: int = 20) -> dict:
     for name, (launch, nbytes) in kernels.items():
         launch()
         torch.cuda.synchronize()
-        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record()
-        for _ in range(repeats):
-            launch()
-        end.record()
-        end.synchronize()
-        seconds = start.elapsed_time(end) * 1e-3 / repeats
-        out[name] = {"us": seconds * 1e6, "bytes": nbytes, "GBps": nbytes / seconds / 1e9,
+        ms, launches = time_kernel_family(lambda: [launch() for _ in range(repeats)], name)
+        seconds = ms * 1e-3
+        out[name] = {"us": seconds * 1e6, "launches": launches, "bytes": nbytes, "GBps": nbytes / seconds / 1e9,
                      "hbm_frac": nbytes / seconds / 1e9 / HBM_PEAK_GBPS}
     return out
 

@@ -398,7 +398,8 @@ int plm_boundary_layers(const int32_t shape[3], const float *smoke, float *layer
 
 /* Kernel timing for the benchmark: while enabled, the library brackets every launch of the named kernel
    family ("rb_sweeps", "advect", ...) with HIP events on the launch stream and accumulates their elapsed
-   time; a single-kernel family ("rb_sweep_tile_top", "rb_sweep_tile") binds the events to the kernel's own
+   time; a single-kernel family ("rb_sweep_tile_top", "rb_sweep_tile", "narrow_f16", "widen_f16",
+   "narrow_center_velocity") binds the events to the kernel's own
    dispatch (hipExtLaunchKernelGGL), so they time the kernel alone. plm_timing_read synchronises those
    events and returns total milliseconds and launch count. No counterpart in the reference. */
 int plm_timing_enable(const char *family);

@@ -7,6 +7,7 @@
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,6 +35,17 @@ struct TimingScope {
     hipStream_t stream;
     int slot;
 };
+
+// Launches `kernel` with the bound event pair of `timing` when it is active (hipExtLaunchKernelGGL), else as
+// an ordinary launch.
+template <typename Kernel, typename... Args>
+void launch_timed(const TimingScope &timing, Kernel kernel, dim3 grid, dim3 block, hipStream_t stream, Args... args) {
+    const hipEvent_t e0 = timing.start(), e1 = timing.stop();
+    if (e0 != nullptr)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, e0, e1, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+}
 
 #define PLM_REQUIRE(cond, ...)                                                                     \
     do {                                                                                           \

@@ -93,7 +93,9 @@ int plm_narrow_f16(int64_t count, const float *src, void *dst, void *stream) {
     PLM_REQUIRE(count >= 0 && (count == 0 || (src && dst)), "plm_narrow_f16: bad arguments");
     PLM_REQUIRE(aligned(src, 16) && aligned(dst, 8), "plm_narrow_f16: src must be 16-byte and dst 8-byte aligned");
     if (count == 0) return 0;
-    k_narrow_f16<<<grid_quads(count), CACHE_BLOCK, 0, as_stream(stream)>>>(count, src, static_cast<uint16_t *>(dst));
+    const TimingScope t("narrow_f16", as_stream(stream), true);
+    launch_timed(t, k_narrow_f16, grid_quads(count), dim3(CACHE_BLOCK), as_stream(stream), count, src,
+                 static_cast<uint16_t *>(dst));
     return check_launch("plm_narrow_f16");
 }
 
@@ -101,8 +103,9 @@ int plm_widen_f16(int64_t count, const void *src, float *dst, void *stream) {
     PLM_REQUIRE(count >= 0 && (count == 0 || (src && dst)), "plm_widen_f16: bad arguments");
     PLM_REQUIRE(aligned(src, 8) && aligned(dst, 16), "plm_widen_f16: src must be 8-byte and dst 16-byte aligned");
     if (count == 0) return 0;
-    k_widen_f16<<<grid_quads(count), CACHE_BLOCK, 0, as_stream(stream)>>>(count, static_cast<const uint16_t *>(src),
-                                                                          dst);
+    const TimingScope t("widen_f16", as_stream(stream), true);
+    launch_timed(t, k_widen_f16, grid_quads(count), dim3(CACHE_BLOCK), as_stream(stream), count,
+                 static_cast<const uint16_t *>(src), dst);
     return check_launch("plm_widen_f16");
 }
 
@@ -110,8 +113,9 @@ int plm_narrow_center_velocity(const plm_grid_info *info, const float *u, const 
                                void *center, void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && center, "plm_narrow_center_velocity: NULL array");
-    k_narrow_center_velocity<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(
-        *info, u, v, w, static_cast<uint16_t *>(center));
+    const TimingScope t("narrow_center_velocity", as_stream(stream), true);
+    launch_timed(t, k_narrow_center_velocity, grid3(cell_dims(*info)), block3(), as_stream(stream), *info, u, v, w,
+                 static_cast<uint16_t *>(center));
     return check_launch("plm_narrow_center_velocity");
 }
 

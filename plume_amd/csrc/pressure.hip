@@ -19,8 +19,6 @@
 #include <mutex>
 #include <type_traits>
 
-#include <hip/hip_ext.h>
-
 #include "common.h"
 
 namespace plm {
