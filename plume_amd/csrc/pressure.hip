@@ -18,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "common.h"
 
@@ -416,6 +417,16 @@ __device__ __forceinline__ float prolong_at_row(const T *__restrict__ coarse, Di
     return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
 }
 
+// f(integral_constant<int, 0>) .. f(integral_constant<int, N - 1>), in order, each index a compile-time constant
+template <typename F, int... U>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, U...>) {
+    (f(std::integral_constant<int, U>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD with its own L2. Renumbering so
 // that XCD x runs one contiguous range of tiles keeps a tile's halo neighbours in the same L2.
 constexpr int XCDS = 8;
@@ -432,7 +443,11 @@ __device__ __forceinline__ float divide_small(float x, float d, float r) {
     float q = x * r;
     const float e = __builtin_fmaf(-q, d, x);
     q = __builtin_fmaf(e, r, q);
-    if (__builtin_expect(__builtin_fabsf(x) < 0x1p-100f, 0)) q = x / d;
+    // tested once per wavefront (a ballot is a scalar value), so the common case saves no exec mask
+    const bool tiny = __builtin_fabsf(x) < 0x1p-100f;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(tiny) != 0, 0)) {
+        if (tiny) q = x / d;
+    }
     return q;
 }
 
@@ -601,24 +616,30 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
         const float target = divide_small(t0 - cm2 * rhs_c, t1[SIGMA], rcp[SIGMA]);
         b[LJ * LK] = q_c + relaxation * (target - q_c);
     };
-#pragma unroll
-    for (int u = 0; u < RED_ITERS; u++) {
-        if (wave + u * WAVES < RED_PAIRS) {
-            const int row = red_row[u] * LK;
-            if (red_sigma[u] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[u]);
-            else update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[u]);
-        }
-    }
+    // Iteration u of a colour takes pairs u WAVES .. u WAVES + WAVES - 1 (one per wavefront), and pairs are
+    // numbered class 0 first: an iteration whose pairs all lie in one class, or all exist, knows it at compile
+    // time, so only the iterations that straddle a class or the end test at run time.
+    static_for<RED_ITERS>([&](auto u_c) {
+        constexpr int U = decltype(u_c)::value, FIRST = U * WAVES, LAST = FIRST + WAVES - 1;
+        if (LAST >= RED_PAIRS && wave + FIRST >= RED_PAIRS) return;
+        const int row = red_row[U] * LK;
+        if constexpr (LAST < RED_CLASS) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[U]);
+        else if constexpr (FIRST >= RED_CLASS) update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[U]);
+        else if (red_sigma[U] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[U]);
+        else update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[U]);
+    });
     edge();
     __syncthreads();
-#pragma unroll
-    for (int u = 0; u < BLACK_ITERS; u++) {
-        if (wave + u * WAVES < BLACK_PAIRS) {
-            const int row = black_row[u] * LK;
-            if (black_sigma[u] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[u]);
-            else update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[u]);
-        }
-    }
+    static_for<BLACK_ITERS>([&](auto u_c) {
+        constexpr int U = decltype(u_c)::value, FIRST = U * WAVES, LAST = FIRST + WAVES - 1;
+        if (LAST >= BLACK_PAIRS && wave + FIRST >= BLACK_PAIRS) return;
+        const int row = black_row[U] * LK;
+        if constexpr (LAST < BLACK_CLASS) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[U]);
+        else if constexpr (FIRST >= BLACK_CLASS)
+            update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[U]);
+        else if (black_sigma[U] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[U]);
+        else update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[U]);
+    });
     __syncthreads();
     constexpr int OUT_ROWS = TI * TJ / WAVES;
     float v[OUT_ROWS];
