@@ -91,3 +91,28 @@ def test_a_hot_tiny_recipe_sheet_agrees_with_the_oracle_in_heat(device, tmp_path
     assert worst["R"] <= 3
     _, rows = read_numbers(gpu.numbers)
     assert max(float(row["peak_heat"]) for row in rows) == pytest.approx(1.0, abs=1.0 / 255)
+
+
+def test_a_variation_runs_like_the_recipe_it_resolves_to(device, tmp_path):
+    """A child recipe (extends the tiny recipe, overrides its seed and, by --set, its emitter radius) gives the
+    same frames and numbers as the merged recipe written out whole, and its settings log names the parent and
+    the set."""
+    import tomli
+
+    from plume_amd.pipeline import run_recipe
+
+    (tmp_path / "tiny.toml").write_text(TINY)
+    child = tmp_path / "child.toml"
+    child.write_text('extends = "tiny.toml"\nseed = 11\n')
+    varied = run_recipe(child, device="cuda", source="child.toml", out=tmp_path / "out",
+                        sets=("emitter.puff.radius_m=0.25",))
+    merged_text = TINY.replace("seed = 3", "seed = 11").replace("radius_m = 0.2", "radius_m = 0.25")
+    assert merged_text != TINY
+    merged = tmp_path / "child_merged.toml"
+    merged.write_text(merged_text)
+    whole = run_recipe(merged, device="cuda", source="child.toml")
+    assert varied.recipe.seed == 11 and varied.recipe.emitter[0].radius_m == 0.25
+    assert all(np.array_equal(a, b) for a, b in zip(varied.assembled.frames, whole.assembled.frames))
+    assert varied.numbers == whole.numbers
+    resolved = tomli.loads((tmp_path / "out" / "settings.toml").read_text())["resolved"]
+    assert resolved["extends_chain"] == ["tiny.toml"] and resolved["sets"] == ["emitter.puff.radius_m=0.25"]
