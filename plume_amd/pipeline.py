@@ -88,3 +88,25 @@ def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", s
         (Path(out) / "settings.toml").write_text(recipe_to_toml(recipe, derived, device_name, origin, facts),
                                                  newline="\n")
     return RunResult(recipe, derived, device_name, raw_frames, assembled, numbers, steps, seconds)
+
+
+def main(argv: list[str] | None = None) -> int:
+    """`python -m plume_amd.pipeline RECIPE [--preview] [--set key=value ...] [--out DIR]`: runs the recipe on
+    the HIP path, prints its numbers file and stage times, and with --out writes numbers.txt and settings.toml."""
+    import argparse
+
+    parser = argparse.ArgumentParser(prog="python -m plume_amd.pipeline", description=main.__doc__)
+    parser.add_argument("recipe", type=Path)
+    parser.add_argument("--preview", action="store_true")
+    parser.add_argument("--set", dest="sets", action="append", default=[], metavar="KEY=VALUE")
+    parser.add_argument("--out", type=Path, default=None)
+    args = parser.parse_args(argv)
+    result = run_recipe(args.recipe, preview=args.preview, device="cuda", out=args.out, sets=tuple(args.sets))
+    print(result.numbers, end="")
+    print(f"\nsubsteps {result.steps.total_steps}, simulate {result.seconds['simulate']:.3f} s, render "
+          f"{result.seconds['render']:.3f} s, assemble {result.seconds['assemble']:.3f} s")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
