@@ -322,19 +322,125 @@ __device__ __forceinline__ float potential(int32_t seed, int component, float px
     return lerp_(value_noise(seed, first, px, py, pz), value_noise(seed, first + 3, px, py, pz), blend);
 }
 
-// turbulence_kernel, sim/turbulence.py:35-53, on all three face arrays
+// The lattice value of a full corner hash (lattice_value's last step)
+__device__ __forceinline__ float corner_of(uint32_t h) { return (float)(h >> 8) * (2.0f / 16777215.0f) - 1.0f; }
+
+// value_noise's trilinear blend of its 8 corner values, in its order
+__device__ __forceinline__ float blend8(const float (&c)[2][2][2], float tx, float ty, float tz) {
+    const float c00 = lerp_(c[0][0][0], c[1][0][0], tx);
+    const float c10 = lerp_(c[0][1][0], c[1][1][0], tx);
+    const float c01 = lerp_(c[0][0][1], c[1][0][1], tx);
+    const float c11 = lerp_(c[0][1][1], c[1][1][1], tx);
+    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+}
+
+// value_noise of one stream (its seed-and-stream hash prefix `hs`) at two points lo and hi that are equal except
+// along axis A, where 0 <= hi - lo < 1 lattice unit: both points' corners lie in one 3 x 2 x 2 block of lattice
+// points (3 along A), which is hashed once (12 corner hashes instead of 16, and shared outer hashes). The
+// corner values and the blend are value_noise's, so both results are bit-identical to two value_noise calls.
+template <int A>
+__device__ __forceinline__ void value_noise_pair(uint32_t hs, const float (&lo)[3], const float (&hi)[3],
+                                                 float &v_lo, float &v_hi) {
+    const float f[3] = {floorf(lo[0]), floorf(lo[1]), floorf(lo[2])};
+    const float fa = floorf(hi[A]);
+    const int base[3] = {(int)f[0], (int)f[1], (int)f[2]};
+    const bool shift = (int)fa != base[A];  // hi's cell is lo's, or the next along A
+    const float tl[3] = {fade(lo[0] - f[0]), fade(lo[1] - f[1]), fade(lo[2] - f[2])};
+    float th[3] = {tl[0], tl[1], tl[2]};
+    th[A] = fade(hi[A] - fa);
+    constexpr int E0 = A == 0 ? 3 : 2, E1 = A == 1 ? 3 : 2, E2 = A == 2 ? 3 : 2;
+    float lattice[E0][E1][E2];
+#pragma unroll
+    for (int di = 0; di < E0; di++) {
+        const uint32_t h0 = hash_u32(hs ^ (uint32_t)(base[0] + di + LATTICE_OFFSET));
+#pragma unroll
+        for (int dj = 0; dj < E1; dj++) {
+            const uint32_t h1 = hash_u32(h0 ^ (uint32_t)(base[1] + dj + LATTICE_OFFSET));
+#pragma unroll
+            for (int dk = 0; dk < E2; dk++)
+                lattice[di][dj][dk] = corner_of(hash_u32(h1 ^ (uint32_t)(base[2] + dk + LATTICE_OFFSET)));
+        }
+    }
+    float c_lo[2][2][2], c_hi[2][2][2];
+#pragma unroll
+    for (int di = 0; di < 2; di++)
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+            for (int dk = 0; dk < 2; dk++) {
+                c_lo[di][dj][dk] = lattice[di][dj][dk];
+                const float same = lattice[di][dj][dk];
+                const float next = lattice[di + (A == 0)][dj + (A == 1)][dk + (A == 2)];
+                c_hi[di][dj][dk] = shift ? next : same;
+            }
+    v_lo = blend8(c_lo, tl[0], tl[1], tl[2]);
+    v_hi = blend8(c_hi, th[0], th[1], th[2]);
+}
+
+__device__ __forceinline__ uint32_t stream_prefix(int32_t seed, int32_t stream) {
+    return hash_u32(hash_u32((uint32_t)seed + (uint32_t)LATTICE_OFFSET) ^ (uint32_t)(stream + LATTICE_OFFSET));
+}
+
+// potential() of component `comp` at the two points face - h and face + h (h = 0.5 cells along axis A), both
+// time slices through value_noise_pair
+template <int A>
+__device__ __forceinline__ void potential_pair(int32_t seed, int comp, const float (&face)[3], float to_lattice,
+                                               float pattern_time, float &minus, float &plus) {
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float h = a == A ? 0.5f : 0.0f;
+        lo[a] = (face[a] - h) * to_lattice;
+        hi[a] = (face[a] + h) * to_lattice;
+    }
+    const float slice_lo = floorf(pattern_time);
+    const float blend = fade(pattern_time - slice_lo);
+    const int32_t first = (1 << 24) + 3 * (int32_t)slice_lo + comp;
+    float lo0, hi0, lo1, hi1;
+    value_noise_pair<A>(stream_prefix(seed, first), lo, hi, lo0, hi0);
+    value_noise_pair<A>(stream_prefix(seed, first + 3), lo, hi, lo1, hi1);
+    minus = lerp_(lo0, lo1, blend);
+    plus = lerp_(hi0, hi1, blend);
+}
+
+// turbulence_kernel on one face array (AXIS) with the paired noise: b = AXIS + 1, c = AXIS + 2 (mod 3)
+template <int AXIS>
+__device__ __forceinline__ void turbulence_paired(const plm_grid_info &g, float *field, const Dims &d, int i, int j,
+                                                  int k, int32_t seed, float to_lattice, float strength,
+                                                  float pattern_time, float dt) {
+    constexpr int B = (AXIS + 1) % 3, C = (AXIS + 2) % 3;
+    const float face[3] = {(float)i + (AXIS == 0 ? 0.0f : 0.5f), (float)j + (AXIS == 1 ? 0.0f : 0.5f),
+                           (float)k + (AXIS == 2 ? 0.0f : 0.5f)};
+    float c_minus, c_plus, b_minus, b_plus;
+    potential_pair<B>(seed, C, face, to_lattice, pattern_time, c_minus, c_plus);
+    potential_pair<C>(seed, B, face, to_lattice, pattern_time, b_minus, b_plus);
+    const float curl = ((c_plus - c_minus) - (b_plus - b_minus)) / to_lattice;
+    float *p = field + d.at(i, j, k);
+    *p = *p + strength * curl * dt;
+}
+
+// turbulence_kernel, sim/turbulence.py:35-53, on all three face arrays. PAIRED (the host's choice when half a
+// cell is well under half a lattice unit, cell_m / scale_m < 0.99) evaluates each potential's two points with
+// one shared lattice block; otherwise every point's noise is evaluated on its own. Both are bit-identical.
+template <bool PAIRED>
 __global__ __launch_bounds__(256) void k_turbulence(plm_grid_info g, Tri<float> f, int32_t seed, float scale_m,
                                                     float strength, float pattern_time, float dt) {
     int axis, i, j, k;
     Dims d;
     if (!thread_face(g, axis, d, i, j, k)) return;
+    const float to_lattice = g.cell_m / scale_m;
+    if constexpr (PAIRED) {
+        if (axis == 0) turbulence_paired<0>(g, f.p[0], d, i, j, k, seed, to_lattice, strength, pattern_time, dt);
+        else if (axis == 1) turbulence_paired<1>(g, f.p[1], d, i, j, k, seed, to_lattice, strength, pattern_time, dt);
+        else turbulence_paired<2>(g, f.p[2], d, i, j, k, seed, to_lattice, strength, pattern_time, dt);
+        return;
+    }
     const float face[3] = {(float)i + (axis == 0 ? 0.0f : 0.5f), (float)j + (axis == 1 ? 0.0f : 0.5f),
                            (float)k + (axis == 2 ? 0.0f : 0.5f)};
     const int b = (axis + 1) % 3, c = (axis + 2) % 3;
     float hb[3] = {0.0f, 0.0f, 0.0f}, hc[3] = {0.0f, 0.0f, 0.0f};
     hb[b] = 0.5f;
     hc[c] = 0.5f;
-    const float to_lattice = g.cell_m / scale_m;
     const float c_plus = potential(seed, c, (face[0] + hb[0]) * to_lattice, (face[1] + hb[1]) * to_lattice,
                                    (face[2] + hb[2]) * to_lattice, pattern_time);
     const float c_minus = potential(seed, c, (face[0] - hb[0]) * to_lattice, (face[1] - hb[1]) * to_lattice,
@@ -479,7 +585,11 @@ int plm_turbulence(const plm_grid_info *info, float *u, float *v, float *w, int3
     if (strength == 0.0f) return 0;
     hipStream_t s = as_stream(stream);
     TimingScope t("turbulence", s);
-    k_turbulence<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
+    if (info->cell_m / scale_m < 0.99f)
+        k_turbulence<true><<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
+                                                       pattern_time, dt);
+    else
+        k_turbulence<false><<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
                                                        pattern_time, dt);
     return check_launch("plm_turbulence");
 }

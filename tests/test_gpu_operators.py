@@ -232,6 +232,21 @@ def test_turbulence_matches_oracle(device, oracle32, cells, faces):
         assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
 
 
+@pytest.mark.parametrize("scale_m", [0.011, 0.0102, 0.5, 0.009])
+def test_turbulence_is_bit_exact_across_noise_scales(device, oracle32, scale_m):
+    """On 1 cm cells: 0.011 m puts half a cell at 0.45 lattice units (the two points of each curl difference often
+    lie in neighbouring lattice cells), 0.0102 m at 0.49 (the largest the shared-lattice form takes), 0.5 m at
+    0.01; at 0.009 m a cell is more than one lattice unit and every point's noise is evaluated on its own."""
+    from oracle.sim import apply_turbulence as oracle_turbulence
+    from plume_amd.sim.turbulence import apply_turbulence
+
+    gpu, orc = pair(device, oracle32, (31, 33, 29), DEFAULT_FACES)
+    apply_turbulence(gpu, 11, scale_m, 0.12, 0.2, 1.63, DT)
+    oracle_turbulence(orc, 11, scale_m, 0.12, 0.2, 1.63, DT)
+    for name in ("u", "v", "w"):
+        assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
+
+
 # ---------------------------------------------------------------------- pressure and multigrid
 
 @pytest.mark.parametrize("cells, faces", GRIDS)
