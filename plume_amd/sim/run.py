@@ -33,6 +33,19 @@ class StepLog:
         return self.total_steps - base
 
 
+_TUNED_GRIDS: set[tuple[int, int, int]] = set()
+
+
+def tune_sweep_tiles(solve, cells) -> None:
+    """Pins the fastest tile shape of the two-colour sweep for each multigrid level of this grid size, once
+    per process (a few milliseconds; every shape gives bit-identical results, so only the speed changes)."""
+    key = tuple(int(c) for c in cells)
+    if key in _TUNED_GRIDS or not hasattr(solve, "autotune"):
+        return
+    solve.autotune()
+    _TUNED_GRIDS.add(key)
+
+
 def simulate(recipe, derived, device: str, capture: Capture) -> StepLog:
     """Runs the whole recipe, calling `capture(frame_index, grid)` once per frame (sim/run.py:42-50)."""
     frames = simulate_frames(recipe, derived, device)
@@ -48,6 +61,7 @@ def simulate_frames(recipe, derived, device: str) -> Generator[tuple[int, Grid],
     """Yields (frame_index, grid) after each frame's steps and returns the step log (sim/run.py:53-85)."""
     grid = make_grid(derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k, device)
     stepper = Stepper(grid, recipe)
+    tune_sweep_tiles(stepper.pressure_solve, derived.cells)
     cap = StepCap(recipe)
     base_done = 0
 
