@@ -1,9 +1,9 @@
 // The viewer's use of the path: its frame cache (view/frame_cache.py) keeps each frame's smoke, temperature
 // and cell-centred velocity as IEEE binary16, and its live run (view/live_run.py) reads the smoke's six
 // boundary layers. All four kernels are streaming copies with a conversion, bound by HBM bandwidth.
-//   - k_narrow_f16 / k_widen_f16: 4 values per thread, one 16-byte float load or store and one 8-byte half
-//     store or load; rounding to binary16 is round-to-nearest-even (v_cvt_f16_f32 under the default mode),
-//     subnormal halves kept. A vec3h array widens as a flat array of 3N halves.
+//   - k_narrow_f16 / k_widen_f16: 8 values per thread in two contiguous passes of the block, each a 16-byte
+//     float access and an 8-byte half access; rounding to binary16 is round-to-nearest-even (v_cvt_f16_f32
+//     under the default mode), subnormal halves kept. A vec3h array widens as a flat array of 3N halves.
 //   - k_narrow_center_velocity: center_velocity_kernel (sim/vorticity.py:17-22) and narrow_vector_kernel
 //     (view/frame_cache.py:38-42) fused: the FP32 centre 0.5 (a + b) per axis, rounded once to binary16 and
 //     stored as 3 halves per cell (the vec3h layout), with no FP32 vec3 array in between.
@@ -14,33 +14,54 @@ namespace plm {
 namespace {
 
 constexpr int CACHE_BLOCK = 256;
+constexpr int PER_THREAD = 8;  // values per thread: two 16-byte float accesses
 
 __device__ __forceinline__ uint16_t half_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 __device__ __forceinline__ float half_value(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// A block converts PER_THREAD * CACHE_BLOCK values; its q-th pass takes the q-th run of 4 * CACHE_BLOCK of them,
+// 4 per thread, so each load and store of a wavefront is contiguous, and a thread's passes are in flight together.
 __global__ __launch_bounds__(CACHE_BLOCK) void k_narrow_f16(int64_t count, const float *__restrict__ src,
                                                             uint16_t *__restrict__ dst) {
-    const int64_t first = ((int64_t)blockIdx.x * CACHE_BLOCK + threadIdx.x) * 4;
-    if (first + 4 <= count) {
-        const float4 v = *reinterpret_cast<const float4 *>(src + first);
-        const uint32_t lo = (uint32_t)half_bits(v.x) | ((uint32_t)half_bits(v.y) << 16);
-        const uint32_t hi = (uint32_t)half_bits(v.z) | ((uint32_t)half_bits(v.w) << 16);
-        *reinterpret_cast<uint2 *>(dst + first) = make_uint2(lo, hi);
-    } else {
-        for (int64_t e = first; e < count; e++) dst[e] = half_bits(src[e]);
+    const int64_t base = (int64_t)blockIdx.x * CACHE_BLOCK * PER_THREAD + 4 * (int64_t)threadIdx.x;
+    float4 v[PER_THREAD / 4];
+#pragma unroll
+    for (int q = 0; q < PER_THREAD / 4; q++) {
+        const int64_t e = base + (int64_t)q * 4 * CACHE_BLOCK;
+        v[q] = e + 4 <= count ? *reinterpret_cast<const float4 *>(src + e) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int q = 0; q < PER_THREAD / 4; q++) {
+        const int64_t e = base + (int64_t)q * 4 * CACHE_BLOCK;
+        if (e + 4 <= count) {
+            const uint32_t lo = (uint32_t)half_bits(v[q].x) | ((uint32_t)half_bits(v[q].y) << 16);
+            const uint32_t hi = (uint32_t)half_bits(v[q].z) | ((uint32_t)half_bits(v[q].w) << 16);
+            *reinterpret_cast<uint2 *>(dst + e) = make_uint2(lo, hi);
+        } else {
+            for (int64_t x = e; x < count && x < e + 4; x++) dst[x] = half_bits(src[x]);
+        }
     }
 }
 
 __global__ __launch_bounds__(CACHE_BLOCK) void k_widen_f16(int64_t count, const uint16_t *__restrict__ src,
                                                            float *__restrict__ dst) {
-    const int64_t first = ((int64_t)blockIdx.x * CACHE_BLOCK + threadIdx.x) * 4;
-    if (first + 4 <= count) {
-        const uint2 h = *reinterpret_cast<const uint2 *>(src + first);
-        *reinterpret_cast<float4 *>(dst + first) =
-            make_float4(half_value((uint16_t)(h.x & 0xffffu)), half_value((uint16_t)(h.x >> 16)),
-                        half_value((uint16_t)(h.y & 0xffffu)), half_value((uint16_t)(h.y >> 16)));
-    } else {
-        for (int64_t e = first; e < count; e++) dst[e] = half_value(src[e]);
+    const int64_t base = (int64_t)blockIdx.x * CACHE_BLOCK * PER_THREAD + 4 * (int64_t)threadIdx.x;
+    uint2 h[PER_THREAD / 4];
+#pragma unroll
+    for (int q = 0; q < PER_THREAD / 4; q++) {
+        const int64_t e = base + (int64_t)q * 4 * CACHE_BLOCK;
+        h[q] = e + 4 <= count ? *reinterpret_cast<const uint2 *>(src + e) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int q = 0; q < PER_THREAD / 4; q++) {
+        const int64_t e = base + (int64_t)q * 4 * CACHE_BLOCK;
+        if (e + 4 <= count) {
+            *reinterpret_cast<float4 *>(dst + e) =
+                make_float4(half_value((uint16_t)(h[q].x & 0xffffu)), half_value((uint16_t)(h[q].x >> 16)),
+                            half_value((uint16_t)(h[q].y & 0xffffu)), half_value((uint16_t)(h[q].y >> 16)));
+        } else {
+            for (int64_t x = e; x < count && x < e + 4; x++) dst[x] = half_value(src[x]);
+        }
     }
 }
 
@@ -78,7 +99,9 @@ __global__ __launch_bounds__(BX * BY) void k_boundary_layers(Dims n, const float
     layers[((int64_t)face * extent_a + a) * extent_b + b] = value;
 }
 
-inline dim3 grid_quads(int64_t count) { return dim3((unsigned)((count + 4 * CACHE_BLOCK - 1) / (4 * CACHE_BLOCK))); }
+inline dim3 grid_quads(int64_t count) {
+    return dim3((unsigned)((count + PER_THREAD * CACHE_BLOCK - 1) / (PER_THREAD * CACHE_BLOCK)));
+}
 
 bool aligned(const void *p, uintptr_t bytes) { return (reinterpret_cast<uintptr_t>(p) & (bytes - 1)) == 0; }
 
