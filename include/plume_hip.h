@@ -15,7 +15,8 @@
  *   - Every call returns 0 on success, PLM_EINVAL for a bad argument (nothing launched), or the
  *     hipError_t of a failed launch. plm_last_error() gives this thread's last message.
  *   - The library allocates no device memory behind the caller's back: every buffer, scratch included,
- *     comes in as an argument.
+ *     comes in as an argument. The one exception is plm_smoke_budget_noalloc, whose scratch is a static
+ *     device array of fixed size, loaded with the library.
  */
 #ifndef PLUME_HIP_H
 #define PLUME_HIP_H
@@ -346,6 +347,17 @@ int plm_line_sums(const int32_t shape[3], const float *a, const float *b, const 
    lines: 3*ny*nz floats; totals (device doubles, 4): before, semi-Lagrangian, MacCormack, factor. */
 int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat, float *next, float *lines,
                      double *totals, void *stream);
+/* No counterpart in the reference: plm_smoke_budget for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels and results,
+   but the line sums and FP64 totals live in a static device array of the library (each GPU holds its own, of
+   3 * PLM_BUDGET_NOALLOC_MAX_LINES floats plus 4 doubles, until the process ends), so nothing is allocated.
+   Grids with ny * nz above PLM_BUDGET_NOALLOC_MAX_LINES are rejected with PLM_EINVAL. totals (device, 4
+   doubles, or NULL) receives before, semi-Lagrangian, MacCormack and factor. Calls on one GPU share the
+   array: calls that overlap on one GPU (other streams or threads) race. Call with the stream's device
+   current. */
+#define PLM_BUDGET_NOALLOC_MAX_LINES 262144
+int plm_smoke_budget_noalloc(const int32_t shape[3], const float *data, const float *hat, float *next,
+                             double *totals, void *stream);
 /* scale_kernel, sim/conserve.py:27-30. */
 int plm_scale(int64_t count, float *f, float factor, void *stream);
 

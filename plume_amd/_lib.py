@@ -116,6 +116,7 @@ SIGNATURES = {
                          _P, _P, _P],
     "plm_line_sums": [_P, _P, _P, _P, _P, _P],
     "plm_smoke_budget": [_P, _P, _P, _P, _P, _P, _P],
+    "plm_smoke_budget_noalloc": [_P, _P, _P, _P, _P, _P],
     "plm_scale": [_L, _P, _F, _P],
     "plm_max_abs3": [_INFO, _P, _P, _P, _P, _P],
     "plm_light_volume": [_P, _P, _P, _F, _F, _I, _P, _P],

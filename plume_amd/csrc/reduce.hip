@@ -1,6 +1,8 @@
 // Deterministic reductions: the smoke budget's line sums and totals (sim/reduce.py, sim/conserve.py) and the
 // CFL cap's fastest face velocity (sim/step_cap.py). No float atomics: sums run in a fixed order, and the
 // maximum (order-free) uses an integer atomic on the non-negative float's bits.
+#include <mutex>
+
 #include "common.h"
 
 namespace plm {
@@ -44,9 +46,11 @@ __global__ __launch_bounds__(LINE_THREADS) void k_line_sums(Dims n, const float 
 constexpr int BUDGET_THREADS = 1024;
 
 // TripleSum's host float64 totals (sim/reduce.py:37) by a fixed-order tree, then SmokeBudget.hold's factor
-// (sim/conserve.py:48-50): totals = before, semi-Lagrangian, MacCormack, factor.
+// (sim/conserve.py:48-50): totals = before, semi-Lagrangian, MacCormack, factor. `copy`, when not null,
+// receives the same four values.
 __global__ __launch_bounds__(BUDGET_THREADS) void k_budget_totals(int plane, const float *__restrict__ lines,
-                                                                  double *__restrict__ totals) {
+                                                                  double *__restrict__ totals,
+                                                                  double *__restrict__ copy) {
     __shared__ double s[3][BUDGET_THREADS];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int t = threadIdx.x; t < plane; t += BUDGET_THREADS)
@@ -66,6 +70,8 @@ __global__ __launch_bounds__(BUDGET_THREADS) void k_budget_totals(int plane, con
     totals[1] = semi_lagrangian;
     totals[2] = maccormack;
     totals[3] = maccormack > 0.0 ? target / maccormack : 1.0;
+    if (copy != nullptr)
+        for (int q = 0; q < 4; q++) copy[q] = totals[q];
 }
 
 // scale_kernel, sim/conserve.py:27-30, applied only when the factor is not 1 (sim/conserve.py:51-52)
@@ -110,6 +116,38 @@ __global__ __launch_bounds__(256) void k_max_abs3(Tri<const float> f, int64_t nu
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));
 }
 
+// The scratch of plm_smoke_budget_noalloc: the four FP64 totals, then the line sums of up to
+// PLM_BUDGET_NOALLOC_MAX_LINES (j, k) lines of the three fields. Each GPU holds its own copy until the process
+// ends; calls on one GPU share it.
+constexpr size_t NOALLOC_LINES_OFFSET = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_LINES_OFFSET + 3 * sizeof(float) * (size_t)PLM_BUDGET_NOALLOC_MAX_LINES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_address[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device. In host code the array's
+// name is only a stand-in for the symbol, so kernels get this address, never the name.
+int scratch_address(void **address) {
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return fail((int)e, "plm_smoke_budget_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(PLM_EINVAL, "plm_smoke_budget_noalloc: device %d beyond the library's %d", device, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_address[device] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(e != hipSuccess ? (int)e : PLM_EINVAL, "plm_smoke_budget_noalloc: hipGetSymbolAddress: %s",
+                        hipGetErrorString(e));
+        g_scratch_address[device] = p;
+    }
+    *address = g_scratch_address[device];
+    return 0;
+}
+
 inline dim3 grid1(int64_t count, int64_t cap = 16384) {
     int64_t blocks = (count + 255) / 256;
     return dim3((unsigned)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap));
@@ -141,10 +179,33 @@ int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat
     TimingScope t("budget", s);
     k_line_sums<<<(n.n1 * n.n2 + LINE_THREADS - 1) / LINE_THREADS, LINE_THREADS, 0, s>>>(n, data, hat, next, lines);
     PLM_TRY(check_launch("plm_smoke_budget line sums"));
-    k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, totals);
+    k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, totals, nullptr);
     PLM_TRY(check_launch("plm_smoke_budget totals"));
     k_scale_by_total<<<grid1(n.count()), 256, 0, s>>>(n.count(), next, totals);
     return check_launch("plm_smoke_budget scale");
+}
+
+int plm_smoke_budget_noalloc(const int32_t shape[3], const float *data, const float *hat, float *next,
+                             double *totals, void *stream) {
+    PLM_TRY(validate_dims(shape, "plm_smoke_budget_noalloc"));
+    PLM_REQUIRE(data && hat && next, "plm_smoke_budget_noalloc: NULL array");
+    const Dims n = dims_of(shape);
+    PLM_REQUIRE((int64_t)n.n1 * n.n2 <= PLM_BUDGET_NOALLOC_MAX_LINES,
+                "plm_smoke_budget_noalloc: ny * nz = %d * %d lines exceeds the static scratch's %d", n.n1, n.n2,
+                PLM_BUDGET_NOALLOC_MAX_LINES);
+    void *base = nullptr;
+    PLM_TRY(scratch_address(&base));
+    double *held = static_cast<double *>(base);
+    float *lines = reinterpret_cast<float *>(static_cast<unsigned char *>(base) + NOALLOC_LINES_OFFSET);
+    hipStream_t s = as_stream(stream);
+    TimingScope t("budget", s);
+    // every word the kernels read is written first in this call: nothing to clear
+    k_line_sums<<<(n.n1 * n.n2 + LINE_THREADS - 1) / LINE_THREADS, LINE_THREADS, 0, s>>>(n, data, hat, next, lines);
+    PLM_TRY(check_launch("plm_smoke_budget_noalloc line sums"));
+    k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, held, totals);
+    PLM_TRY(check_launch("plm_smoke_budget_noalloc totals"));
+    k_scale_by_total<<<grid1(n.count()), 256, 0, s>>>(n.count(), next, held);
+    return check_launch("plm_smoke_budget_noalloc scale");
 }
 
 int plm_scale(int64_t count, float *f, float factor, void *stream) {

@@ -100,3 +100,14 @@ def test_the_product_refuses_the_cpu_device():
 
     with pytest.raises(DeviceError):
         init_device("cpu")
+
+
+def test_no_alloc_budget_rejects_large_planes_on_the_host():
+    """plm_smoke_budget_noalloc checks the plane against its static scratch before any HIP call."""
+    import plume_amd._lib as lib
+
+    shape = lib.int3((4, 1024, 512))
+    rc = lib.LIB.plm_smoke_budget_noalloc(shape, ctypes.c_void_p(256), ctypes.c_void_p(256), ctypes.c_void_p(256),
+                                          None, None)
+    assert rc == -1
+    assert b"static scratch" in lib.LIB.plm_last_error()
