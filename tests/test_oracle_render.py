@@ -217,3 +217,100 @@ def test_near_origin_orthographic_rays_are_the_same_lines(oracle64):
         close = render_oracle(oracle64, g, view, settings, layout, params=near)
         assert far[..., THICKNESS].max() > 0.9
         assert np.allclose(close, far, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------------- normal field (render/normal_field.py)
+
+def test_gaussian_weights_are_a_normalised_sampled_gaussian_of_sigma():
+    # render/normal_field.py:64-69 and the radius ceil(3 sigma) of :52: properties that do not restate the
+    # formula; a wrong sigma (2 sigma^2 vs sigma^2), radius or normalisation fails one of them
+    from oracle.render import gaussian_weights
+
+    for sigma in (0.7, 1.0, 1.5, 2.5):
+        radius = math.ceil(3.0 * sigma)
+        w = gaussian_weights(sigma, radius).astype(np.float64)
+        assert w.shape == (2 * radius + 1,)
+        assert w.sum() == pytest.approx(1.0, abs=1e-6)
+        np.testing.assert_array_equal(w, w[::-1])
+        assert w[radius + 1] / w[radius] == pytest.approx(math.exp(-1.0 / (2.0 * sigma**2)), rel=1e-6)
+        offsets = np.arange(-radius, radius + 1)
+        # the kernel truncated at 3 sigma keeps 97% of a continuous Gaussian's variance
+        assert (offsets**2 * w).sum() == pytest.approx(sigma**2, rel=0.05)
+    np.testing.assert_array_equal(gaussian_weights(1.0, 0), np.ones(1, dtype=np.float32))
+
+
+def test_blur_axis_of_an_impulse_lays_out_the_weights_with_clamped_edges(oracle64):
+    # blur_axis_kernel (render/normal_field.py:12-22): dst[c] = sum_o w[o + r] src[clamp(c + o)] along `axis`
+    from oracle.render import gaussian_weights
+
+    sigma, radius = 1.2, 4
+    weights = gaussian_weights(sigma, radius).astype(np.float64)
+    shape = (12, 9, 10)
+    for axis in range(3):
+        for position in (5, 0, shape[axis] - 1):
+            src = np.zeros(shape)
+            index = [3, 4, 5]
+            index[axis] = position
+            src[tuple(index)] = 1.0
+            dst = np.zeros(shape)
+            oracle64.blur_axis(src, dst, weights, radius, axis)
+            expected = np.zeros(shape[axis])
+            for c in range(shape[axis]):
+                expected[c] = sum(weights[o + radius] for o in range(-radius, radius + 1)
+                                  if min(max(c + o, 0), shape[axis] - 1) == position)
+            line = [slice(v, v + 1) for v in (3, 4, 5)]
+            line[axis] = slice(None)
+            np.testing.assert_allclose(dst[tuple(line)].ravel(), expected, rtol=0, atol=1e-15)
+            rest = dst.copy()
+            rest[tuple(line)] = 0.0
+            assert not rest.any(), (axis, position)
+    # interior cells of an impulse at the centre read the weights themselves, in order
+    src = np.zeros((11, 1, 1))
+    src[5] = 1.0
+    dst = np.zeros_like(src)
+    oracle64.blur_axis(src, dst, weights, radius, 0)
+    np.testing.assert_allclose(dst.ravel()[1:10], weights, atol=1e-15)
+
+
+def test_blur_keeps_linear_fields_in_the_interior(oracle64):
+    # a symmetric normalised kernel reproduces a linear function: a shifted or lopsided kernel would not
+    from oracle.render import gaussian_weights
+
+    sigma = 1.5
+    radius = math.ceil(3.0 * sigma)
+    weights = gaussian_weights(sigma, radius).astype(np.float64)
+    weights = weights / weights.sum()  # the float32 rounding of the table aside
+    idx = np.indices((20, 18, 16)).astype(np.float64)
+    src = 0.25 * idx[0] - 0.5 * idx[1] + 0.125 * idx[2] + 3.0
+    for axis in range(3):
+        dst = np.zeros_like(src)
+        oracle64.blur_axis(src, dst, weights, radius, axis)
+        inner = [slice(None)] * 3
+        inner[axis] = slice(radius, src.shape[axis] - radius)
+        np.testing.assert_allclose(dst[tuple(inner)], src[tuple(inner)], rtol=0, atol=1e-12)
+
+
+def test_gradient_is_central_inside_and_one_sided_at_the_edges(oracle64):
+    # gradient_kernel (render/normal_field.py:25-38): (src[hi] - src[lo]) / (hi - lo) with hi and lo clamped
+    # one cell either side; an axis of one cell gives 0
+    shape = (7, 6, 5)
+    idx = np.indices(shape).astype(np.float64)
+    affine = 0.5 * idx[0] - 2.0 * idx[1] + 0.25 * idx[2] + 1.0
+    gradient = np.zeros(shape + (3,))
+    oracle64.gradient(affine, gradient)
+    np.testing.assert_array_equal(gradient[..., 0], 0.5)
+    np.testing.assert_array_equal(gradient[..., 1], -2.0)
+    np.testing.assert_array_equal(gradient[..., 2], 0.25)
+    square = idx[0] ** 2
+    oracle64.gradient(square, gradient)
+    n = shape[0]
+    expected = np.array([1.0] + [2.0 * i for i in range(1, n - 1)] + [2.0 * n - 3.0])
+    np.testing.assert_array_equal(gradient[:, 2, 2, 0], expected)
+    np.testing.assert_array_equal(gradient[..., 1:], 0.0)
+    flat = np.zeros((1, 4, 3))
+    flat[0] = np.arange(12.0).reshape(4, 3)
+    gradient = np.zeros((1, 4, 3, 3))
+    oracle64.gradient(flat, gradient)
+    np.testing.assert_array_equal(gradient[..., 0], 0.0)
+    np.testing.assert_array_equal(gradient[..., 1], 3.0)
+    np.testing.assert_array_equal(gradient[..., 2], 1.0)

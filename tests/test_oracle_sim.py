@@ -342,6 +342,49 @@ def test_soft_edge_is_the_smoothstep_of_the_distance(oracle64):
     assert np.allclose(grid.smoke.data, expected, atol=1e-12)
 
 
+def box_soft_edge(distance, extent, softness):
+    """soft_edge (sim/emitters.py:41-50) of numpy distances, in float64."""
+    inner = extent * (1.0 - softness)
+    t = np.clip((distance - inner) / (extent - inner), 0.0, 1.0)
+    return np.where(distance <= inner, 1.0, np.where(distance >= extent, 0.0, 1.0 - t * t * (3 - 2 * t)))
+
+
+def test_box_coverage_is_the_product_of_the_axes_soft_edges_over_half_sizes(oracle64):
+    # coverage of a box (sim/emitters.py:53-67): the product over axes of the soft edge of |x - c| against
+    # half the box size (emitter_params halves size_m, :132). Sizes differ per axis and the centre sits off
+    # the lattice, so a full-size extent, a sum over axes or a swapped axis each change many cells.
+    center, size, softness = (0.05, 0.6, -0.1), (0.5, 0.3, 0.4), 0.4
+    recipe = tiny_recipe(emitter=[{"name": "b", "shape": "box", "center_m": list(center), "size_m": list(size),
+                                   "end_s": 1.0, "smoke_per_s": 1.0, "edge_softness": softness}])
+    grid = small(oracle64, cells=(24, 24, 24), cell_m=0.05)
+    emit(grid, emitter_params(recipe.emitter), 0, 0.0, 1.0)
+    idx = np.indices(grid.cells) + 0.5
+    position = (idx[0] * 0.05 - 0.6, idx[1] * 0.05, idx[2] * 0.05 - 0.6)
+    expected = np.ones(grid.cells)
+    for axis in range(3):
+        expected = expected * box_soft_edge(np.abs(position[axis] - center[axis]), size[axis] / 2, softness)
+    assert 0.0 < expected.max() == 1.0 and ((expected > 0) & (expected < 1)).sum() > 100
+    assert np.allclose(grid.smoke.data, expected, atol=1e-12)
+
+
+def test_box_emitter_holds_temperature_and_outward_velocity_inside(oracle64):
+    # full coverage blends all the way (blend_factor of 1 is 1 for any dt, sim/emitters.py:70-77); a box
+    # uses the same outward direction normalize(x - c) * |v| as a sphere (:105-121)
+    recipe = tiny_recipe(emitter=[{"name": "b", "shape": "box", "center_m": [0.0, 0.6, 0.0],
+                                   "size_m": [0.6, 0.4, 0.5], "end_s": 1.0, "smoke_per_s": 1.0,
+                                   "edge_softness": 0.0, "temperature_k": 700.0, "velocity_mps": [0.0, 3.0, 0.0],
+                                   "velocity_mode": "outward"}])
+    grid = small(oracle64, cells=(24, 24, 24), cell_m=0.05)
+    emit(grid, emitter_params(recipe.emitter), 0, 0.0, 0.01)
+    # cell (12, 12, 12) is centred at (0.025, 0.625, 0.025): inside the box
+    assert grid.temperature.data[12, 12, 12] == 700.0
+    assert grid.temperature.data[12, 12, 20] == 293.0  # z = 0.425 m: outside the 0.25 m half depth
+    # u face (16, 12, 12) at (0.2, 0.625, 0.025): away = (0.2, 0.025, 0.025) from the centre
+    away = np.array([0.2, 0.025, 0.025])
+    assert grid.u.data[16, 12, 12] == pytest.approx(3.0 * away[0] / np.linalg.norm(away), rel=1e-12)
+    assert grid.u.data[8, 12, 12] == pytest.approx(-3.0 * 0.2 / np.linalg.norm(away), rel=1e-12)
+
+
 # ---------------------------------------------------------------------- turbulence (tests/sim/test_turbulence.py)
 
 def pushed(o, seed=1, time_s=0.0, cells=(16, 16, 16), cell_m=0.1, scale_m=0.4, speed_mps=0.2):
