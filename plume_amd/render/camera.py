@@ -1,7 +1,14 @@
-"""The orthographic camera and the domain view (render/camera.py of the reference). Host-side float64.
+"""Orthographic camera and the sheet's domain view (the interface of render/camera.py:16-78), host-side in float64.
 
-Camera space is OpenGL-style: x right, y up, z toward the camera. Elevation tilts the camera to look down;
-azimuth orbits it counterclockwise seen from above, starting on the +z side looking along -z.
+The camera's axes are the columns of one rotation, R = R_y(azimuth) · R_x(-elevation): column 0 is the
+camera's right, column 1 its up and column 2 the direction from the scene toward the camera (camera-space +z,
+OpenGL style). At elevation 0 and azimuth 0 the camera sits on the +z side looking along -z; elevation tilts it
+to look down, azimuth orbits it counterclockwise seen from above.
+
+The domain view is the smallest pixel rectangle on the camera plane that holds the projection of the whole
+domain box: the box's eight corners are projected on right and up, the rectangle spans their extremes, and its
+size in pixels rounds up (with a 1e-6 pixel allowance, so a span of an exact pixel count is not rounded up one
+more by float error). Pixel (0, 0) is the top-left one.
 """
 
 import math
@@ -10,30 +17,43 @@ from dataclasses import dataclass
 import numpy as np
 
 
+def _rotation(elevation_deg: float, azimuth_deg: float) -> np.ndarray:
+    """R_y(azimuth) · R_x(-elevation), written out: columns right, up, toward."""
+    ce, se = math.cos(math.radians(elevation_deg)), math.sin(math.radians(elevation_deg))
+    ca, sa = math.cos(math.radians(azimuth_deg)), math.sin(math.radians(azimuth_deg))
+    # + 0.0 turns the -0.0 of a zero product into +0.0
+    return np.array([[ca, -se * sa, ce * sa],
+                     [0.0, ce, se],
+                     [-sa, -se * ca, ce * ca]]) + 0.0
+
+
 @dataclass(frozen=True)
 class Camera:
+    """World-space unit axes of the camera."""
+
     right: np.ndarray
     up: np.ndarray
     toward: np.ndarray  # from the scene toward the camera: camera-space +z
 
+    @property
+    def basis(self) -> np.ndarray:
+        """3×3 matrix whose rows are right, up and toward (world → camera space)."""
+        return np.stack([self.right, self.up, self.toward])
+
     def to_camera_space(self, world_vector: np.ndarray) -> np.ndarray:
-        return np.array([world_vector @ self.right, world_vector @ self.up, world_vector @ self.toward])
+        return self.basis @ np.asarray(world_vector, dtype=np.float64)
 
 
 def make_camera(elevation_deg: float, azimuth_deg: float) -> Camera:
-    """The camera's world-space basis for an elevation and azimuth in degrees (render/camera.py:26-34)."""
-    elevation = math.radians(elevation_deg)
-    azimuth = math.radians(azimuth_deg)
-    toward = np.array([math.sin(azimuth) * math.cos(elevation), math.sin(elevation),
-                       math.cos(azimuth) * math.cos(elevation)])
-    right = np.array([math.cos(azimuth), 0.0, -math.sin(azimuth)])
-    up = np.cross(toward, right)
-    return Camera(right=right, up=up / np.linalg.norm(up), toward=toward)
+    """The camera for an elevation and an azimuth in degrees (render/camera.py:26-34)."""
+    rotation = _rotation(elevation_deg, azimuth_deg)
+    return Camera(right=rotation[:, 0].copy(), up=rotation[:, 1].copy(), toward=rotation[:, 2].copy())
 
 
 @dataclass(frozen=True)
 class DomainView:
-    """A pixel grid on the camera plane covering the domain's projection; pixel (0, 0) is top-left."""
+    """A pixel grid on the camera plane: its top-left corner (left_m, top_m) in camera-plane metres, the pixel
+    pitch and the size in pixels (render/camera.py:37-52)."""
 
     camera: Camera
     left_m: float
@@ -43,26 +63,31 @@ class DomainView:
     height_px: int
 
     def to_pixels(self, world_point: np.ndarray) -> tuple[float, float]:
-        x = (world_point @ self.camera.right - self.left_m) / self.pixel_m
-        y = (self.top_m - world_point @ self.camera.up) / self.pixel_m
-        return float(x), float(y)
+        """Continuous pixel coordinates (x to the right, y down) of a world point's projection."""
+        plane_x, plane_y = np.asarray(world_point, dtype=np.float64) @ np.stack([self.camera.right, self.camera.up]).T
+        return float((plane_x - self.left_m) / self.pixel_m), float((self.top_m - plane_y) / self.pixel_m)
+
+
+def _pixels_spanning(extent_m: float, pixel_m: float) -> int:
+    return math.ceil(extent_m / pixel_m - 1e-6)
 
 
 def make_domain_view(camera: Camera, domain_min_m, domain_size_m, cell_m: float, px_per_cell: float) -> DomainView:
-    """The smallest pixel grid at `px_per_cell` covering the projected domain box (render/camera.py:55-71)."""
-    low = np.array(domain_min_m)
-    size = np.array(domain_size_m)
-    corners = [low + size * np.array([a, b, c]) for a in (0, 1) for b in (0, 1) for c in (0, 1)]
-    xs = [corner @ camera.right for corner in corners]
-    ys = [corner @ camera.up for corner in corners]
+    """The smallest view at `px_per_cell` pixels per cell covering the projected domain box
+    (render/camera.py:55-71)."""
+    low = np.asarray(domain_min_m, dtype=np.float64)
+    size = np.asarray(domain_size_m, dtype=np.float64)
+    picks = np.array([[a, b, c] for a in (0.0, 1.0) for b in (0.0, 1.0) for c in (0.0, 1.0)])
+    projected = np.array([[(low + size * pick) @ axis for axis in (camera.right, camera.up)] for pick in picks])
+    lo, hi = projected.min(axis=0), projected.max(axis=0)
     pixel_m = cell_m / px_per_cell
-    return DomainView(camera=camera, left_m=min(xs), top_m=max(ys), pixel_m=pixel_m,
-                      width_px=math.ceil((max(xs) - min(xs)) / pixel_m - 1e-6),
-                      height_px=math.ceil((max(ys) - min(ys)) / pixel_m - 1e-6))
+    return DomainView(camera=camera, left_m=float(lo[0]), top_m=float(hi[1]), pixel_m=pixel_m,
+                      width_px=_pixels_spanning(hi[0] - lo[0], pixel_m),
+                      height_px=_pixels_spanning(hi[1] - lo[1], pixel_m))
 
 
 def sheet_domain_view(recipe, derived) -> DomainView:
-    """The sheet camera's domain view of a recipe (render/camera.py:74-78)."""
-    camera = make_camera(recipe.camera.elevation_deg, recipe.camera.azimuth_deg)
-    return make_domain_view(camera, derived.domain_min_m, derived.size_m, recipe.domain.cell_m,
-                            recipe.render.px_per_cell)
+    """The view `plume run` renders every frame over: the recipe's camera around the whole domain
+    (render/camera.py:74-78)."""
+    return make_domain_view(make_camera(recipe.camera.elevation_deg, recipe.camera.azimuth_deg),
+                            derived.domain_min_m, derived.size_m, recipe.domain.cell_m, recipe.render.px_per_cell)

@@ -1,10 +1,14 @@
-"""The frame loop (sim/run.py of the reference): warmup, then each frame's steps followed by a capture.
+"""The frame loop (the interface of sim/run.py:22-85): warmup, then each frame's steps followed by a capture.
 
-Time runs in blocks of the base step count; a block's start is always `base steps so far * dt`, and the CFL
-cap may split a block into more, shorter steps. Frame i is captured after warmup_s + (i + 1) / fps.
+The run is a schedule of blocks, each `base` steps of the recipe's step length long: the warmup in blocks of
+one frame's steps (the last one shorter when the warmup is not a whole number of frames), then one block per
+frame. A block's start time is the number of base steps before it times dt (never a running sum, so the clock
+does not drift). At the start of each block the CFL cap measures the fastest gas and may ask for more steps
+than `base`; the block is then split into that many equal steps over the same interval. Frame i is yielded once
+its block is done, i.e. after warmup_s + (i + 1) / fps of simulated time.
 """
 
-from collections.abc import Callable, Generator
+from collections.abc import Callable, Generator, Iterator
 from dataclasses import dataclass
 
 from plume_amd.sim.grid import Grid, make_grid
@@ -16,7 +20,8 @@ Capture = Callable[[int, Grid], None]
 
 @dataclass(frozen=True)
 class StepLog:
-    """How many steps each part of the run took, after the CFL cap (sim/run.py:22-39)."""
+    """Steps taken by the warmup and by each frame once the CFL cap has had its say, beside the recipe's base
+    counts (sim/run.py:22-39)."""
 
     warmup_steps: int
     steps_per_frame: tuple[int, ...]
@@ -29,8 +34,35 @@ class StepLog:
 
     @property
     def added_steps(self) -> int:
-        base = self.base_warmup_steps + self.base_steps_per_frame * len(self.steps_per_frame)
-        return self.total_steps - base
+        """How many steps the cap added to the recipe's own, warmup included."""
+        return self.total_steps - (self.base_warmup_steps + self.base_steps_per_frame * len(self.steps_per_frame))
+
+
+@dataclass(frozen=True)
+class _Block:
+    label: str
+    base: int  # steps of the recipe's dt the block stands for
+    frame: int | None  # the frame captured after it; None for warmup
+
+
+def _schedule(warmup_steps: int, steps_per_frame: int, frames: int) -> Iterator[_Block]:
+    whole, rest = divmod(warmup_steps, steps_per_frame)
+    for _ in range(whole):
+        yield _Block("warmup", steps_per_frame, None)
+    if rest:
+        yield _Block("warmup", rest, None)
+    for frame in range(frames):
+        yield _Block(f"frame {frame}", steps_per_frame, frame)
+
+
+def _step_times(base_before: int, base: int, steps: int, dt: float) -> list[tuple[float, float]]:
+    """(start time, length) of each step of a block that begins after `base_before` base steps. Unsplit, step
+    n starts at (base_before + n) dt exactly as a loop without the cap would; split, the block's length
+    base * dt is shared by `steps` equal steps."""
+    if steps == base:
+        return [((base_before + n) * dt, dt) for n in range(steps)]
+    start, length = base_before * dt, base * dt
+    return [(start + n * length / steps, length / steps) for n in range(steps)]
 
 
 _TUNED_GRIDS: set[tuple[int, int, int]] = set()
@@ -47,45 +79,33 @@ def tune_sweep_tiles(solve, cells) -> None:
 
 
 def simulate(recipe, derived, device: str, capture: Capture) -> StepLog:
-    """Runs the whole recipe, calling `capture(frame_index, grid)` once per frame (sim/run.py:42-50)."""
+    """The callback form of `simulate_frames`: `capture(frame_index, grid)` after each frame; returns the step
+    log (sim/run.py:42-50)."""
     frames = simulate_frames(recipe, derived, device)
-    while True:
-        try:
-            frame, grid = next(frames)
-        except StopIteration as finished:
-            return finished.value
-        capture(frame, grid)
+    try:
+        while True:
+            capture(*next(frames))
+    except StopIteration as done:
+        return done.value
 
 
 def simulate_frames(recipe, derived, device: str) -> Generator[tuple[int, Grid], None, StepLog]:
-    """Yields (frame_index, grid) after each frame's steps and returns the step log (sim/run.py:53-85)."""
+    """Yields (frame_index, live grid) after each frame's block and returns the StepLog (sim/run.py:53-85)."""
     grid = make_grid(derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k, device)
     stepper = Stepper(grid, recipe)
     tune_sweep_tiles(stepper.pressure_solve, derived.cells)
     cap = StepCap(recipe)
-    base_done = 0
-
-    def run_block(base_steps: int, label: str) -> int:
-        nonlocal base_done
-        start_s = base_done * derived.dt_s
-        length_s = base_steps * derived.dt_s
-        steps = cap.plan(grid, start_s, length_s, base_steps, label).steps
-        for index in range(steps):
-            if steps == base_steps:  # the cap did not bind: exactly the uncapped loop's times
-                stepper.step((base_done + index) * derived.dt_s, derived.dt_s)
-            else:
-                stepper.step(start_s + index * length_s / steps, length_s / steps)
-        base_done += base_steps
-        return steps
-
-    warmup_steps = 0
-    remaining = derived.warmup_steps
-    while remaining > 0:
-        block = min(remaining, derived.steps_per_frame)
-        warmup_steps += run_block(block, "warmup")
-        remaining -= block
-    frame_steps = []
-    for frame in range(recipe.time.frames):
-        frame_steps.append(run_block(derived.steps_per_frame, f"frame {frame}"))
-        yield frame, grid
-    return StepLog(warmup_steps, tuple(frame_steps), derived.steps_per_frame, derived.warmup_steps)
+    dt = derived.dt_s
+    base_before = 0
+    warmup_taken, frame_taken = 0, []
+    for block in _schedule(derived.warmup_steps, derived.steps_per_frame, recipe.time.frames):
+        steps = cap.plan(grid, base_before * dt, block.base * dt, block.base, block.label).steps
+        for time_s, length_s in _step_times(base_before, block.base, steps, dt):
+            stepper.step(time_s, length_s)
+        base_before += block.base
+        if block.frame is None:
+            warmup_taken += steps
+            continue
+        frame_taken.append(steps)
+        yield block.frame, grid
+    return StepLog(warmup_taken, tuple(frame_taken), derived.steps_per_frame, derived.warmup_steps)
