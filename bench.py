@@ -15,7 +15,12 @@ One JSON line on stdout. Besides the contract's keys it carries:
   recipe_run     the smoke-puff recipe run end to end (80 x 100 x 80 cells, 203 substeps, 64 frames,
                  sheet assembly; again in the normal layout): stage wall times beside the reference's
                  published RTX 2080 Ti ones;
-  cpu_baseline   the FP32 CPU oracle (a scalar C port of the same operators, OpenMP) on a bounded sample.
+  mixed, ensemble, adjoint
+                 the three additions at their own workloads (256 x 384 x 256 mixed-precision projection;
+                 32-member batched ensemble; 64 x 96 x 64 differentiable rollout), as --mixed, --ensemble and
+                 --adjoint print them alone (--no-additions leaves them out);
+  cpu_baseline   the C port of the same operators (the oracle's source) built with -O3 -march=native
+                 -fopenmp, on a bounded sample at all host threads and at one thread, with the CPU model.
 """
 
 import argparse
@@ -25,6 +30,7 @@ import math
 import os
 import sys
 import time
+import traceback
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
@@ -111,26 +117,55 @@ def copy_probe(mib: int, repeats: int = 20) -> float:
     return 2.0 * count * 4 / seconds / 1e9
 
 
-def cpu_baseline(gpu_grid, recipe, derived, time_s, steps, threads):
-    """The FP32 oracle (scalar C port, OpenMP) on the same warm state for `steps` substeps."""
+def cpu_facts() -> dict:
+    """The host CPU: lscpu's model name, sockets, cores per socket and threads per core, and the logical CPUs
+    this process may run on."""
+    import subprocess
+
+    facts = {"logical_cpus_available": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return facts
+    wanted = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+              "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in text.splitlines():
+        key, _, value = line.partition(":")
+        if key.strip() in wanted:
+            facts[wanted[key.strip()]] = value.strip()
+    return facts
+
+
+def cpu_baseline(gpu_grid, recipe, derived, time_s, steps_by_threads):
+    """The C port of the substep's operators (oracle/plume_oracle.c, the oracle's own source) built for this
+    host with gcc -O3 -march=native -fopenmp, FP32, on the GPU's warm state: {threads: seconds for that many
+    substeps} for each (threads, substeps) pair."""
+    import tempfile
+
     import numpy as np
 
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle import Oracle
+    from oracle.build import build_native
     from oracle.sim import Stepper, make_grid
 
-    o = Oracle("f32")
-    grid = make_grid(o, derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k)
-    for name in ("u", "v", "w", "smoke", "temperature"):
-        getattr(grid, name).data[...] = getattr(gpu_grid, name).data.cpu().numpy()
-    grid.pressure[...] = gpu_grid.pressure.cpu().numpy()
-    stepper = Stepper(grid, recipe)
-    start = time.perf_counter()
-    for index in range(steps):
-        stepper.step(time_s + index * derived.dt_s, derived.dt_s)
-    elapsed = time.perf_counter() - start
-    assert np.isfinite(grid.smoke.data).all()
-    return elapsed
+    library = build_native(Path(tempfile.mkdtemp(prefix="plume_cpu_")) / "liborc_native_f32.so")
+    o = Oracle("f32", library=library)
+    out = {}
+    for threads, steps in steps_by_threads:
+        o.set_threads(threads)
+        grid = make_grid(o, derived, recipe.domain.cell_m, recipe.domain.faces, recipe.domain.ambient_k)
+        for name in ("u", "v", "w", "smoke", "temperature"):
+            getattr(grid, name).data[...] = getattr(gpu_grid, name).data.cpu().numpy()
+        grid.pressure[...] = gpu_grid.pressure.cpu().numpy()
+        stepper = Stepper(grid, recipe)
+        start = time.perf_counter()
+        for index in range(steps):
+            stepper.step(time_s + index * derived.dt_s, derived.dt_s)
+        out[threads] = (time.perf_counter() - start, steps)
+        assert np.isfinite(grid.smoke.data).all()
+        print(f"[bench] cpu baseline: {threads} threads, {steps} substeps in {out[threads][0]:.2f} s",
+              file=sys.stderr, flush=True)
+    return out
 
 
 def frame_cache_bench(grid, repeats: int = 20) -> dict:
@@ -385,6 +420,8 @@ def main() -> int:
     parser.add_argument("--adjoint", action="store_true",
                         help="run the 64x96x64 differentiable-rollout benchmark instead and exit")
     parser.add_argument("--adjoint-repeats", type=int, default=3)
+    parser.add_argument("--no-additions", action="store_true",
+                        help="leave the mixed-precision, ensemble and adjoint sub-benchmarks out of the line")
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run the streaming-copy probe over MIB MiB (HBM counter calibration) and exit")
     args = parser.parse_args()
@@ -537,16 +574,36 @@ def main() -> int:
     if args.cpu_steps > 0:
         import dataclasses
 
-        threads = min(os.cpu_count() or 1, 16)
-        cpu_cycles = max(1, round(mean_cycles))  # the oracle runs a fixed count, as the reference does
+        # all the logical CPUs the host gives this process (OMP_NUM_THREADS when it names fewer), and one
+        threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30)
+        cpu_cycles = max(1, round(mean_cycles))  # the port runs a fixed count, as the reference does
         cpu_recipe = dataclasses.replace(recipe, solver=dataclasses.replace(recipe.solver,
                                                                             multigrid_cycles=cpu_cycles))
-        cpu_s = cpu_baseline(grid, cpu_recipe, derived, step_index * dt, args.cpu_steps, threads)
-        result["cpu_baseline"] = {"value": cells * args.cpu_steps / cpu_s, "unit": "cell-substeps/s",
-                                  "cores": threads, "kind": "port",
-                                  "sample": f"{args.cpu_steps} substeps of the same 128x192x128 scene from the "
-                                            f"GPU's warm state, FP32 C oracle with OpenMP, {cpu_cycles} V-cycles",
-                                  "seconds": cpu_s}
+        single_steps = max(2, args.cpu_steps // 8)
+        timings = cpu_baseline(grid, cpu_recipe, derived, step_index * dt,
+                               [(threads, args.cpu_steps), (1, single_steps)])
+        many_s, _ = timings[threads]
+        one_s, _ = timings[1]
+        result["cpu_baseline"] = {
+            "value": cells * args.cpu_steps / many_s, "unit": "cell-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} substeps ({single_steps} at 1 thread) of the same 128x192x128 scene from "
+                      f"the GPU's warm state: the C port of the operators (oracle/plume_oracle.c) built with gcc -O3 "
+                      f"-march=native -fopenmp, FP32, {cpu_cycles} V-cycles per substep",
+            "seconds": many_s, "single_thread": {"value": cells * single_steps / one_s, "seconds": one_s,
+                                                 "substeps": single_steps},
+            "cpu": cpu_facts(),
+            "reference_published": "Warp CPU 0.31 M cell-substeps/s on 4 cores (other hardware; context only)"}
+    if not args.no_additions:
+        for key, bench in (("mixed", mixed_bench), ("ensemble", ensemble_bench), ("adjoint", adjoint_bench)):
+            print(f"[bench] addition: {key}", file=sys.stderr, flush=True)
+            try:
+                result[key] = bench(args)
+            except Exception as error:  # the headline line stays; the failure is recorded in its place
+                traceback.print_exc()
+                result[key] = {"error": f"{type(error).__name__}: {error}"}
+        if "cell_substeps_per_s" in result["ensemble"].get("batched", {}):
+            result["ensemble"]["batched_vs_single_grid"] = result["ensemble"]["batched"]["cell_substeps_per_s"] / value
+
     print(json.dumps(result), flush=True)
     return 0
 

@@ -46,10 +46,11 @@ def _structs(real):
 class Oracle:
     """The oracle's kernels in one precision."""
 
-    def __init__(self, precision: str = "f32"):
+    def __init__(self, precision: str = "f32", library: Path | None = None):
+        """`library`: another build of the same source in this precision (the CPU baseline's native build)."""
         if precision not in LIBS:
             raise ValueError(f"precision must be one of {tuple(LIBS)}")
-        path = LIBS[precision]
+        path = LIBS[precision] if library is None else Path(library)
         if not path.is_file():
             from oracle.build import build
 
@@ -115,6 +116,10 @@ class Oracle:
                          float(normal_fade_thickness), int(shade_layout), float(shade_fill))
 
     # ------------------------------------------------------------------ noise (sim/noise.py)
+    def set_threads(self, threads: int) -> None:
+        """OpenMP threads of the following calls."""
+        self.lib.orc_set_threads(int(threads))
+
     def hash_u32(self, x: int) -> int:
         return int(self.lib.orc_hash_u32(x & 0xFFFFFFFF))
 
@@ -224,9 +229,9 @@ class Oracle:
                                self._r(smoke_keep), self._r(heat_keep), self._r(ambient_k))
 
     # ------------------------------------------------------------------ rendering (render/*.py)
-    def light_volume(self, smoke, toward_light, cell_m, opacity_per_density, light):
+    def light_volume(self, smoke, toward_light, cell_m, opacity_per_density, light, max_steps=0):
         self.lib.orc_light_volume(self._shape(smoke), self._p(smoke), self._vec(toward_light), self._r(cell_m),
-                                  self._r(opacity_per_density), self._p(light))
+                                  self._r(opacity_per_density), int(max_steps), self._p(light))
 
     def blur_axis(self, src, dst, weights, radius, axis):
         self.lib.orc_blur_axis(self._shape(src), self._p(src), self._p(dst), self._p(weights), int(radius),

@@ -16,6 +16,20 @@ FLAGS = ["-O3", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
          "-Wno-unused-function"]
 
 
+def build_native(target: Path, verbose: bool = False) -> Path:
+    """The same source as a CPU timing baseline rather than a checker: FP32, `-O3 -march=native -fopenmp`
+    with gcc's default contraction (FMA where the host has it), built for the machine it runs on. Its
+    results are not bit-identical to the reference's operation order; only its speed is used."""
+    tmp = Path(str(target) + ".tmp")
+    command = ["gcc", "-O3", "-march=native", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-Wall",
+               "-Wno-unused-function", str(SOURCE), "-o", str(tmp), "-lm"]
+    if verbose:
+        print(" ".join(command), flush=True)
+    subprocess.run(command, check=True)
+    tmp.replace(target)
+    return target
+
+
 def build(verbose: bool = False) -> dict[str, Path]:
     for precision, target in TARGETS.items():
         defines = ["-DORC_DOUBLE"] if precision == "f64" else []

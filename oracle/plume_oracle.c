@@ -12,6 +12,7 @@
  * Arrays are C-order [i][j][k], k contiguous. A 3-d field is passed as a pointer plus its shape n[3].
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -855,8 +856,10 @@ static void slab_range(const REAL *origin, const REAL *dir, const REAL *bmin, co
 }
 
 /* light_volume_kernel, render/light_volume.py:18-33 */
+/* max_steps > 0: the benchmark's fixed-count mode (no reference counterpart), that many equal steps per
+   cell over the same path instead of half-cell steps */
 void orc_light_volume(const int *n, const REAL *smoke, const REAL *toward_light, REAL cell_m,
-                      REAL opacity_per_density, REAL *light) {
+                      REAL opacity_per_density, int max_steps, REAL *light) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < n[0]; i++)
         for (int j = 0; j < n[1]; j++)
@@ -867,6 +870,7 @@ void orc_light_volume(const int *n, const REAL *smoke, const REAL *toward_light,
                 slab_range(start, toward_light, zero, box_max, span);
                 REAL length = span[1];
                 int steps = (int)CEIL(length / R(0.5));
+                if (max_steps > 0 && steps > 0) steps = max_steps;
                 REAL column = R(0.0);
                 if (steps > 0) {
                     REAL step = length / (REAL)steps;
@@ -1016,3 +1020,6 @@ void orc_raymarch(const orc_info *info, const REAL *smoke, const REAL *temperatu
             pix[4] = normal[2];
         }
 }
+
+/* OpenMP threads of the following calls (the benchmark's CPU baseline times the port at several counts) */
+void orc_set_threads(int threads) { omp_set_num_threads(threads > 0 ? threads : 1); }

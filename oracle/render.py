@@ -26,11 +26,11 @@ def gaussian_weights(sigma: float, radius: int) -> np.ndarray:
     return (weights / weights.sum()).astype(np.float32)
 
 
-def light_volume(o, smoke, toward_light_world_dir, opacity_per_density, cell_m):
+def light_volume(o, smoke, toward_light_world_dir, opacity_per_density, cell_m, max_steps=0):
     direction = np.asarray(toward_light_world_dir, dtype=np.float64)
     unit = direction / np.linalg.norm(direction)
     light = np.zeros(smoke.shape, dtype=o.dtype)
-    o.light_volume(smoke, unit, cell_m, opacity_per_density, light)
+    o.light_volume(smoke, unit, cell_m, opacity_per_density, light, max_steps)
     return light
 
 
@@ -54,13 +54,15 @@ def view_params(o, view):
     return o.view(camera.right, camera.up, camera.toward, view.left_m, view.top_m, view.pixel_m, 0, (0.0, 0.0, 0.0))
 
 
-def render_oracle(o, grid, view, settings, channels, params=None, height=None, width=None):
-    """The raw frame (height, width, 5) of grid.smoke / grid.temperature (FrameRenderer.render)."""
+def render_oracle(o, grid, view, settings, channels, params=None, height=None, width=None, max_steps=0,
+                  light_steps=0):
+    """The raw frame (height, width, 5) of grid.smoke / grid.temperature (FrameRenderer.render). max_steps and
+    light_steps > 0 are the benchmark's fixed sample counts per ray and per light path."""
     shade = channels == SHADE_LAYOUT
     smoke, temperature = grid.smoke.data, grid.temperature.data
     if shade:
         light = light_volume(o, smoke, toward_light_world(view.camera, settings.light_dir),
-                             settings.opacity_per_density, grid.info.cell_m)
+                             settings.opacity_per_density, grid.info.cell_m, light_steps)
         gradient = np.zeros((1, 1, 1, 3), dtype=o.dtype)
     else:
         gradient = normal_gradient(o, smoke, settings.normal_blur_cells)
@@ -71,5 +73,5 @@ def render_oracle(o, grid, view, settings, channels, params=None, height=None, w
     height = view.height_px if height is None else height
     width = view.width_px if width is None else width
     image = np.zeros((height, width, 5), dtype=o.dtype)
-    o.raymarch(grid.info, smoke, temperature, gradient, light, params, look, image)
+    o.raymarch(grid.info, smoke, temperature, gradient, light, params, look, image, max_steps)
     return image

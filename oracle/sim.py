@@ -223,22 +223,53 @@ def level_infos(o: Oracle, cells, cell_m, origin_m, open_lo, open_hi, max_levels
 
 
 class Multigrid:
-    """sim/multigrid.py:97-147 (the cycle runs in C, orc_multigrid)."""
+    """sim/multigrid.py:97-147 (the cycle runs in C, orc_multigrid).
 
-    def __init__(self, grid: Grid, cycles: int, sweeps: int):
+    tolerance > 0 is the build's converge-to-a-residual mode (the recipe keys multigrid_tolerance and
+    multigrid_max_cycles, which the reference lacks): cycles run one at a time, from none, while the FP64
+    relative residual |rhs - A q| / |rhs| of the top level is above `tolerance`, at most `max_cycles`. Each
+    cycle is the reference's V-cycle, so n single cycles equal one call of n cycles. The cycle counts of the
+    solves are kept in `cycles_log`."""
+
+    def __init__(self, grid: Grid, cycles: int, sweeps: int, tolerance: float = 0.0, max_cycles: int = 16):
         self.cycles = cycles
         self.sweeps = sweeps
+        self.tolerance = float(tolerance)
+        self.max_cycles = int(max_cycles)
+        self.cycles_log: list[int] = []
         self.levels = level_infos(grid.o, grid.cells, grid.cell_m, grid.origin_m, grid.open_lo, grid.open_hi)
         dt = grid.o.dtype
         self.q = [None] + [np.zeros(c, dtype=dt) for c, _ in self.levels[1:]]
         self.rhs = [None] + [np.zeros(c, dtype=dt) for c, _ in self.levels[1:]]
         self.residual = [np.zeros(c, dtype=dt) for c, _ in self.levels]
 
+    def _run(self, grid: Grid, cycles: int) -> None:
+        grid.o.multigrid([info for _, info in self.levels], self.q, self.rhs, self.residual, cycles, self.sweeps)
+
     def __call__(self, grid: Grid) -> None:
         self.q[0] = grid.pressure
         self.rhs[0] = grid.divergence
-        grid.o.multigrid([info for _, info in self.levels], self.q, self.rhs, self.residual, self.cycles,
-                         self.sweeps)
+        if self.tolerance <= 0.0:
+            self._run(grid, self.cycles)
+            return
+        ran = 0
+        while ran < self.max_cycles and relative_residual(grid) > self.tolerance:
+            self._run(grid, 1)
+            ran += 1
+        self.cycles_log.append(ran)
+
+
+def relative_residual(grid: Grid) -> float:
+    """|rhs - A q| / |rhs| of the top level in FP64 (the residual itself in the oracle's precision); a zero
+    rhs gives 0 when q solves it and inf otherwise."""
+    r = np.zeros_like(grid.pressure)
+    grid.o.residual(grid.info, grid.pressure, grid.divergence, r)
+    r2 = float(np.dot(r.ravel().astype(np.float64), r.ravel().astype(np.float64)))
+    b = grid.divergence.ravel().astype(np.float64)
+    b2 = float(np.dot(b, b))
+    if b2 > 0.0:
+        return math.sqrt(r2) / math.sqrt(b2)
+    return math.inf if r2 > 0.0 else 0.0
 
 
 def poisson_residual(grid: Grid, q=None, rhs=None) -> float:
@@ -254,7 +285,8 @@ def make_pressure_solve(grid: Grid, solver):
     """sim/pressure_solver.py:9-13."""
     if solver.pressure_solver == "gauss_seidel":
         return GaussSeidel(solver.pressure_iterations)
-    return Multigrid(grid, solver.multigrid_cycles, solver.multigrid_sweeps)
+    return Multigrid(grid, solver.multigrid_cycles, solver.multigrid_sweeps,
+                     getattr(solver, "multigrid_tolerance", 0.0), getattr(solver, "multigrid_max_cycles", 16))
 
 
 # ---------------------------------------------------------------------- budget

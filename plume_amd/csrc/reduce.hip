@@ -91,10 +91,13 @@ __global__ __launch_bounds__(256) void k_scale(int64_t count, float *__restrict_
 
 __global__ void k_clear_u32(unsigned *p) { *p = 0u; }
 
-// max |f| over the three face arrays; |f| >= 0, so its float bits order like unsigned integers
+// max |f| over the three face arrays. |f| has its sign bit clear, so its bits order like unsigned integers,
+// with +inf above every finite value and every NaN above +inf: the maximum is taken on the bits, so a NaN
+// anywhere comes out as NaN (as NumPy's max propagates it, sim/step_cap.py:50-55) instead of being dropped
+// the way fmaxf drops it.
 __global__ __launch_bounds__(256) void k_max_abs3(Tri<const float> f, int64_t nu, int64_t nv, int64_t nw,
                                                   unsigned *__restrict__ out) {
-    float best = 0.0f;
+    unsigned best = 0u;
     const int64_t total = nu + nv + nw;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total; c += (int64_t)gridDim.x * blockDim.x) {
         float x;
@@ -104,16 +107,16 @@ __global__ __launch_bounds__(256) void k_max_abs3(Tri<const float> f, int64_t nu
             x = f.p[1][c - nu];
         else
             x = f.p[2][c - nu - nv];
-        best = fmaxf(best, fabsf(x));
+        best = max(best, __float_as_uint(x) & 0x7fffffffu);
     }
-    __shared__ float s[256];
+    __shared__ unsigned s[256];
     s[threadIdx.x] = best;
     __syncthreads();
     for (int half = 128; half > 0; half /= 2) {
-        if ((int)threadIdx.x < half) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + half]);
+        if ((int)threadIdx.x < half) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + half]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));
+    if (threadIdx.x == 0) atomicMax(out, s[0]);
 }
 
 // The scratch of plm_smoke_budget_noalloc: the four FP64 totals, then the line sums of up to

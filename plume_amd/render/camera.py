@@ -86,6 +86,21 @@ def make_domain_view(camera: Camera, domain_min_m, domain_size_m, cell_m: float,
                       height_px=_pixels_spanning(hi[1] - lo[1], pixel_m))
 
 
+def square_domain_view(camera: Camera, domain_min_m, domain_size_m, pixels: int) -> DomainView:
+    """A `pixels` x `pixels` view centred on the projected domain box, with the pixel pitch that fits the box's
+    longer projected side (no reference counterpart: the benchmark's fixed 128² and 256² images)."""
+    low = np.asarray(domain_min_m, dtype=np.float64)
+    size = np.asarray(domain_size_m, dtype=np.float64)
+    picks = np.array([[a, b, c] for a in (0.0, 1.0) for b in (0.0, 1.0) for c in (0.0, 1.0)])
+    projected = np.array([[(low + size * pick) @ axis for axis in (camera.right, camera.up)] for pick in picks])
+    lo, hi = projected.min(axis=0), projected.max(axis=0)
+    span = float(max(hi - lo))
+    pixel_m = span / pixels
+    center = (lo + hi) / 2.0
+    return DomainView(camera=camera, left_m=float(center[0] - span / 2.0), top_m=float(center[1] + span / 2.0),
+                      pixel_m=pixel_m, width_px=pixels, height_px=pixels)
+
+
 def sheet_domain_view(recipe, derived) -> DomainView:
     """The view `plume run` renders every frame over: the recipe's camera around the whole domain
     (render/camera.py:74-78)."""

@@ -12,13 +12,18 @@ reach the GPU:
   (coarse multigrid levels, reductions) overlap with each other and with the large members' bandwidth-bound
   ones;
 - the pressure solves run their tolerance mode with a fixed number of device-gated V-cycles (the most any
-  member needed in the eager frames, plus one), so nothing in the graph reads back to the host.
+  member needed in the eager frames, plus one), so nothing in the graph reads back to the host. Each solve in
+  the graph adds 1 to its member's device counter when it ends with the residual still above tolerance; the
+  frame loop reads the counters once per replayed frame (beside the CFL plan's read). A frame in which any
+  solve stayed open is undone from a snapshot of the fields taken at its start and run again eagerly, where
+  every solve continues to its target, and the graph for that step length is captured again with more cycles.
+  A replayed frame therefore equals the eager frame bit for bit.
 
 Two substeps are captured because AdvectedField.swap exchanges buffers by reference: after two substeps every
 field is back in the buffer the graph was captured with. The substep count of a frame is therefore common to
 all members (the largest any member's CFL cap asks for) and rounded up to an even number; more, shorter
-substeps keep every member within its cap. Frames in which an emitter is active (their launches depend on the
-time) and frames whose step length has no graph yet run eagerly, member by member.
+substeps keep every member within its cap. Frames in which an emitter is active or any member has turbulence
+(their launches depend on the time, which a replay cannot change) run eagerly, member by member.
 """
 
 from dataclasses import dataclass
@@ -85,6 +90,10 @@ class Ensemble:
         self.frames_done = 0
         self.extra_cycles = 1  # device-gated V-cycles captured beyond the most any eager solve needed
         self.log: list[tuple[int, int, bool]] = []  # (frame, substeps, replayed)
+        self.redone: list[int] = []  # frames replayed with an open gate, then run again eagerly
+        # per member: solves in replayed graphs that ended above tolerance (device, cleared each frame)
+        self.open_solves = torch.zeros(len(self.members), dtype=torch.int32, device=device)
+        self._snapshot: list[list[torch.Tensor]] | None = None
 
     @property
     def cells(self) -> int:
@@ -110,6 +119,30 @@ class Ensemble:
         return any(np.float32(p.start_s) < np.float32(end_s) and np.float32(p.end_s) > np.float32(start_s)
                    for m in self.members for p in m.stepper.emitters.params)
 
+    @property
+    def turbulent(self) -> bool:
+        """Some member's turbulence is on: its noise pattern moves with the time (sim/turbulence.py:59-67),
+        which a replayed graph would freeze at the capture's time."""
+        return any(m.recipe.forces.turbulence.strength_mps2 != 0.0 for m in self.members)
+
+    def _state(self, m: Member) -> list[torch.Tensor]:
+        """What a member carries from one frame to the next: the fields' data and the pressure (the warm
+        start). Everything else is rewritten every substep."""
+        return [f.data for f in m.grid.velocity_fields()] + [m.grid.smoke.data, m.grid.temperature.data,
+                                                              m.grid.pressure]
+
+    def _take_snapshot(self) -> None:
+        if self._snapshot is None:
+            self._snapshot = [[torch.empty_like(t) for t in self._state(m)] for m in self.members]
+        for m, saved in zip(self.members, self._snapshot):
+            for dst, src in zip(saved, self._state(m)):
+                dst.copy_(src)
+
+    def _restore_snapshot(self) -> None:
+        for m, saved in zip(self.members, self._snapshot):
+            for dst, src in zip(self._state(m), saved):
+                dst.copy_(src)
+
     def step_eager(self, time_s: float, dt: float) -> None:
         for m in self.members:
             m.stepper.step(time_s, dt)
@@ -120,6 +153,8 @@ class Ensemble:
         solve's cycle count and allocates every scratch array."""
         if self.emitting(time_s, time_s + 2.0 * dt):
             raise ValueError("an emitter is active: its launches depend on the time and cannot be replayed")
+        if self.turbulent:
+            raise ValueError("turbulence is on: its pattern depends on the time and cannot be replayed")
         solves = []
         for m in self.members:
             solve = m.stepper.pressure_solve
@@ -135,8 +170,12 @@ class Ensemble:
                     s.wait_stream(main)
                 for index, m in enumerate(self.members):
                     with torch.cuda.stream(self.streams[index % len(self.streams)]):
-                        m.stepper.step(time_s, dt)
-                        m.stepper.step(time_s + dt, dt)
+                        for substep in range(2):
+                            m.stepper.step(time_s + substep * dt, dt)
+                            solve = m.stepper.pressure_solve
+                            if solve in solves:  # its gate: 1 while above tolerance (plm_vcycle)
+                                count = self.open_solves[index:index + 1]
+                                count.add_(solve.norm_scratch[1:2].ne(0.0).to(torch.int32))
                 for s in self.streams:
                     main.wait_stream(s)
         finally:
@@ -150,13 +189,24 @@ class Ensemble:
         start_s = self.frames_done * self.frame_s
         steps = self.frame_steps(start_s)
         dt = self.frame_s / steps
-        quiet = not self.emitting(start_s, start_s + self.frame_s)
+        quiet = not self.emitting(start_s, start_s + self.frame_s) and not self.turbulent
         replayed = False
         if replay and quiet and self.frames_done > 0:
             graph = self.graphs.get(dt) or self.capture(start_s, dt, self.extra_cycles)
+            self._take_snapshot()
+            self.open_solves.zero_()
             for _ in range(steps // 2):
                 graph.replay()
             replayed = True
+            if any(self.open_solves.tolist()):
+                # some solve ended above tolerance: undo the frame, run it eagerly (each solve continues to its
+                # target) and give the next capture the cycle counts the eager solves needed
+                self._restore_snapshot()
+                for index in range(steps):
+                    self.step_eager(start_s + index * dt, dt)
+                self.graphs.pop(dt, None)
+                self.redone.append(self.frames_done)
+                replayed = False
         else:
             for index in range(steps):
                 self.step_eager(start_s + index * dt, dt)

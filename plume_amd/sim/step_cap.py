@@ -35,7 +35,11 @@ class StepCap:
     def plan(self, grid, start_s: float, length_s: float, base_steps: int, label: str) -> StepPlan:
         """Steps for the block from `start_s` lasting `length_s`, at least `base_steps`
         (sim/step_cap.py:38-48)."""
-        speed = max(self.grid_speed(grid), self.emitter_speed(start_s, start_s + length_s))
+        grid_speed = self.grid_speed(grid)
+        if not math.isfinite(grid_speed):
+            # the reference's math.ceil of a NaN or infinite speed raises here too (sim/step_cap.py:38-48)
+            raise ValueError(f"{label}: the fastest face velocity is {grid_speed}: the simulation has blown up")
+        speed = max(grid_speed, self.emitter_speed(start_s, start_s + length_s))
         needed = math.ceil(speed * length_s / (self.max_cells * self.cell_m))
         steps = max(base_steps, needed)
         if steps > max(self.max_substeps, base_steps):

@@ -13,6 +13,7 @@ sys.path.insert(0, str(Path(__file__).resolve().parent))
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU (MI355X) and libplume_hip.so")
     config.addinivalue_line("markers", "slow: long-running")
+    config.addinivalue_line("markers", "config: parity at a benchmark configuration's full size (GPU, minutes)")
 
 
 @pytest.fixture(scope="session")

@@ -416,6 +416,33 @@ def test_multigrid_with_odd_sweep_counts_is_bit_exact(device, oracle32, sweeps):
     assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
+@pytest.mark.parametrize("sweeps", [2, 3])
+@pytest.mark.parametrize("faces", [DEFAULT_FACES, ALL_CLOSED, ALL_OPEN], ids=["default", "closed", "open"])
+@pytest.mark.parametrize("cells", [(45, 38, 70), (90, 90, 127), (70, 66, 45)], ids=["odd", "k-full", "wide"])
+def test_projection_with_mid_levels_is_bit_exact(device, oracle32, cells, faces, sweeps):
+    """Grids with levels between the simulation grid and the LDS tail (their sweeps are tile launches with the
+    coupling table, an odd count ending in a copy back): the whole projection equals the oracle's bit for bit on
+    odd sizes, overhanging coarse cells, k extents that do not fill a 64-wide tile, and every face kind."""
+    from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
+    from plume_amd.sim.multigrid import Multigrid, PLM_LDS_CELLS, build_levels
+    from plume_amd.sim.pressure import project
+
+    gpu, orc = jet(cells, faces, device, oracle32)
+    levels = build_levels(gpu)
+    tail = 0
+    mid = []
+    for index in reversed(range(len(levels))):
+        tail += int(np.prod(levels[index].cells))
+        if tail > PLM_LDS_CELLS and index > 0:
+            mid.append(levels[index].cells)
+    assert mid, "the grid must have a level between the simulation grid and the tail"
+    project(gpu, Multigrid(gpu, 2, sweeps))
+    oracle_project(orc, OracleMultigrid(orc, 2, sweeps))
+    assert bit_equal(to_np(gpu.pressure), orc.pressure), mid
+    for name in ("u", "v", "w"):
+        assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
+
+
 def test_timed_sweeps_are_bit_exact_and_report_each_launch(device, oracle32):
     """While a launch family is timed, its sweeps launch with the event pair bound to the dispatch
     (hipExtLaunchKernelGGL). The solve must stay bit-identical to the oracle, and the events must give one
@@ -505,6 +532,24 @@ def test_grid_speed_is_the_exact_maximum(device, oracle32):
     gpu.w.data[3, 4, 5] = -7.25
     expected = max(float(np.abs(f.data).max()) for f in orc.velocity_fields())
     assert StepCap(tiny_recipe()).grid_speed(gpu) == max(expected, 7.25)
+
+
+def test_a_nan_velocity_stops_the_step_cap(device, oracle32):
+    # NumPy's max propagates a NaN into math.ceil, which raises (sim/step_cap.py:38-55): the device maximum
+    # must not drop it
+    import math
+
+    from plume_amd.sim.step_cap import StepCap
+    from support import tiny_recipe
+
+    gpu, _ = pair(device, oracle32, (31, 33, 29), DEFAULT_FACES)
+    gpu.v.data[30, 1, 2] = float("nan")
+    cap = StepCap(tiny_recipe())
+    assert math.isnan(cap.grid_speed(gpu))
+    with pytest.raises(ValueError, match="blown up"):
+        cap.plan(gpu, 0.0, 0.1, 3, "frame 0")
+    gpu.v.data[30, 1, 2] = float("-inf")
+    assert cap.grid_speed(gpu) == math.inf
 
 
 # ---------------------------------------------------------------------- rendering

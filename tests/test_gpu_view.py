@@ -114,3 +114,122 @@ def test_the_puff_run_notes_the_ceiling_once_the_smoke_reaches_it(device):
     for note in notes:
         assert all(name in FACE_NAMES and (name == "top" or getattr(faces, name) == "open") for name in note)
     assert math.isfinite(watch.layers.abs().max().item())
+
+
+# ---------------------------------------------------------------------- the viewer's perspective march
+
+PERSPECTIVE_CELLS = (16, 16, 16)
+PERSPECTIVE_CELL_M = 0.1
+
+
+def perspective_scene(device, oracle32, smoke, temperature=None):
+    from support import ALL_CLOSED, gpu_grid, oracle_grid
+
+    gpu = gpu_grid(PERSPECTIVE_CELLS, PERSPECTIVE_CELL_M, ALL_CLOSED, device=device)
+    orc = oracle_grid(oracle32, PERSPECTIVE_CELLS, PERSPECTIVE_CELL_M, ALL_CLOSED)
+    fields = {"smoke": smoke} if temperature is None else {"smoke": smoke, "temperature": temperature}
+    for name, values in fields.items():
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        getattr(gpu, name).data.copy_(torch.from_numpy(values))
+        getattr(orc, name).data[...] = values
+    return gpu, orc
+
+
+def blob16(center, radius):
+    idx = np.indices(PERSPECTIVE_CELLS).astype(np.float64) + 0.5
+    return np.exp(-sum((idx[a] - center[a]) ** 2 for a in range(3)) / (2 * radius**2))
+
+
+def perspective_params(view, distance_m):
+    """A perspective camera `distance_m` out along the view's axis whose plane at the world origin matches the
+    domain view's pixels (plane coordinates scale by 1 / distance), as the viewer sets it up
+    (view/viewport.py:152,225)."""
+    from plume_amd._lib import VIEW_PERSPECTIVE, float3
+    from plume_amd.render.frame_renderer import domain_view_params
+
+    camera = view.camera
+    center_x = view.left_m + view.width_px * view.pixel_m / 2
+    center_y = view.top_m - view.height_px * view.pixel_m / 2
+    params = domain_view_params(view)
+    params.perspective = VIEW_PERSPECTIVE
+    params.eye = float3(camera.right * center_x + camera.up * center_y + camera.toward * distance_m)
+    params.left_m = (view.left_m - center_x) / distance_m
+    params.top_m = (view.top_m - center_y) / distance_m
+    params.pixel_m = view.pixel_m / distance_m
+    return params
+
+
+def gpu_march(gpu, renderer, params, height, width, device):
+    image = torch.zeros((height, width, 5), dtype=torch.float32, device=device)
+    renderer.prepare(gpu.smoke.data, gpu.info.cell_m)
+    renderer.march(gpu.smoke.data, gpu.temperature.data, gpu.info, params, image)
+    torch.cuda.synchronize()
+    return image.cpu().numpy()
+
+
+@pytest.mark.parametrize("layout", ["shade", "normal"])
+@pytest.mark.parametrize("distance_m, rel", [(3.0, 2e-5), (200.0, 3e-4)])
+def test_the_perspective_march_matches_the_oracle(device, oracle32, layout, distance_m, rel):
+    """plm_raymarch with perspective = 1 (render/raymarch.py:72-83: rays from the eye through the 1 m plane)
+    against the oracle's march with the same view: within 2e-5 of each channel's scale with the eye 3 m out;
+    with the eye 200 m out, sample positions carry FP32 rounding of 1.5e-5 m and this dense, hot blob turns
+    the last-bit differences of exp and pow into more: the FP32 oracle itself differs from the FP64 oracle by
+    8.4e-5 of the heat scale there, so the bar is 3e-4 (the reference's CPU-against-GPU image bar is 1-2%,
+    tests/cli/test_cpu_gpu.py:21-33)."""
+    from oracle.render import render_oracle
+    from plume_amd.recipe.schema import NORMAL_LAYOUT, SHADE_LAYOUT, RenderSettings
+    from plume_amd.render.camera import make_camera, make_domain_view
+    from plume_amd.render.frame_renderer import FrameRenderer
+
+    gpu, orc = perspective_scene(device, oracle32, 3.0 * blob16((8.0, 7.0, 9.0), 3.0),
+                                 293.0 + 900.0 * blob16((8.0, 7.0, 9.0), 2.0))
+    channels = SHADE_LAYOUT if layout == "shade" else NORMAL_LAYOUT
+    view = make_domain_view(make_camera(25.0, 40.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), PERSPECTIVE_CELL_M, 2.0)
+    settings = RenderSettings()
+    renderer = FrameRenderer(view, settings, PERSPECTIVE_CELLS, channels, device)
+    params = perspective_params(view, distance_m)
+    image = gpu_march(gpu, renderer, params, view.height_px, view.width_px, device)
+    expected = render_oracle(oracle32, orc, view, settings, channels, params=params)
+    for channel in range(5):
+        scale = max(float(np.abs(expected[..., channel]).max()), 1e-30)
+        worst = float(np.abs(image[..., channel] - expected[..., channel]).max())
+        assert worst <= rel * scale + 1e-6, (channel, worst, scale)
+    assert expected[..., 1].max() > 0.5
+
+
+def test_a_far_narrow_perspective_camera_matches_the_orthographic_march(device, oracle32):
+    """tests/render/test_view_cameras.py:42-56 on the HIP path: at 200 m the perspective image is within 0.01
+    of the orthographic one (thickness, shade) and within 1% of the peak heat."""
+    from plume_amd.recipe.schema import SHADE_LAYOUT, RenderSettings
+    from plume_amd.render.camera import make_camera, make_domain_view
+    from plume_amd.render.frame_renderer import FrameRenderer
+
+    gpu, _ = perspective_scene(device, oracle32, 3.0 * blob16((8.0, 7.0, 9.0), 3.0),
+                               293.0 + 900.0 * blob16((8.0, 7.0, 9.0), 2.0))
+    view = make_domain_view(make_camera(25.0, 40.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), PERSPECTIVE_CELL_M, 2.0)
+    renderer = FrameRenderer(view, RenderSettings(), PERSPECTIVE_CELLS, SHADE_LAYOUT, device)
+    ortho = gpu_march(gpu, renderer, renderer.view_params, view.height_px, view.width_px, device)
+    persp = gpu_march(gpu, renderer, perspective_params(view, 200.0), view.height_px, view.width_px, device)
+    assert np.abs(persp[..., 1] - ortho[..., 1]).max() < 0.01
+    assert np.abs(persp[..., 2] - ortho[..., 2]).max() < 0.01
+    assert np.abs(persp[..., 0] - ortho[..., 0]).max() < 0.01 * ortho[..., 0].max()
+    assert ortho[..., 1].max() > 0.9
+
+
+def test_an_eye_inside_the_smoke_sees_only_the_smoke_in_front_of_it(device, oracle32):
+    """tests/render/test_view_cameras.py:59-69 on the HIP path: the eye 1.1 m from the back wall inside uniform
+    smoke of density 0.5 sees thickness 1 - exp(-2 * 0.5 * 1.1)."""
+    from plume_amd._lib import VIEW_PERSPECTIVE, float3
+    from plume_amd.recipe.schema import SHADE_LAYOUT, RenderSettings
+    from plume_amd.render.camera import make_camera, make_domain_view
+    from plume_amd.render.frame_renderer import FrameRenderer, domain_view_params
+
+    gpu, _ = perspective_scene(device, oracle32, np.full(PERSPECTIVE_CELLS, 0.5))
+    view = make_domain_view(make_camera(0.0, 0.0), (-0.8, 0.0, -0.8), (1.6, 1.6, 1.6), PERSPECTIVE_CELL_M, 2.0)
+    renderer = FrameRenderer(view, RenderSettings(opacity_per_density=2.0), PERSPECTIVE_CELLS, SHADE_LAYOUT, device)
+    params = domain_view_params(view)
+    params.perspective = VIEW_PERSPECTIVE
+    params.eye = float3((0.0, 0.8, 0.3))  # 1.1 m from the back wall (z = -0.8), looking along -z
+    params.left_m, params.top_m, params.pixel_m = -0.5, 0.5, 1.0 / 9.0
+    image = gpu_march(gpu, renderer, params, 9, 9, device)
+    assert image[4, 4, 1] == pytest.approx(1.0 - math.exp(-2.0 * 0.5 * 1.1), rel=1e-4)

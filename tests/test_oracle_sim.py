@@ -641,6 +641,42 @@ def test_multigrid_matches_the_dense_solution_in_fp64(oracle64):
     assert np.allclose(grid.pressure.ravel(), exact, rtol=1e-8, atol=1e-12)
 
 
+@pytest.mark.parametrize("faces", [ALL_CLOSED, DEFAULT_FACES], ids=["closed", "open"])
+def test_tolerance_mode_runs_the_fewest_cycles_that_reach_the_target(oracle32, faces):
+    # the converge-to-a-residual mode: the residual after the solve is at or below the target, one cycle
+    # fewer is above it, and the cycles are the plain mode's (n single cycles equal one call of n)
+    from oracle.sim import relative_residual
+
+    cells = (15, 17, 13)
+    grid = jet_grid(oracle32, cells, faces)
+    project(grid, GaussSeidel(0))
+    compute_divergence(grid)
+    start = grid.pressure.copy()
+    solve = Multigrid(grid, 2, 2, tolerance=1e-4, max_cycles=30)
+    solve(grid)
+    ran = solve.cycles_log[-1]
+    assert 2 <= ran < 30
+    assert relative_residual(grid) <= 1e-4
+    grid.pressure[...] = start
+    Multigrid(grid, ran - 1, 2)(grid)
+    assert relative_residual(grid) > 1e-4
+    grid.pressure[...] = start
+    Multigrid(grid, 1, 2)(grid)
+    for _ in range(ran - 1):
+        Multigrid(grid, 1, 2)(grid)
+    after_singles = grid.pressure.copy()
+    grid.pressure[...] = start
+    Multigrid(grid, ran, 2)(grid)
+    assert np.array_equal(after_singles.view(np.uint32), grid.pressure.view(np.uint32))
+    # a solve that starts converged runs none; the cap binds
+    solve(grid)
+    assert solve.cycles_log[-1] == 0
+    grid.pressure[...] = start
+    capped = Multigrid(grid, 2, 2, tolerance=1e-12, max_cycles=3)
+    capped(grid)
+    assert capped.cycles_log == [3]
+
+
 def test_prolongation_reproduces_affine_fields(oracle64):
     # trilinear prolongation of an affine coarse field is exact where no clamp acts
     coarse = np.indices((5, 6, 4)).astype(np.float64)
