@@ -1166,6 +1166,8 @@ constexpr int WAVE_CELLS = 128;  // levels this small run on one wavefront, whos
 // cells per colour per thread on block levels, with and without stored couplings (a level with more colour
 // slots than that times TAIL_THREADS stays out of the tail)
 constexpr int BLOCK_PLAN = 2, BLOCK_PLAN_ONES = 3;
+// dynamic LDS the tail may take (gfx950: 160 KB per CU, less the level table staged beside it)
+constexpr size_t TAIL_LDS_BYTES = 150 * 1024;
 
 // Division by a level extent through a multiply-high: m = ceil(2^32 / d) gives floor(t / d) exactly for
 // t, d < 2^16 (the error t (m d - 2^32) / (d 2^32) stays below 1 / d).
@@ -1184,21 +1186,20 @@ FastDiv fast_div(int d) {
     return f;
 }
 
+// Each tail level lives in LDS zero-bordered: its q, rhs and (for the level being restricted) residual as
+// (n0 + 2) (n1 + 2) (n2 + 2) arrays, cell (i, j, k) at ((i + 1)(n1 + 2) + j + 1)(n2 + 2) + k + 1. The border of
+// q stays +0.0, the q a pass reads for a neighbour outside the level (a ghost or a closed face, whose term the
+// reference adds as 0), so a neighbour is one address add away and no mask or select is needed.
 struct TailLevels {
     int count;      // levels in the tail
     int zero_top;   // the top level's q starts at 0 (a coarse level the V-cycle just cleared)
     int wave_from;  // first level (within the tail) from which every level has <= WAVE_CELLS cells; count if none
     LevelArgs level[MAX_TAIL];
-    int res_offset;  // float offset in LDS of the residual of the level being restricted
-    int offset[MAX_TAIL];  // float offset of the level's q in LDS; its rhs follows at offset + cells
-    int cells[MAX_TAIL];
+    int qoff[MAX_TAIL];  // float offset in LDS of the level's padded q; its padded rhs follows at roff
+    int roff[MAX_TAIL];
+    int res_offset;  // the padded residual of the level being restricted
+    int q_floats;    // floats from qoff[0] through the last q and rhs: cleared at the start
     FastDiv div_n1[MAX_TAIL], div_n2[MAX_TAIL], div_half[MAX_TAIL];  // by n1, n2 and (n2 + 1) / 2
-};
-
-struct LdsQ {
-    const float *q;
-    Dims n;
-    __device__ float operator()(int i, int j, int k) const { return q[n.at(i, j, k)]; }
 };
 
 struct BlockSync {
@@ -1229,7 +1230,6 @@ struct TailLevel {
     // uniform values costs a move and a select per value)
     const Coupling *cp;
 };
-
 
 // A level's extents alone, for the other level of a restrict or prolong pass
 struct TailDims {
@@ -1270,22 +1270,40 @@ __device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
     return index == L.n[a] - 1 ? L.fill[a] : 1.0f;
 }
 
-// LDS word 0 holds +0.0: the q a pass reads for a neighbour outside the level (a ghost or a closed face,
-// whose term the reference adds as 0).
-constexpr int LDS_ZERO = 0;
-constexpr int LDS_HEAD = 64;  // floats before the first level
+// the padded index of cell (i, j, k) of a level of extents n
+__device__ __forceinline__ int pad_at(const int (&n)[3], int i, int j, int k) {
+    return ((i + 1) * (n[1] + 2) + j + 1) * (n[2] + 2) + k + 1;
+}
 
 // What one tail pass needs of one cell, fixed for a visit of its level (neighbor_sum's couplings and their
-// sum, sim/pressure.py:52-77; cell_volume * cell_m^2, :47-49 and :86-92): `packed` = cell index | bit 16 + s
-// set when neighbour s (x-, x+, y-, y+, z-, z+) is inside the level, or -1 for no cell. ONES: every fill_hi
-// of the level is 1, so every in-domain coupling is area / (0.5 (1 + 1)) = 1 exactly and only t1 (through
-// the open-face ghosts) differs from the simulation grid's; the couplings are then not stored.
+// sum, sim/pressure.py:52-77; cell_volume * cell_m^2, :47-49 and :86-92): p = the cell's padded index, or -1 for
+// no cell. ONES: every fill_hi of the level is 1, so every in-domain coupling is area / (0.5 (1 + 1)) = 1
+// exactly, every volume is 1 and the scale is (1 cell_m) cell_m for all cells; only t1 (through the open-face
+// ghosts) differs from the simulation grid's, and the couplings and scale are not stored. An outside
+// neighbour's coupling multiplies the border's +0.0, adding +0.0 as the reference adds 0 for it.
 template <bool ONES>
-struct PlanCell {
-    int packed;
-    float t1, scale;
-    float cp[ONES ? 1 : 6];
+struct PlanCell;
+template <>
+struct PlanCell<true> {
+    int p;
+    float t1;
 };
+template <>
+struct PlanCell<false> {
+    int p;
+    float t1;
+    float scale;
+    float cp[6];
+};
+
+// the scale (volume cm^2) of a planned cell: uniform on a ONES level
+template <bool ONES>
+__device__ __forceinline__ float plan_scale(const PlanCell<ONES> &p, float uniform_scale) {
+    if constexpr (ONES)
+        return uniform_scale;
+    else
+        return p.scale;
+}
 
 // Builds a cell's plan from the level's geometry and coupling table, with the reference's operations: the
 // area (1 F_b) F_c, the in-domain coupling (the area itself unless the pair holds the axis' last cell, then
@@ -1295,7 +1313,6 @@ __device__ __forceinline__ PlanCell<ONES> plan_cell(const TailLevel &L, int i, i
     PlanCell<ONES> p;
     const int cell[3] = {i, j, k};
     const bool last[3] = {i == L.n[0] - 1, j == L.n[1] - 1, k == L.n[2] - 1};
-    int mask = 0;
     float t1 = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -1308,24 +1325,24 @@ __device__ __forceinline__ PlanCell<ONES> plan_cell(const TailLevel &L, int i, i
             in1 = L.cp->in[a][area_index][1];
         }
         const bool in_lo = cell[a] > 0, in_hi = !last[a];
-        mask |= (in_lo ? 1 : 0) << (2 * a);
-        mask |= (in_hi ? 1 : 0) << (2 * a + 1);
         const float cp_lo = ONES ? 1.0f : (last[a] ? in1 : area);
         const float cp_hi = ONES ? 1.0f : (cell[a] + 1 == L.n[a] - 1 ? in1 : area);
         const float ghost_lo = L.cp->ghost[a][area_index][last[a] ? 1 : 0];
         const float ghost_hi = L.cp->ghost[a][area_index][1];
         t1 = t1 + (in_lo ? cp_lo : L.open_lo[a] == 1 ? ghost_lo : 0.0f);
         t1 = t1 + (in_hi ? cp_hi : L.open_hi[a] == 1 ? ghost_hi : 0.0f);
-        if (!ONES) {
+        if constexpr (!ONES) {
             p.cp[2 * a] = cp_lo;
             p.cp[2 * a + 1] = cp_hi;
         }
     }
-    float vol = fill_at(L, 0, i) * fill_at(L, 1, j);
-    vol = vol * fill_at(L, 2, k);
-    p.scale = vol * L.cell_m * L.cell_m;
+    if constexpr (!ONES) {
+        float vol = fill_at(L, 0, i) * fill_at(L, 1, j);
+        vol = vol * fill_at(L, 2, k);
+        p.scale = vol * L.cell_m * L.cell_m;
+    }
     p.t1 = t1;
-    p.packed = ((i * L.n[1] + j) * L.n[2] + k) | (mask << 16);
+    p.p = pad_at(L.n, i, j, k);
     return p;
 }
 
@@ -1345,39 +1362,42 @@ struct ColourPlan {
             if (x < total && k < L.n[2]) {
                 cell[u] = plan_cell<ONES>(L, i, j, k);
             } else {
-                cell[u].packed = -1;
+                cell[u].p = -1;
                 cell[u].t1 = 0.0f;
-                cell[u].scale = 1.0f;
-                if (!ONES)
+                if constexpr (!ONES) {
+                    cell[u].scale = 1.0f;
                     for (int s = 0; s < 6; s++) cell[u].cp[s] = 1.0f;
+                }
             }
         }
     }
 };
 
-// t0 of neighbor_sum for a planned cell: the six terms added in the reference's order; a neighbour outside
-// reads the zero word, so its term is +0.0 as the reference's is.
+// A level's six neighbour strides in its padded layout (x-, x+, y-, y+, z-, z+) and the uniform scale of a
+// ONES level
+struct LevelStrides {
+    int s[6];
+    float scale;
+};
+
+__device__ __forceinline__ LevelStrides level_strides(const TailLevel &L) {
+    const int row = L.n[2] + 2, plane = (L.n[1] + 2) * row;
+    return LevelStrides{{-plane, plane, -row, row, -1, 1}, (1.0f * L.cell_m) * L.cell_m};
+}
+
+// t0 of neighbor_sum for a planned cell: the six terms added in the reference's order
 template <bool ONES>
-__device__ __forceinline__ float plan_t0(const PlanCell<ONES> &p, const float *q, const int (&stride)[6],
-                                         int zero) {
-    const int c = p.packed & 0xffff, mask = p.packed >> 16;
+__device__ __forceinline__ float plan_t0(const PlanCell<ONES> &p, const float *q, const int (&stride)[6]) {
     float t0 = 0.0f;
 #pragma unroll
     for (int s = 0; s < 6; s++) {
-        const float qn = q[(mask >> s) & 1 ? c + stride[s] : zero];
-        t0 = t0 + (ONES ? qn : p.cp[s] * qn);
+        const float qn = q[p.p + stride[s]];
+        if constexpr (ONES)
+            t0 = t0 + qn;
+        else
+            t0 = t0 + p.cp[s] * qn;
     }
     return t0;
-}
-
-struct LevelStrides {
-    int s[6];
-    int zero;  // LDS_ZERO relative to the level's q
-};
-
-__device__ __forceinline__ LevelStrides level_strides(const TailLevel &L, int offset) {
-    const int plane = L.n[1] * L.n[2];
-    return LevelStrides{{-plane, plane, -L.n[2], L.n[2], -1, 1}, LDS_ZERO - offset};
 }
 
 // One colour pass over the planned cells (red_black_sweep_kernel, sim/pressure.py:80-92)
@@ -1387,63 +1407,59 @@ __device__ __forceinline__ void plan_pass(const ColourPlan<ONES, K, NT> &plan, f
 #pragma unroll
     for (int u = 0; u < K; u++) {
         const PlanCell<ONES> &p = plan.cell[u];
-        if (p.packed < 0 || p.t1 == 0.0f) continue;
-        const int c = p.packed & 0xffff;
-        const float t0 = plan_t0<ONES>(p, q, st.s, st.zero);
-        const float source = p.scale * rhs[c];
+        if (p.p < 0 || p.t1 == 0.0f) continue;
+        const float t0 = plan_t0<ONES>(p, q, st.s);
+        const float source = plan_scale<ONES>(p, st.scale) * rhs[p.p];
         const float target = (t0 - source) / p.t1;
-        q[c] = q[c] + relaxation * (target - q[c]);
+        q[p.p] = q[p.p] + relaxation * (target - q[p.p]);
     }
 }
 
-// residual_kernel (sim/multigrid.py:36-43) of the planned cells into res
+// residual_kernel (sim/multigrid.py:36-43) of the planned cells into res (padded like q)
 template <bool ONES, int K, int NT>
 __device__ __forceinline__ void plan_residual(const ColourPlan<ONES, K, NT> &plan, const float *q, const float *rhs,
                                               float *res, const LevelStrides &st) {
 #pragma unroll
     for (int u = 0; u < K; u++) {
         const PlanCell<ONES> &p = plan.cell[u];
-        if (p.packed < 0) continue;
-        const int c = p.packed & 0xffff;
-        const float t0 = plan_t0<ONES>(p, q, st.s, st.zero);
-        res[c] = rhs[c] - (t0 - p.t1 * q[c]) / p.scale;
+        if (p.p < 0) continue;
+        const float t0 = plan_t0<ONES>(p, q, st.s);
+        res[p.p] = rhs[p.p] - (t0 - p.t1 * q[p.p]) / plan_scale<ONES>(p, st.scale);
     }
 }
 
-// restrict_kernel (sim/multigrid.py:46-60) from the stored residuals, and the coarse q cleared (:143)
+// restrict_kernel (sim/multigrid.py:46-60) from the stored residuals into the coarse rhs (the coarse q is
+// already +0.0: every tail level's q starts cleared, the reference's coarse.q.zero_(), :143)
 template <int NT, typename Sync>
 __device__ __forceinline__ void tail_restrict(const TailDims &F, const TailDims &C, const float *res, float *crhs,
-                                              float *cq, int tid, Sync sync) {
-    const Dims nf{F.n[0], F.n[1], F.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
-    for (int c = tid; c < nc.count(); c += NT) {
-        const int r = C.div_n2(c), k = c - r * nc.n2;
-        const int i = C.div_n1(r), j = r - i * nc.n1;
+                                              int tid, Sync sync) {
+    const int cells = C.n[0] * C.n[1] * C.n[2];
+    for (int c = tid; c < cells; c += NT) {
+        const int r = C.div_n2(c), k = c - r * C.n[2];
+        const int i = C.div_n1(r), j = r - i * C.n[1];
         float total = 0.0f, count = 0.0f;
         for (int di = 0; di < 2; di++)
             for (int dj = 0; dj < 2; dj++)
                 for (int dk = 0; dk < 2; dk++) {
                     const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
-                    if (fi < nf.n0 && fj < nf.n1 && fk < nf.n2) {
-                        total = total + res[nf.at(fi, fj, fk)];
+                    if (fi < F.n[0] && fj < F.n[1] && fk < F.n[2]) {
+                        total = total + res[pad_at(F.n, fi, fj, fk)];
                         count = count + 1.0f;
                     }
                 }
-        crhs[c] = total / count;
-        cq[c] = 0.0f;
+        crhs[pad_at(C.n, i, j, k)] = total / count;
     }
     sync();
 }
 
-// A visit of one level with the thread's plans of both colours: `sweeps` red-black sweeps, and with a
-// coarser level, the residual stored and restricted onto it.
+// The sweeps of one visit of a level with the thread's plans of both colours and, with a coarser level, the
+// residual stored and restricted onto it.
 template <bool ONES, int K, int NT, typename Sync>
-__device__ __forceinline__ void visit(const TailLevel &L, const TailDims &C, int offset, float *q,
-                                           const float *rhs, float *res, float *crhs, float *cq, int sweeps,
-                                           float relaxation, int tid, Sync sync) {
-    ColourPlan<ONES, K, NT> red, black;
-    red.build(L, 0, tid);
-    black.build(L, 1, tid);
-    const LevelStrides st = level_strides(L, offset);
+__device__ __forceinline__ void visit_planned(const ColourPlan<ONES, K, NT> &red, const ColourPlan<ONES, K, NT> &black,
+                                              const TailLevel &L, const TailDims &C, float *q, const float *rhs,
+                                              float *res, float *crhs, int sweeps, float relaxation, int tid,
+                                              Sync sync) {
+    const LevelStrides st = level_strides(L);
     for (int s = 0; s < sweeps; s++) {
         plan_pass(red, q, rhs, st, relaxation);
         sync();
@@ -1454,19 +1470,47 @@ __device__ __forceinline__ void visit(const TailLevel &L, const TailDims &C, int
         plan_residual(red, q, rhs, res, st);
         plan_residual(black, q, rhs, res, st);
         sync();
-        tail_restrict<NT>(TailDims{{L.n[0], L.n[1], L.n[2]}, L.div_n1, L.div_n2}, C, res, crhs, cq, tid, sync);
+        tail_restrict<NT>(TailDims{{L.n[0], L.n[1], L.n[2]}, L.div_n1, L.div_n2}, C, res, crhs, tid, sync);
     }
+}
+
+template <bool ONES, int K, int NT, typename Sync>
+__device__ __forceinline__ void visit(const TailLevel &L, const TailDims &C, float *q, const float *rhs, float *res,
+                                      float *crhs, int sweeps, float relaxation, int tid, Sync sync) {
+    ColourPlan<ONES, K, NT> red, black;
+    red.build(L, 0, tid);
+    black.build(L, 1, tid);
+    visit_planned(red, black, L, C, q, rhs, res, crhs, sweeps, relaxation, tid, sync);
+}
+
+// prolongation (sample_index_space of the coarse correction at i / 2 - 1/4, sim/multigrid.py:63-68, with
+// sim/sample.py:10-29's clamp, taps and nested lerps) read from a padded coarse q
+__device__ __forceinline__ float prolong_padded(const float *cq, const TailDims &C, int i, int j, int k) {
+    const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(C.n[0] - 1));
+    const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(C.n[1] - 1));
+    const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(C.n[2] - 1));
+    const int a = (int)x, b = (int)y, c = (int)z;
+    const int row = C.n[2] + 2, plane = (C.n[1] + 2) * row;
+    const int di = a + 1 < C.n[0] ? plane : 0, dj = b + 1 < C.n[1] ? row : 0, dk = c + 1 < C.n[2] ? 1 : 0;
+    const float tx = x - (float)a, ty = y - (float)b, tz = z - (float)c;
+    const int o = pad_at(C.n, a, b, c), o1 = o + dk;
+    const float c00 = lerp_(cq[o], cq[o + di], tx);
+    const float c10 = lerp_(cq[o + dj], cq[o + di + dj], tx);
+    const float c01 = lerp_(cq[o1], cq[o1 + di], tx);
+    const float c11 = lerp_(cq[o1 + dj], cq[o1 + di + dj], tx);
+    return lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
 }
 
 // prolong_add from level C into L (sim/multigrid.py:63-68, 145)
 template <int NT, typename Sync>
 __device__ __forceinline__ void tail_prolong(const TailDims &L, const TailDims &C, float *q, const float *cq,
                                              int tid, Sync sync) {
-    const Dims nf{L.n[0], L.n[1], L.n[2]}, nc{C.n[0], C.n[1], C.n[2]};
-    for (int c = tid; c < nf.count(); c += NT) {
-        const int r = L.div_n2(c), k = c - r * nf.n2;
-        const int i = L.div_n1(r), j = r - i * nf.n1;
-        q[c] = q[c] + prolongation(cq, nc, i, j, k);
+    const int cells = L.n[0] * L.n[1] * L.n[2];
+    for (int c = tid; c < cells; c += NT) {
+        const int r = L.div_n2(c), k = c - r * L.n[2];
+        const int i = L.div_n1(r), j = r - i * L.n[1];
+        const int p = pad_at(L.n, i, j, k);
+        q[p] = q[p] + prolong_padded(cq, C, i, j, k);
     }
     sync();
 }
@@ -1476,12 +1520,14 @@ __device__ __forceinline__ int coarsest_sweeps(const TailLevel &L) {
     return max(20, 4 * longest);  // COARSEST_SWEEPS, COARSEST_SWEEPS_PER_CELL (sim/multigrid.py:30-31)
 }
 
-// ONES: every tail level has fill_hi (1, 1, 1) (its in-domain couplings are all exactly 1)
+// ONES: every tail level has fill_hi (1, 1, 1) (its in-domain couplings are all exactly 1). The first
+// KEEP block levels build their plans once, on the way down, and keep them in registers for the way up.
 template <bool ONES>
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
                                                           const float *__restrict__ rhs_top, int sweeps,
                                                           const double *gate) {
     constexpr int K = ONES ? BLOCK_PLAN_ONES : BLOCK_PLAN;
+    constexpr int KEEP = ONES ? 2 : 1;
     if (gated_off(gate)) return;
     extern __shared__ float lds[];
     // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
@@ -1494,50 +1540,87 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
         uint32_t *dst = reinterpret_cast<uint32_t *>(&t);
         for (int w = threadIdx.x; w < words; w += TAIL_THREADS) dst[w] = src[w];
     }
-    if (threadIdx.x == 0) lds[LDS_ZERO] = 0.0f;
     __syncthreads();
-    for (int c = threadIdx.x; c < t.cells[0]; c += TAIL_THREADS) {
-        lds[t.offset[0] + c] = t.zero_top ? 0.0f : q_top[c];
-        lds[t.offset[0] + t.cells[0] + c] = rhs_top[c];
+    for (int w = threadIdx.x; w < t.q_floats; w += TAIL_THREADS) lds[t.qoff[0] + w] = 0.0f;
+    __syncthreads();
+    const int tid = threadIdx.x;
+    {
+        const TailDims T = load_dims(t, 0);
+        const int cells = T.n[0] * T.n[1] * T.n[2];
+        for (int c = tid; c < cells; c += TAIL_THREADS) {
+            const int r = T.div_n2(c), k = c - r * T.n[2];
+            const int i = T.div_n1(r), j = r - i * T.n[1];
+            const int p = pad_at(T.n, i, j, k);
+            if (!t.zero_top) lds[t.qoff[0] + p] = q_top[c];
+            lds[t.roff[0] + p] = rhs_top[c];
+        }
     }
     __syncthreads();
-    const int last = t.count - 1, tid = threadIdx.x;
+    const int last = t.count - 1;
     const int block_levels = min(t.wave_from, last);
     float *res = lds + t.res_offset;
-    auto q_of = [&](int m) { return lds + t.offset[m]; };
-    auto rhs_of = [&](int m) { return lds + t.offset[m] + t.cells[m]; };
-    // down on the whole block: smooth, residual + restrict, coarse q = 0 (sim/multigrid.py:137-143)
-    for (int m = 0; m < block_levels; m++)
-        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m + 1), t.offset[m], q_of(m), rhs_of(m), res,
-                                        rhs_of(m + 1), q_of(m + 1), sweeps, 1.0f, tid, BlockSync{});
+    auto q_of = [&](int m) { return lds + t.qoff[m]; };
+    auto rhs_of = [&](int m) { return lds + t.roff[m]; };
+    // down on the whole block: smooth, residual + restrict (sim/multigrid.py:137-143); the first KEEP levels'
+    // plans stay for the way up
+    ColourPlan<ONES, K, TAIL_THREADS> red0, black0, red1, black1;
+    auto down_kept = [&](int m, ColourPlan<ONES, K, TAIL_THREADS> &red,
+                         ColourPlan<ONES, K, TAIL_THREADS> &black) __attribute__((always_inline)) {
+        const TailLevel L = load_level(t, m);
+        red.build(L, 0, tid);
+        black.build(L, 1, tid);
+        visit_planned(red, black, L, load_dims(t, m + 1), q_of(m), rhs_of(m), res, rhs_of(m + 1), sweeps, 1.0f, tid,
+                      BlockSync{});
+    };
+    auto up_kept = [&](int m, const ColourPlan<ONES, K, TAIL_THREADS> &red,
+                       const ColourPlan<ONES, K, TAIL_THREADS> &black) __attribute__((always_inline)) {
+        tail_prolong<TAIL_THREADS>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, BlockSync{});
+        visit_planned(red, black, load_level(t, m), load_dims(t, m), q_of(m), rhs_of(m), res, nullptr, sweeps, 1.0f,
+                      tid, BlockSync{});
+    };
+    if (block_levels > 0) down_kept(0, red0, black0);
+    if (KEEP > 1 && block_levels > 1) down_kept(1, red1, black1);
+    for (int m = KEEP; m < block_levels; m++)
+        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m + 1), q_of(m), rhs_of(m), res, rhs_of(m + 1),
+                                     sweeps, 1.0f, tid, BlockSync{});
     if (t.wave_from <= last) {
         // the smallest levels on wavefront 0: down, the coarsest solve, and back up
         if (tid < WAVE) {
             for (int m = t.wave_from; m < last; m++)
-                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m + 1), t.offset[m], q_of(m), rhs_of(m), res,
-                               rhs_of(m + 1), q_of(m + 1), sweeps, 1.0f, tid, WaveSync{});
+                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m + 1), q_of(m), rhs_of(m), res, rhs_of(m + 1),
+                                     sweeps, 1.0f, tid, WaveSync{});
             const TailLevel B = load_level(t, last);
-            visit<ONES, 1, WAVE>(B, load_dims(t, last), t.offset[last], q_of(last), rhs_of(last), res, nullptr, nullptr,
-                           coarsest_sweeps(B), 1.5f, tid, WaveSync{});
+            visit<ONES, 1, WAVE>(B, load_dims(t, last), q_of(last), rhs_of(last), res, nullptr, coarsest_sweeps(B),
+                                 1.5f, tid, WaveSync{});
             for (int m = last - 1; m >= t.wave_from; m--) {
                 tail_prolong<WAVE>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, WaveSync{});
-                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m), t.offset[m], q_of(m), rhs_of(m), res, nullptr,
-                               nullptr, sweeps, 1.0f, tid, WaveSync{});
+                visit<ONES, 1, WAVE>(load_level(t, m), load_dims(t, m), q_of(m), rhs_of(m), res, nullptr, sweeps,
+                                     1.0f, tid, WaveSync{});
             }
         }
         __syncthreads();
     } else {
         const TailLevel B = load_level(t, last);
-        visit<ONES, K, TAIL_THREADS>(B, load_dims(t, last), t.offset[last], q_of(last), rhs_of(last), res, nullptr,
-                                        nullptr, coarsest_sweeps(B), 1.5f, tid, BlockSync{});
+        visit<ONES, K, TAIL_THREADS>(B, load_dims(t, last), q_of(last), rhs_of(last), res, nullptr, coarsest_sweeps(B),
+                                     1.5f, tid, BlockSync{});
     }
     // up on the whole block: prolong-add, smooth (sim/multigrid.py:145-147)
-    for (int m = block_levels - 1; m >= 0; m--) {
+    for (int m = block_levels - 1; m >= KEEP; m--) {
         tail_prolong<TAIL_THREADS>(load_dims(t, m), load_dims(t, m + 1), q_of(m), q_of(m + 1), tid, BlockSync{});
-        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m), t.offset[m], q_of(m), rhs_of(m), res,
-                                        nullptr, nullptr, sweeps, 1.0f, tid, BlockSync{});
+        visit<ONES, K, TAIL_THREADS>(load_level(t, m), load_dims(t, m), q_of(m), rhs_of(m), res, nullptr, sweeps, 1.0f,
+                                     tid, BlockSync{});
     }
-    for (int c = tid; c < t.cells[0]; c += TAIL_THREADS) q_top[c] = lds[t.offset[0] + c];
+    if (KEEP > 1 && block_levels > 1) up_kept(1, red1, black1);
+    if (block_levels > 0) up_kept(0, red0, black0);
+    {
+        const TailDims T = load_dims(t, 0);
+        const int cells = T.n[0] * T.n[1] * T.n[2];
+        for (int c = tid; c < cells; c += TAIL_THREADS) {
+            const int r = T.div_n2(c), k = c - r * T.n[2];
+            const int i = T.div_n1(r), j = r - i * T.n[1];
+            q_top[c] = lds[t.qoff[0] + pad_at(T.n, i, j, k)];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
@@ -2359,26 +2442,46 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
         ones = with;
         h.tail_start = l;
     }
+    // The tail's LDS: each level's q and rhs zero-bordered ((n0 + 2)(n1 + 2)(n2 + 2) floats each) and the padded
+    // residual of the largest level that restricts; a tail that would not fit drops its largest level.
+    auto padded = [&](int l) {
+        return (infos[l].dims[0] + 2) * (infos[l].dims[1] + 2) * (infos[l].dims[2] + 2);
+    };
+    auto tail_floats = [&](int from) {
+        int floats = 0, restricted = 0;
+        for (int l = from; l < n_levels; l++) {
+            floats += 2 * padded(l);
+            if (l + 1 < n_levels) restricted = std::max(restricted, padded(l));
+        }
+        return floats + restricted;
+    };
+    while (h.tail_start < n_levels && sizeof(float) * (size_t)tail_floats(h.tail_start) > TAIL_LDS_BYTES)
+        h.tail_start++;
+    ones = true;
+    for (int l = h.tail_start; l < n_levels; l++)
+        ones = ones && infos[l].fill_hi[0] == 1.0f && infos[l].fill_hi[1] == 1.0f && infos[l].fill_hi[2] == 1.0f;
     h.tail_ones = ones;
     if (h.tail_start < n_levels) {
         h.tail.count = n_levels - h.tail_start;
-        int offset = LDS_HEAD, restricted = 0;
+        int offset = 0, restricted = 0;
         for (int m = 0; m < h.tail.count; m++) {
             h.tail.level[m] = h.level[h.tail_start + m];
             const plm_grid_info &g = infos[h.tail_start + m];
-            h.tail.cells[m] = g.dims[0] * g.dims[1] * g.dims[2];
-            h.tail.offset[m] = offset;
-            offset += 2 * h.tail.cells[m];
+            h.tail.qoff[m] = offset;
+            h.tail.roff[m] = offset + padded(h.tail_start + m);
+            offset += 2 * padded(h.tail_start + m);
             h.tail.div_n1[m] = fast_div(g.dims[1]);
             h.tail.div_n2[m] = fast_div(g.dims[2]);
             h.tail.div_half[m] = fast_div((g.dims[2] + 1) / 2);
-            if (m + 1 < h.tail.count) restricted = std::max(restricted, h.tail.cells[m]);
+            if (m + 1 < h.tail.count) restricted = std::max(restricted, padded(h.tail_start + m));
         }
+        h.tail.q_floats = offset;
         h.tail.res_offset = offset;
         h.tail_bytes = sizeof(float) * (size_t)(offset + restricted);
         h.tail.wave_from = h.tail.count;
         for (int m = h.tail.count - 1; m >= 0; m--) {
-            if (h.tail.cells[m] > WAVE_CELLS || colour_slots(infos[h.tail_start + m]) > WAVE) break;
+            const plm_grid_info &g = infos[h.tail_start + m];
+            if (g.dims[0] * g.dims[1] * g.dims[2] > WAVE_CELLS || colour_slots(g) > WAVE) break;
             h.tail.wave_from = m;
         }
         PLM_TRY(h.tail_ones ? allow_tail_lds<true>(h.tail_bytes) : allow_tail_lds<false>(h.tail_bytes));

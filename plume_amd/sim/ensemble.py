@@ -159,6 +159,7 @@ class Ensemble:
         for m in self.members:
             solve = m.stepper.pressure_solve
             if isinstance(solve, Multigrid) and solve.tolerance > 0.0:
+                solve.settle()
                 solve.graph_cycles = max(1, solve.max_cycles_seen + extra_cycles)
                 solves.append(solve)
         torch.cuda.synchronize(self.device)

@@ -86,6 +86,11 @@ class Multigrid:
         self.cycles_run = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._predicted = 1
         self.max_cycles_seen = 0
+        # the last solve's cycle count, copied to pinned host memory without waiting and read by the next
+        # solve (one host round trip per solve: the gate's)
+        self._count_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self._count_event = torch.cuda.Event()
+        self._count_ready = None
         # Set for graph capture: every tolerance-mode solve then launches exactly this many gated cycles and
         # reads nothing back (cycles past convergence exit on the device gate).
         self.graph_cycles = None
@@ -106,14 +111,32 @@ class Multigrid:
             return
         # Tolerance mode: run the cycle count the last solve needed (cycles past convergence exit at once
         # on the device gate), then add single cycles while the gate says the target is not met. One host
-        # read per solve in the usual case, instead of a launch per cycle up to the cap.
+        # read per solve in the usual case (the gate), instead of a launch per cycle up to the cap; the
+        # count of cycles that ran comes back without a wait and sets the next solve's first launch.
+        if self._count_ready is not None and self._count_ready.query():
+            self._note_cycles(int(self._count_host[0]))
+            self._count_ready = None
         predicted = max(1, min(self._predicted, self.max_cycles))
         self._cycles(predicted, resume=False)
         ran = predicted
         while ran < self.max_cycles and float(self.norm_scratch[1].item()) != 0.0:
             self._cycles(1, resume=True)
             ran += 1
-        self._predicted = max(1, int(self.cycles_run[0].item()))
+        if ran > predicted:  # every added cycle ran: the count is known here
+            self._note_cycles(ran)
+        self._count_host.copy_(self.cycles_run[:1], non_blocking=True)
+        self._count_event.record()
+        self._count_ready = self._count_event
+
+    def settle(self) -> None:
+        """Waits for the last solve's cycle count and takes it into the prediction (max_cycles_seen)."""
+        if self._count_ready is not None:
+            self._count_ready.synchronize()
+            self._note_cycles(int(self._count_host[0]))
+            self._count_ready = None
+
+    def _note_cycles(self, count: int) -> None:
+        self._predicted = max(1, count)
         self.max_cycles_seen = max(self.max_cycles_seen, self._predicted)
 
     def _cycles(self, count: int, resume: bool) -> None:
