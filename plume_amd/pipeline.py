@@ -1,8 +1,9 @@
-"""One recipe run through the simulation and render path to the sheet frames, the numbers file and the
-settings log (cli/run.py:43-130 of the reference, without its PNG, JSON and contact-sheet outputs): load the
+"""One recipe run through the simulation and render path to the sheet, its JSON, the numbers file and the
+settings log (cli/run.py:43-130 of the reference, without its contact sheets and channel strip): load the
 recipe (its extends chain, --set items and optionally its --preview form), simulate on the HIP path capturing a
-raw frame after each frame's substeps, assemble the sheet frames, and measure them."""
+raw frame after each frame's substeps, assemble the sheet frames, measure them, and write the files."""
 
+import sys
 import time
 from dataclasses import dataclass
 from pathlib import Path
@@ -17,6 +18,7 @@ from plume_amd.render.camera import sheet_domain_view
 from plume_amd.render.frame_renderer import FrameRenderer
 from plume_amd.report.numbers import RunFacts, measure, numbers_text
 from plume_amd.sheet.assemble import AssembledFrames, assemble, seat_point
+from plume_amd.sheet.outputs import SheetFacts, recipe_source, sheet_image, sheet_json, write_png
 from plume_amd.sim.device import init_device
 from plume_amd.sim.run import StepLog, simulate
 
@@ -31,6 +33,7 @@ class RunResult:
     numbers: str
     steps: StepLog
     seconds: dict[str, float]  # wall time of "simulate" (render excluded), "render" and "assemble"
+    files: dict[str, Path]  # what was written: "sheet", "json", "numbers", "settings" (empty without `out`)
 
 
 def run_facts(recipe: Recipe, derived: Derived, backend: str, assembled: AssembledFrames, source: str) -> RunFacts:
@@ -52,10 +55,12 @@ def sheet_numbers(raw_frames: list[np.ndarray], recipe: Recipe, derived: Derived
 
 
 def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", source: str | None = None,
-               out: Path | None = None, sets: tuple[str, ...] = ()) -> RunResult:
+               out: Path | None = None, sets: tuple[str, ...] = (), command: str | None = None) -> RunResult:
     """Runs a recipe (with its extends chain and `--set` items) on the HIP path and returns its sheet frames and
-    numbers; with `out`, also writes `numbers.txt` and the settings log `settings.toml` there. `source` is the
-    recipe path the numbers file names (default: as given)."""
+    numbers. With `out` it also writes, into `out` (into `out`/preview for a preview, as cli/run.py:82-86 does):
+    the sheet `<name>.png`, its `<name>.json`, `numbers.txt` and the settings log `settings.toml`. `source` is
+    the recipe path the numbers file and the JSON name (default: relative to its git repository's root);
+    `command` the command line the JSON records as written_by (default: the `plume run` line of this call)."""
     import torch
 
     recipe, origin = load_recipe_with_origin(Path(recipe_path), preview=preview, sets=tuple(sets))
@@ -77,22 +82,36 @@ def run_recipe(recipe_path: Path, preview: bool = False, device: str = "cuda", s
     torch.cuda.synchronize()
     seconds["simulate"] = time.perf_counter() - start - seconds["render"]
     start = time.perf_counter()
-    assembled, numbers = sheet_numbers(raw_frames, recipe, derived, device_name,
-                                       source if source is not None else str(recipe_path))
+    source = source if source is not None else recipe_source(Path(recipe_path))
+    assembled, numbers = sheet_numbers(raw_frames, recipe, derived, device_name, source)
     seconds["assemble"] = time.perf_counter() - start
+    files: dict[str, Path] = {}
     if out is not None:
-        Path(out).mkdir(parents=True, exist_ok=True)
-        (Path(out) / "numbers.txt").write_text(numbers, newline="\n")
-        facts = {"warmup_steps_run": steps.warmup_steps, "steps_per_frame": list(steps.steps_per_frame),
-                 "total_steps": steps.total_steps}
-        (Path(out) / "settings.toml").write_text(recipe_to_toml(recipe, derived, device_name, origin, facts),
-                                                 newline="\n")
-    return RunResult(recipe, derived, device_name, raw_frames, assembled, numbers, steps, seconds)
+        folder = Path(out) / "preview" if preview else Path(out)
+        folder.mkdir(parents=True, exist_ok=True)
+        name = recipe.output.name
+        files = {"sheet": folder / f"{name}.png", "json": folder / f"{name}.json",
+                 "numbers": folder / "numbers.txt", "settings": folder / "settings.toml"}
+        write_png(files["sheet"], sheet_image(assembled.frames, recipe.output.columns))
+        if command is None:
+            command = " ".join(["plume", "run", str(recipe_path), *(["--preview"] if preview else []),
+                                *(f"--set {item}" for item in sets), "--out", str(out)])
+        facts = SheetFacts(frames=len(assembled.frames), columns=recipe.output.columns,
+                           frame_px=recipe.output.frame_px, fps=derived.fps, frame_m=assembled.frame_m,
+                           seat=assembled.seat, channels=recipe.output.channels, source=source, written_by=command)
+        files["json"].write_text(sheet_json(facts), newline="\n")
+        files["numbers"].write_text(numbers, newline="\n")
+        steps_facts = {"warmup_steps_run": steps.warmup_steps, "steps_per_frame": list(steps.steps_per_frame),
+                       "total_steps": steps.total_steps}
+        files["settings"].write_text(recipe_to_toml(recipe, derived, device_name, origin, steps_facts),
+                                     newline="\n")
+    return RunResult(recipe, derived, device_name, raw_frames, assembled, numbers, steps, seconds, files)
 
 
 def main(argv: list[str] | None = None) -> int:
     """`python -m plume_amd.pipeline RECIPE [--preview] [--set key=value ...] [--out DIR]`: runs the recipe on
-    the HIP path, prints its numbers file and stage times, and with --out writes numbers.txt and settings.toml."""
+    the HIP path, prints its numbers file and stage times, and with --out writes the sheet PNG and JSON,
+    numbers.txt and settings.toml."""
     import argparse
 
     parser = argparse.ArgumentParser(prog="python -m plume_amd.pipeline", description=main.__doc__)
@@ -101,7 +120,8 @@ def main(argv: list[str] | None = None) -> int:
     parser.add_argument("--set", dest="sets", action="append", default=[], metavar="KEY=VALUE")
     parser.add_argument("--out", type=Path, default=None)
     args = parser.parse_args(argv)
-    result = run_recipe(args.recipe, preview=args.preview, device="cuda", out=args.out, sets=tuple(args.sets))
+    result = run_recipe(args.recipe, preview=args.preview, device="cuda", out=args.out, sets=tuple(args.sets),
+                        command=" ".join(["plume", "run", *(argv if argv is not None else sys.argv[1:])]))
     print(result.numbers, end="")
     print(f"\nsubsteps {result.steps.total_steps}, simulate {result.seconds['simulate']:.3f} s, render "
           f"{result.seconds['render']:.3f} s, assemble {result.seconds['assemble']:.3f} s")

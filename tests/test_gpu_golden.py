@@ -116,3 +116,43 @@ def test_a_variation_runs_like_the_recipe_it_resolves_to(device, tmp_path):
     assert varied.numbers == whole.numbers
     resolved = tomli.loads((tmp_path / "out" / "settings.toml").read_text())["resolved"]
     assert resolved["extends_chain"] == ["tiny.toml"] and resolved["sets"] == ["emitter.puff.radius_m=0.25"]
+
+
+@pytest.mark.parametrize("channels", ["heat,thickness,shade,unused", "heat,thickness,normal_x,normal_y"])
+def test_a_run_writes_the_sheet_and_its_json_and_rewrites_them_byte_for_byte(device, tmp_path, channels):
+    """tests/cli/test_run.py:27-74 of the reference on the HIP path: the sheet PNG and JSON beside numbers.txt and
+    settings.toml, the JSON's facts, the sheet holding the assembled frames, and identical bytes on a rerun;
+    a preview goes into a preview folder."""
+    import json
+
+    from PIL import Image
+
+    from plume_amd.pipeline import run_recipe
+
+    path = tmp_path / "tiny.toml"
+    path.write_text(TINY + f'channels = "{channels}"\n')
+    first = run_recipe(path, device="cuda", out=tmp_path / "one")
+    second = run_recipe(path, device="cuda", out=tmp_path / "two")
+    names = ("tiny.png", "tiny.json", "numbers.txt", "settings.toml")
+    assert sorted(p.name for p in first.files.values()) == sorted(names)
+    for name in names:
+        assert (tmp_path / "one" / name).is_file(), name
+        if name == "tiny.json":
+            continue  # written_by names the output folder
+        assert (tmp_path / "one" / name).read_bytes() == (tmp_path / "two" / name).read_bytes(), name
+    sheet = json.loads((tmp_path / "one" / "tiny.json").read_text())
+    assert sheet["frames"] == 4 and sheet["columns"] == 2 and sheet["frame_px"] == 64 and sheet["fps"] == 4.0
+    assert sheet["channels"] == channels and sheet["written_by"].startswith("plume run ")
+    assert sheet["frame_m"] == round(first.assembled.frame_m, 6)
+    image = np.asarray(Image.open(tmp_path / "one" / "tiny.png"))
+    assert image.shape == (128, 128, 4)
+    for index, frame in enumerate(first.assembled.frames):
+        row, column = divmod(index, 2)
+        np.testing.assert_array_equal(image[row * 64:(row + 1) * 64, column * 64:(column + 1) * 64], frame)
+    if channels.endswith("unused"):
+        assert np.all(image[..., 3] == 255)
+    else:
+        empty = (image[..., 0] == 0) & (image[..., 1] == 0)
+        assert np.all(image[empty, 2:] == 128)
+    preview = run_recipe(path, preview=True, device="cuda", out=tmp_path / "p")
+    assert json.loads((tmp_path / "p" / "preview" / "tiny.json").read_text())["frames"] == len(preview.assembled.frames)
