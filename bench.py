@@ -275,15 +275,16 @@ def ensemble_bench(args) -> dict:
     """Batched ensemble: 32 smoke puffs with seeded sizes over 48^3, 64x96x64, 96x128x96
     and 128x192x128, emitter radii 4-12 cells, alpha in [0.05, 0.2] and epsilon in [0, 0.5]. The 12 emitting
     frames and one more run eagerly; then the same number of frames is timed twice from the state reached:
-    batched (one captured HIP graph launch per two substeps of all 32 members, members on 4 streams) and plain
-    (every member stepped eagerly, one after the other, as a single-grid run does)."""
+    batched (one captured HIP graph launch per two substeps of all 32 members, the members on 8 streams as
+    concurrent graph branches) and plain (every member stepped eagerly, one after the other, as a single-grid
+    run does)."""
     import torch
 
     from plume_amd.sim.device import init_device
     from plume_amd.sim.ensemble import Ensemble, ensemble_tables
 
     device = init_device("cuda")
-    ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device, streams=4)
+    ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device)
     ens.autotune()
     cells = ens.cells
     print(f"[bench] ensemble of {len(ens.members)}: {cells} cells", file=sys.stderr, flush=True)
@@ -577,8 +578,9 @@ def main() -> int:
         # all the logical CPUs the host gives this process (OMP_NUM_THREADS when it names fewer), and one
         threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30)
         cpu_cycles = max(1, round(mean_cycles))  # the port runs a fixed count, as the reference does
-        cpu_recipe = dataclasses.replace(recipe, solver=dataclasses.replace(recipe.solver,
-                                                                            multigrid_cycles=cpu_cycles))
+        # a fixed cycle count, without the per-cycle residual checks of the tolerance mode
+        cpu_recipe = dataclasses.replace(recipe, solver=dataclasses.replace(recipe.solver, multigrid_cycles=cpu_cycles,
+                                                                            multigrid_tolerance=0.0))
         single_steps = max(2, args.cpu_steps // 8)
         timings = cpu_baseline(grid, cpu_recipe, derived, step_index * dt,
                                [(threads, args.cpu_steps), (1, single_steps)])

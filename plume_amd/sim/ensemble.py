@@ -8,9 +8,10 @@ reach the GPU:
 - after the emitting frames, two substeps of every member are captured once into one HIP graph (through
   torch.cuda.CUDAGraph) and replayed: one launch from the host per two substeps of the whole ensemble,
   instead of ~100 host-side launches per member and substep;
-- members are spread over a few side streams inside the graph, so the small members' latency-bound kernels
-  (coarse multigrid levels, reductions) overlap with each other and with the large members' bandwidth-bound
-  ones;
+- members are spread over side streams inside the graph (8 by default, the largest members first onto the
+  least loaded stream by cells), so the graph's branches run concurrently: the members' latency-bound kernels
+  (coarse multigrid levels, the one-workgroup tails, reductions) overlap with each other and with the large
+  members' bandwidth-bound ones;
 - the pressure solves run their tolerance mode with a fixed number of device-gated V-cycles (the most any
   member needed in the eager frames, plus one), so nothing in the graph reads back to the host. Each solve in
   the graph adds 1 to its member's device counter when it ends with the residual still above tolerance; the
@@ -70,8 +71,20 @@ def ensemble_tables(count: int, seed: int, frames: int = 48, sizes=ENSEMBLE_SIZE
     return tables
 
 
+def balanced_streams(cells: list[int], streams: int) -> list[int]:
+    """A stream per member, the largest members first onto the least loaded stream (longest processing time
+    first, by cells), so the branches of a replayed graph finish close together."""
+    load = [0] * streams
+    out = [0] * len(cells)
+    for index in sorted(range(len(cells)), key=lambda i: -cells[i]):
+        k = min(range(streams), key=lambda s: (load[s], s))
+        out[index] = k
+        load[k] += cells[index]
+    return out
+
+
 class Ensemble:
-    def __init__(self, tables: list[dict], device: str, streams: int = 4):
+    def __init__(self, tables: list[dict], device: str, streams: int = 8):
         self.device = device
         self.members = []
         for table in tables:
@@ -86,6 +99,7 @@ class Ensemble:
         self.frame_s = frame_dt.pop()
         self.base_steps = base.pop()
         self.streams = [torch.cuda.Stream(device=device) for _ in range(max(1, streams))]
+        self.stream_of = balanced_streams([m.cells for m in self.members], len(self.streams))
         self.graphs: dict[float, torch.cuda.CUDAGraph] = {}
         self.frames_done = 0
         self.extra_cycles = 1  # device-gated V-cycles captured beyond the most any eager solve needed
@@ -93,6 +107,9 @@ class Ensemble:
         self.redone: list[int] = []  # frames replayed with an open gate, then run again eagerly
         # per member: solves in replayed graphs that ended above tolerance (device, cleared each frame)
         self.open_solves = torch.zeros(len(self.members), dtype=torch.int32, device=device)
+        # per member: the most cycles any solve ran in the replays of the current frame (device)
+        self.frame_cycles = torch.zeros(len(self.members), dtype=torch.int32, device=device)
+        self.budgets: dict[float, list[int | None]] = {}  # per graph: each member's captured cycle budget
         self._snapshot: list[list[torch.Tensor]] | None = None
 
     @property
@@ -156,12 +173,15 @@ class Ensemble:
         if self.turbulent:
             raise ValueError("turbulence is on: its pattern depends on the time and cannot be replayed")
         solves = []
+        budgets: list[int | None] = []
         for m in self.members:
             solve = m.stepper.pressure_solve
+            budget = None
             if isinstance(solve, Multigrid) and solve.tolerance > 0.0:
                 solve.settle()
-                solve.graph_cycles = max(1, solve.max_cycles_seen + extra_cycles)
+                budget = solve.graph_cycles = max(1, solve.recent_max + extra_cycles)
                 solves.append(solve)
+            budgets.append(budget)
         torch.cuda.synchronize(self.device)
         graph = torch.cuda.CUDAGraph()
         try:
@@ -169,20 +189,24 @@ class Ensemble:
                 main = torch.cuda.current_stream()
                 for s in self.streams:
                     s.wait_stream(main)
-                for index, m in enumerate(self.members):
-                    with torch.cuda.stream(self.streams[index % len(self.streams)]):
+                for index in sorted(range(len(self.members)), key=lambda i: -self.members[i].cells):
+                    m = self.members[index]
+                    with torch.cuda.stream(self.streams[self.stream_of[index]]):
                         for substep in range(2):
                             m.stepper.step(time_s + substep * dt, dt)
                             solve = m.stepper.pressure_solve
                             if solve in solves:  # its gate: 1 while above tolerance (plm_vcycle)
                                 count = self.open_solves[index:index + 1]
                                 count.add_(solve.norm_scratch[1:2].ne(0.0).to(torch.int32))
+                                most = self.frame_cycles[index:index + 1]
+                                torch.maximum(most, solve.cycles_run[0:1], out=most)
                 for s in self.streams:
                     main.wait_stream(s)
         finally:
             for solve in solves:
                 solve.graph_cycles = None
         self.graphs[dt] = graph
+        self.budgets[dt] = budgets
         return graph
 
     def run_frame(self, replay: bool = True) -> int:
@@ -196,10 +220,22 @@ class Ensemble:
             graph = self.graphs.get(dt) or self.capture(start_s, dt, self.extra_cycles)
             self._take_snapshot()
             self.open_solves.zero_()
+            self.frame_cycles.zero_()
             for _ in range(steps // 2):
                 graph.replay()
             replayed = True
-            if any(self.open_solves.tolist()):
+            opened, most = torch.stack([self.open_solves, self.frame_cycles]).tolist()
+            if not any(opened):
+                # the cycles the replayed solves ran set the members' predictions; a budget two or more cycles
+                # above what a member now needs (plus the margin) wastes gated launches: capture again
+                stale = False
+                for m, count, budget in zip(self.members, most, self.budgets[dt]):
+                    if budget is not None:
+                        m.stepper.pressure_solve.note_cycles(count)
+                        stale = stale or budget > m.stepper.pressure_solve.recent_max + self.extra_cycles
+                if stale:
+                    self.graphs.pop(dt, None)
+            if any(opened):
                 # some solve ended above tolerance: undo the frame, run it eagerly (each solve continues to its
                 # target) and give the next capture the cycle counts the eager solves needed
                 self._restore_snapshot()

@@ -13,6 +13,7 @@ residual + restrict, and every level of at most a few thousand cells solved by o
 """
 
 import ctypes
+from collections import deque
 from dataclasses import dataclass
 
 import torch
@@ -26,6 +27,7 @@ PLM_LDS_CELLS = 8192  # include/plume_hip.h: levels this small (together) are so
 COARSEST_SWEEPS = 20
 COARSEST_SWEEPS_PER_CELL = 4
 COARSEST_RELAXATION = 1.5
+RECENT_SOLVES = 8  # the solves whose cycle counts set a replayed graph's budget
 SMOOTHING_RELAXATION = 1.0
 
 
@@ -86,6 +88,7 @@ class Multigrid:
         self.cycles_run = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._predicted = 1
         self.max_cycles_seen = 0
+        self.recent = deque(maxlen=RECENT_SOLVES)  # cycle counts of the latest solves
         # the last solve's cycle count, copied to pinned host memory without waiting and read by the next
         # solve (one host round trip per solve: the gate's)
         self._count_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
@@ -138,6 +141,16 @@ class Multigrid:
     def _note_cycles(self, count: int) -> None:
         self._predicted = max(1, count)
         self.max_cycles_seen = max(self.max_cycles_seen, self._predicted)
+        self.recent.append(self._predicted)
+
+    def note_cycles(self, count: int) -> None:
+        """Takes the cycle count of a solve run elsewhere (a replayed graph) into the prediction."""
+        self._note_cycles(int(count))
+
+    @property
+    def recent_max(self) -> int:
+        """The most cycles any of the latest solves ran (the most ever, before any solve)."""
+        return max(self.recent) if self.recent else max(1, self.max_cycles_seen)
 
     def _cycles(self, count: int, resume: bool) -> None:
         q, rhs, residual = self._pointers()
