@@ -97,7 +97,9 @@ struct Coupling {
 struct LevelArgs {
     plm_grid_info g;
     Coupling cp;
-    int unit;  // every coupling and volume is exactly 1 (the simulation grid)
+    int unit;     // every coupling and volume is exactly 1 (the simulation grid)
+    int flat;     // fill_hi (1, 1, 1): every coupling between two cells and every volume is exactly 1
+    float ghost;  // the open-face ghost coupling of a flat level (1 on a unit level)
 };
 
 Coupling make_coupling(const plm_grid_info &g) {
@@ -137,6 +139,8 @@ LevelArgs level_args(const plm_grid_info &g) {
     L.g = g;
     L.cp = make_coupling(g);
     L.unit = unit_level(g) ? 1 : 0;
+    L.flat = g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f ? 1 : 0;
+    L.ghost = L.cp.ghost[0][0][0];
     return L;
 }
 
@@ -427,6 +431,20 @@ __device__ __forceinline__ void static_for(F &&f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// t / d for 0 <= t < 2^32 / d: the multiply-high by m = ceil(2^32 / d) is exact there (its error t (m d - 2^32)
+// / (d 2^32) stays below 1 / d). Tile counts and tail indices are far inside that range.
+struct FastDiv {
+    uint32_t d, m;
+    __device__ __forceinline__ int operator()(int t) const { return d == 1 ? t : (int)__umulhi((uint32_t)t, m); }
+};
+
+FastDiv fast_div(int d) {
+    FastDiv f;
+    f.d = (uint32_t)d;
+    f.m = d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d);
+    return f;
+}
+
 // Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD with its own L2. Renumbering so
 // that XCD x runs one contiguous range of tiles keeps a tile's halo neighbours in the same L2.
 constexpr int XCDS = 8;
@@ -435,41 +453,48 @@ __device__ __forceinline__ int xcd_contiguous(int block, int blocks) {
     return x * per + min(x, rem) + block / XCDS;
 }
 
-// x / d, correctly rounded, for the integer sum of couplings d = 4, 5 or 6 of a unit-coupling level's cell,
-// from r = RN(1 / d): q = RN(x r), then Markstein's correction q + RN(x - q d) r, both FMAs. Checked
-// exhaustively against the IEEE quotient over every float x: equal for d = 4 and 5 except x = -0, and for
-// d = 6 except |x| < 2^-125 (subnormal quotients). Such x (and zeros) take the IEEE division.
-__device__ __forceinline__ float divide_small(float x, float d, float r) {
+// x / d, correctly rounded, where d is a cell's sum of couplings t1. For d = 4, 5 or 6, from r = RN(1 / d):
+// q = RN(x r), then Markstein's correction q + RN(x - q d) r, both FMAs. Checked exhaustively against the
+// IEEE quotient over every float x: equal for d = 4 and 5 except x = -0, and for d = 6 except |x| < 2^-125
+// (subnormal quotients). Such x (and zeros), and lanes whose d is another value (`ieee`: a ghost coupling
+// other than 1, a corner cell), take the IEEE division; lanes whose result is not used (`active` false) take
+// neither. The test is one ballot per wavefront (a scalar value), so the common case saves no exec mask.
+__device__ __forceinline__ float divide_flat(float x, float d, float r, bool ieee, bool active) {
     float q = x * r;
     const float e = __builtin_fmaf(-q, d, x);
     q = __builtin_fmaf(e, r, q);
-    // tested once per wavefront (a ballot is a scalar value), so the common case saves no exec mask
-    const bool tiny = __builtin_fabsf(x) < 0x1p-100f;
-    if (__builtin_expect(__builtin_amdgcn_ballot_w64(tiny) != 0, 0)) {
-        if (tiny) q = x / d;
+    const bool slow = active && (ieee || __builtin_fabsf(x) < 0x1p-100f);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(slow) != 0, 0)) {
+        if (slow) q = x / d;
     }
     return q;
 }
 
-// The two-colour tile sweep on a unit-coupling level (every coupling and cell volume exactly 1) for a tile
-// whose i and j halo rows all lie inside the level and whose 64 cells along k all exist, the common case on
-// the large levels. The same values as the general path (k_rb_sweep_tile below), with an update reduced to
-// its arithmetic:
+// The two-colour tile sweep on a flat level: fill_hi (1, 1, 1), so every coupling between two cells of the
+// level and every cell volume is exactly 1, and a face on the level's boundary couples by `ghost` when open
+// (1 on the simulation grid, 1 / (0.5 + ghost_gap) below it) or by 0 when closed. The same values as the
+// general path (k_rb_sweep_tile below), with an update reduced to its arithmetic:
+//   - neighbor_sum's t0 is the plain sum of the six neighbours in the reference's order x-, x+, y-, y+, z-,
+//     z+ (1 q = q exactly; a neighbour outside the level is +0.0 in the tile, the +0.0 the reference adds
+//     for a ghost or a closed face); its t1 = (((((0 + c_x-) + c_x+) + c_y-) + c_y+) + c_z-) + c_z+ is
+//     (4 + c_z-) + c_z+ on a row whose four x and y neighbours lie in the level, a per-lane constant, and is
+//     formed from the row's own couplings on the level's boundary rows;
+//   - the source is cell_m² rhs, cell_m² rounded once, as (1 cell_m) cell_m; the division by t1 is
+//     divide_flat;
 //   - a wavefront updates two rows at once, one per half, of one class: the parity sigma of the cells' k - k0,
 //     which fixes where a cell's k neighbours sit in the colour-split row. The pair is rows (2a, 2t + x) and
 //     (2a + 1, 2t + 1 - x) of the colour's row block, so the second row is a fixed offset from the first that
 //     depends on sigma alone: per-lane constants hold the half's offset and the cell's slot, and the seven
-//     LDS reads of an update are one scalar row base plus one add, then constant offsets;
-//   - Σc per lane is 4 plus the k neighbours that couple (a closed k face takes one from the cells at k = 0
-//     or nz - 1), the volume is 1 (source = cell_m² rhs, cell_m² rounded once, as (1 cell_m) cell_m), and the
-//     division by Σc is divide_small;
-//   - loads and stores of a row go from the row's scalar base pointer.
-// `edge` updates the ring rows' red cells in the k halo (k0 - 1 or k0 + 64) with the general update.
-template <int TI, int TJ, int THREADS, int START, typename T, typename EdgeUpdate>
-__device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *__restrict__ s, const T *__restrict__ q_in,
-                                                    T *__restrict__ q_out, const T *__restrict__ rhs,
-                                                    const T *__restrict__ coarse, Dims nc, float relaxation, int i0,
-                                                    int j0, int k0, EdgeUpdate edge) {
+//     LDS reads of an update are one scalar row base plus one add, then constant offsets.
+// EDGE false: the tile, its ring and its halo lie inside the level and the tile's 64 cells along k exist;
+// loads and stores of a row go from the row's scalar base pointer. EDGE true: rows outside the level, and
+// lanes past its last k, hold +0.0 in the tile and are never updated or stored. `edge` updates the ring
+// rows' red cells in the k halo (k0 - 1 or k0 + 64) with the general update.
+template <int TI, int TJ, int THREADS, int START, bool EDGE, typename T, typename EdgeUpdate>
+__device__ __forceinline__ void sweep_tile_flat(const LevelArgs &L, float *__restrict__ s, const T *__restrict__ q_in,
+                                                T *__restrict__ q_out, const T *__restrict__ rhs,
+                                                const T *__restrict__ coarse, Dims nc, float relaxation, int i0,
+                                                int j0, int k0, EdgeUpdate edge) {
     constexpr int WAVES = THREADS / 64;
     constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
     constexpr uint32_t B = sizeof(T);
@@ -484,8 +509,17 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
     const int m = lane & 31;
     const bool upper = lane >= 32;
     const float cm2 = uniform(L.g.cell_m) * uniform(L.g.cell_m);
-    const bool open_lo = uniform(L.g.open_lo[2]) == 1, open_hi = uniform(L.g.open_hi[2]) == 1;
-    float t1[2], rcp[2];
+    const float ghost = uniform(L.ghost);
+    bool open_lo[3], open_hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        open_lo[a] = uniform(L.g.open_lo[a]) == 1;
+        open_hi[a] = uniform(L.g.open_hi[a]) == 1;
+    }
+    // per class sigma: the lane's cell k = k0 + 2 m + sigma, its z couplings, and t1, RN(1 / t1) and whether
+    // t1 takes the IEEE division on a row whose four x and y neighbours lie in the level
+    float cz_lo[2], cz_hi[2], t1[2], rcp[2];
+    bool ieee[2], k_in[2];
     // per colour c (0 red, 1 black) and class sigma: the lane's LDS index relative to its pair's first row's
     // i-1 neighbour row, and its rhs byte offset relative to the first row's (i n1 + j) n2
     int lds_lane[2][2];
@@ -493,9 +527,12 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
 #pragma unroll
     for (int sigma = 0; sigma < 2; sigma++) {
         const int k = k0 + 2 * m + sigma;
-        const float lo = (k > 0 || open_lo) ? 1.0f : 0.0f, hi = (k < n.n2 - 1 || open_hi) ? 1.0f : 0.0f;
-        t1[sigma] = (4.0f + lo) + hi;  // integers: the reference's sum in any order
+        k_in[sigma] = !EDGE || k < n.n2;
+        cz_lo[sigma] = k > 0 ? 1.0f : open_lo[2] ? ghost : 0.0f;
+        cz_hi[sigma] = k < n.n2 - 1 ? 1.0f : open_hi[2] ? ghost : 0.0f;
+        t1[sigma] = (4.0f + cz_lo[sigma]) + cz_hi[sigma];
         rcp[sigma] = 1.0f / t1[sigma];
+        ieee[sigma] = !(t1[sigma] == 4.0f || t1[sigma] == 5.0f || t1[sigma] == 6.0f);
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             const int step = c == 0 ? 1 - 2 * sigma : 2 * sigma - 1;  // the second row's j offset
@@ -505,47 +542,59 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
     }
     const int g_tile = (i0 * n.n1 + j0) * n.n2;
     // pair p of colour c: its class, the LDS row index of its first row's i-1 neighbour row, and the first
-    // row's (i n1 + j) n2
-    auto decode = [&](int c, int pair, int &sigma, int &lds_row, int &g_row) {
+    // row's tile-relative (ii, jj)
+    auto decode = [&](int c, int pair, int &sigma, int &lds_row, int &ii, int &jj) {
         const int per_class = c == 0 ? RED_CLASS : BLACK_CLASS, per_a = c == 0 ? RED_T : BLACK_T;
         sigma = pair >= per_class;
         const int q = pair - sigma * per_class, a = q / per_a, t = q - a * per_a;
         // red rows (2a, 2t + sigma) of the ring (tile row - 1); black rows (2a, 2t + 1 - sigma) of the tile
-        const int ii = c == 0 ? 2 * a - 1 : 2 * a, jj = c == 0 ? 2 * t + sigma - 1 : 2 * t + 1 - sigma;
+        ii = c == 0 ? 2 * a - 1 : 2 * a;
+        jj = c == 0 ? 2 * t + sigma - 1 : 2 * t + 1 - sigma;
         lds_row = (ii + TH - 1) * LJ + (jj + TH);
-        g_row = g_tile + ii * n12 + jj * n.n2;
     };
+    // the lane's row (i, j) in a pair whose first row is (ii, jj): the first row on the lower half, the second
+    // on the upper
+    auto lane_row = [&](int c, int sigma, int ii, int jj, int &i, int &j) {
+        const int step = c == 0 ? 1 - 2 * sigma : 2 * sigma - 1;
+        i = i0 + ii + (upper ? 1 : 0);
+        j = j0 + jj + (upper ? step : 0);
+    };
+    auto row_inside = [&](int i, int j) { return (unsigned)i < (unsigned)n.n0 && (unsigned)j < (unsigned)n.n1; };
     auto ld = [](const T *base, uint32_t byte_off) {
         return to_f32(*reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off));
+    };
+    auto load_rhs = [&](int c, int pair, int &sigma, int &lds_row) {
+        int ii, jj;
+        decode(c, pair, sigma, lds_row, ii, jj);
+        if (pair >= (c == 0 ? RED_PAIRS : BLACK_PAIRS)) return 0.0f;
+        const uint32_t off = (uint32_t)(g_tile + ii * n12 + jj * n.n2) * B + (sigma ? rhs_lane[c][1] : rhs_lane[c][0]);
+        if (!EDGE) return ld(rhs, off);
+        // on an EDGE tile the first row's offset may lie outside the level; only an inside cell's is used
+        int i, j;
+        lane_row(c, sigma, ii, jj, i, j);
+        float value = 0.0f;
+        if (row_inside(i, j) && (sigma ? k_in[1] : k_in[0])) value = ld(rhs, off);
+        return value;
     };
     float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS];
     int red_sigma[RED_ITERS], red_row[RED_ITERS], black_sigma[BLACK_ITERS], black_row[BLACK_ITERS];
 #pragma unroll
-    for (int u = 0; u < RED_ITERS; u++) {
-        int g_row;
-        decode(0, wave + u * WAVES, red_sigma[u], red_row[u], g_row);
-        rhs_red[u] = 0.0f;
-        if (wave + u * WAVES < RED_PAIRS)
-            rhs_red[u] = ld(rhs, (uint32_t)g_row * B + (red_sigma[u] ? rhs_lane[0][1] : rhs_lane[0][0]));
-    }
+    for (int u = 0; u < RED_ITERS; u++) rhs_red[u] = load_rhs(0, wave + u * WAVES, red_sigma[u], red_row[u]);
 #pragma unroll
-    for (int u = 0; u < BLACK_ITERS; u++) {
-        int g_row;
-        decode(1, wave + u * WAVES, black_sigma[u], black_row[u], g_row);
-        rhs_black[u] = 0.0f;
-        if (wave + u * WAVES < BLACK_PAIRS)
-            rhs_black[u] = ld(rhs, (uint32_t)g_row * B + (black_sigma[u] ? rhs_lane[1][1] : rhs_lane[1][0]));
-    }
+    for (int u = 0; u < BLACK_ITERS; u++)
+        rhs_black[u] = load_rhs(1, wave + u * WAVES, black_sigma[u], black_row[u]);
     // The tile's start values, row by row: row r = wave + u WAVES of the (TI + 4) x (TJ + 4) rows, its offset
     // carried from row to row in scalar registers; the k halo (4 cells a row) by a second loop. All loads of a
     // batch are in flight before the first LDS write (one batch of every row, unless the start adds the
     // prolongation).
     const int own_slot = ((lane + TH) & 1) * HALF + ((lane + TH) >> 1);
     const uint32_t lane_b = (uint32_t)(k0 + lane) * B;
-    // FROM_PROLONG: the prolongation's k part per lane (the same for every row), its i and j part per row
+    const bool k_load = !EDGE || k0 + lane < n.n2;
+    // FROM_PROLONG: the prolongation's k part per lane (the same for every row), its i and j part per row; a
+    // lane past the last k takes the last k's (its value is not used)
     uint32_t pk0 = 0, pk1 = 0;
     float tz = 0.0f;
-    if (START == FROM_PROLONG) prolong_k<T>(nc, k0 + lane, pk0, pk1, tz);
+    if (START == FROM_PROLONG) prolong_k<T>(nc, min(k0 + lane, n.n2 - 1), pk0, pk1, tz);
     auto prolong_row = [&](int i, int j) { return prolong_at_row(coarse, nc, i, j, pk0, pk1, tz); };
     int ii = 0, jj = wave, g = g_tile - TH * n12 + (wave - TH) * n.n2;  // row (ii, jj) at (i0 - 2 + ii, j0 - 2 + jj)
     constexpr int ROWS = LI * LJ / WAVES;
@@ -560,7 +609,7 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
             const int i = i0 - TH + row / LJ, j = j0 - TH + row % LJ;
             const int k = h < 2 ? k0 - TH + h : k0 + TK - 2 + h;
             float value = 0.0f;
-            if (START != FROM_ZERO && k >= 0 && k < n.n2) {
+            if (START != FROM_ZERO && (!EDGE || row_inside(i, j)) && k >= 0 && k < n.n2) {
                 value = ld(q_in, (uint32_t)((i * n.n1 + j) * n.n2 + k) * B);
                 if (START == FROM_PROLONG) value = value + prolongation(coarse, nc, i, j, k);
             }
@@ -572,11 +621,11 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
         float v[BATCH];
 #pragma unroll
         for (int u = 0; u < BATCH; u++) {
-            if (START == FROM_ZERO) {
-                v[u] = 0.0f;
-            } else {
+            v[u] = 0.0f;
+            const int i = i0 - TH + ii, j = j0 - TH + jj;
+            if (START != FROM_ZERO && (!EDGE || (row_inside(i, j) && k_load))) {
                 v[u] = ld(q_in, (uint32_t)g * B + lane_b);
-                if (START == FROM_PROLONG) v[u] = v[u] + prolong_row(i0 - TH + ii, j0 - TH + jj);
+                if (START == FROM_PROLONG) v[u] = v[u] + prolong_row(i, j);
             }
             // next row: WAVES rows on, wrapping into the next i row at most once (WAVES <= LJ)
             jj += WAVES;
@@ -598,8 +647,8 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
     __syncthreads();
     // One update: red_black_sweep_kernel's arithmetic (sim/pressure.py:86-92), the neighbours added in the
     // reference's order x-, x+, y-, y+, z-, z+. `base` indexes the cell's i-1 neighbour; an empty asm keeps it
-    // the one address register that every read offsets from.
-    auto update = [&](auto sigma_c, int base, float rhs_c) {
+    // the one address register that every read offsets from. `write` false leaves the cell as it is.
+    auto update = [&](auto sigma_c, int base, float rhs_c, float t1_c, float rcp_c, bool ieee_c, bool write) {
         constexpr int SIGMA = decltype(sigma_c)::value;
         asm volatile("" : "+v"(base));
         float *b = s + base;
@@ -613,8 +662,42 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
         t0 = t0 + q_jp;
         t0 = t0 + q_km;
         t0 = t0 + q_kp;
-        const float target = divide_small(t0 - cm2 * rhs_c, t1[SIGMA], rcp[SIGMA]);
-        b[LJ * LK] = q_c + relaxation * (target - q_c);
+        const float target = divide_flat(t0 - cm2 * rhs_c, t1_c, rcp_c, ieee_c, !EDGE || write);
+        const float next = q_c + relaxation * (target - q_c);
+        if (!EDGE || write) b[LJ * LK] = next;
+    };
+    // One pair of colour c, class SIGMA: on an EDGE tile a lane whose row lies outside the level (or whose k
+    // does) writes nothing, and a wavefront holding a row on the level's boundary forms that row's t1 from
+    // its couplings (a t1 of 0, a cell coupled to nothing, leaves q as it is).
+    auto pair_update = [&](auto sigma_c, int c, int pair, int lds_row, float rhs_c) {
+        constexpr int SIGMA = decltype(sigma_c)::value;
+        float t = t1[SIGMA], r = rcp[SIGMA];
+        bool slow = ieee[SIGMA], write = true;
+        if constexpr (EDGE) {
+            int sg, lr, ii, jj, i, j;
+            decode(c, pair, sg, lr, ii, jj);
+            lane_row(c, SIGMA, ii, jj, i, j);
+            write = row_inside(i, j) && k_in[SIGMA];
+            const bool interior = i > 0 && i < n.n0 - 1 && j > 0 && j < n.n1 - 1;
+            if (__builtin_amdgcn_ballot_w64(write && !interior) != 0) {
+                if (!interior) {
+                    const float cx_lo = i > 0 ? 1.0f : open_lo[0] ? ghost : 0.0f;
+                    const float cx_hi = i < n.n0 - 1 ? 1.0f : open_hi[0] ? ghost : 0.0f;
+                    const float cy_lo = j > 0 ? 1.0f : open_lo[1] ? ghost : 0.0f;
+                    const float cy_hi = j < n.n1 - 1 ? 1.0f : open_hi[1] ? ghost : 0.0f;
+                    float sum = 0.0f + cx_lo;
+                    sum = sum + cx_hi;
+                    sum = sum + cy_lo;
+                    sum = sum + cy_hi;
+                    sum = sum + cz_lo[SIGMA];
+                    t = sum + cz_hi[SIGMA];
+                    r = t == 4.0f ? 0.25f : t == 5.0f ? 0.2f : 1.0f / 6.0f;
+                    slow = !(t == 4.0f || t == 5.0f || t == 6.0f);
+                    write = write && t != 0.0f;
+                }
+            }
+        }
+        update(sigma_c, lds_row * LK + lds_lane[c][SIGMA], rhs_c, t, r, slow, write);
     };
     // Iteration u of a colour takes pairs u WAVES .. u WAVES + WAVES - 1 (one per wavefront), and pairs are
     // numbered class 0 first: an iteration whose pairs all lie in one class, or all exist, knows it at compile
@@ -622,23 +705,25 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
     static_for<RED_ITERS>([&](auto u_c) {
         constexpr int U = decltype(u_c)::value, FIRST = U * WAVES, LAST = FIRST + WAVES - 1;
         if (LAST >= RED_PAIRS && wave + FIRST >= RED_PAIRS) return;
-        const int row = red_row[U] * LK;
-        if constexpr (LAST < RED_CLASS) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[U]);
-        else if constexpr (FIRST >= RED_CLASS) update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[U]);
-        else if (red_sigma[U] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[0][0], rhs_red[U]);
-        else update(std::integral_constant<int, 1>{}, row + lds_lane[0][1], rhs_red[U]);
+        const int pair = wave + FIRST;
+        if constexpr (LAST < RED_CLASS) pair_update(std::integral_constant<int, 0>{}, 0, pair, red_row[U], rhs_red[U]);
+        else if constexpr (FIRST >= RED_CLASS)
+            pair_update(std::integral_constant<int, 1>{}, 0, pair, red_row[U], rhs_red[U]);
+        else if (red_sigma[U] == 0) pair_update(std::integral_constant<int, 0>{}, 0, pair, red_row[U], rhs_red[U]);
+        else pair_update(std::integral_constant<int, 1>{}, 0, pair, red_row[U], rhs_red[U]);
     });
     edge();
     __syncthreads();
     static_for<BLACK_ITERS>([&](auto u_c) {
         constexpr int U = decltype(u_c)::value, FIRST = U * WAVES, LAST = FIRST + WAVES - 1;
         if (LAST >= BLACK_PAIRS && wave + FIRST >= BLACK_PAIRS) return;
-        const int row = black_row[U] * LK;
-        if constexpr (LAST < BLACK_CLASS) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[U]);
+        const int pair = wave + FIRST;
+        if constexpr (LAST < BLACK_CLASS)
+            pair_update(std::integral_constant<int, 0>{}, 1, pair, black_row[U], rhs_black[U]);
         else if constexpr (FIRST >= BLACK_CLASS)
-            update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[U]);
-        else if (black_sigma[U] == 0) update(std::integral_constant<int, 0>{}, row + lds_lane[1][0], rhs_black[U]);
-        else update(std::integral_constant<int, 1>{}, row + lds_lane[1][1], rhs_black[U]);
+            pair_update(std::integral_constant<int, 1>{}, 1, pair, black_row[U], rhs_black[U]);
+        else if (black_sigma[U] == 0) pair_update(std::integral_constant<int, 0>{}, 1, pair, black_row[U], rhs_black[U]);
+        else pair_update(std::integral_constant<int, 1>{}, 1, pair, black_row[U], rhs_black[U]);
     });
     __syncthreads();
     constexpr int OUT_ROWS = TI * TJ / WAVES;
@@ -651,6 +736,7 @@ __device__ __forceinline__ void sweep_tile_interior(const LevelArgs &L, float *_
 #pragma unroll
     for (int u = 0; u < OUT_ROWS; u++) {
         const int row = wave + u * WAVES;
+        if (EDGE && !(i0 + row / TJ < n.n0 && j0 + row % TJ < n.n1 && k_load)) continue;
         const uint32_t off = (uint32_t)(g_tile + (row / TJ) * n12 + (row % TJ) * n.n2) * B + lane_b;
         *reinterpret_cast<T *>(reinterpret_cast<char *>(q_out) + off) = from_f32<T>(v[u]);
     }
@@ -670,8 +756,8 @@ template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T = float>
 __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T *__restrict__ q_in,
                                                            T *__restrict__ q_out, const T *__restrict__ rhs,
                                                            const T *__restrict__ coarse, Dims nc,
-                                                           float relaxation, const double *gate, int tiles_k,
-                                                           int tiles_j) {
+                                                           float relaxation, const double *gate, FastDiv tiles_k,
+                                                           FastDiv tiles_j) {
     constexpr int WAVES = THREADS / 64;
     constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
     constexpr int RI = TI + 2, RJ = TJ + 2;
@@ -694,7 +780,7 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
     }
     const Dims n = cell_dims(L.g);
     const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    const int tk = tile % tiles_k, tr = tile / tiles_k, tj = tr % tiles_j, ti = tr / tiles_j;
+    const int tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d, ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
     const int k0 = tk * TK, j0 = tj * TJ, i0 = ti * TI;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int half_lane = lane & 31, upper = lane >> 5;
@@ -738,14 +824,17 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         float &cell = at(i, j, k);
         cell = gs_update(L, i, j, k, cell, rhs_c, t0, t1, relaxation);
     };
-    if (UNIT && i0 >= TH && i0 + TI + TH <= n.n0 && j0 >= TH && j0 + TJ + TH <= n.n1 && k0 + TK <= n.n2) {
-        sweep_tile_interior<TI, TJ, THREADS, START, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0, k0,
-                                                       [&]() {
-                                                           int i, j, k;
-                                                           if (red_edge(threadIdx.x, i, j, k)) {
-                                                               update(i, j, k, to_f32(rhs[n.at(i, j, k)]));
-                                                           }
-                                                       });
+    if (UNIT || L.flat == 1) {
+        auto edge_cells = [&]() {
+            int i, j, k;
+            if (red_edge(threadIdx.x, i, j, k)) update(i, j, k, to_f32(rhs[n.at(i, j, k)]));
+        };
+        if (i0 >= TH && i0 + TI + TH <= n.n0 && j0 >= TH && j0 + TJ + TH <= n.n1 && k0 + TK <= n.n2)
+            sweep_tile_flat<TI, TJ, THREADS, START, false, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0,
+                                                              k0, edge_cells);
+        else
+            sweep_tile_flat<TI, TJ, THREADS, START, true, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0,
+                                                             k0, edge_cells);
         return;
     }
     float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS], rhs_edge = 0.0f;
@@ -857,7 +946,7 @@ constexpr int DEFAULT_TILE_SHAPE = 0;
 template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T>
 void launch_tile_kernel(dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const LevelArgs &L, const T *src,
                         T *dst, const T *rhs, const T *coarse, Dims nc, float relaxation, const double *gate,
-                        int tiles_k, int tiles_j) {
+                        FastDiv tiles_k, FastDiv tiles_j) {
     if (e0 != nullptr)
         hipExtLaunchKernelGGL(k_rb_sweep_tile<TI, TJ, THREADS, UNIT, START, T>, grid, dim3(THREADS), 0, s, e0, e1, 0u,
                               L, src, dst, rhs, coarse, nc, relaxation, gate, tiles_k, tiles_j);
@@ -869,7 +958,7 @@ void launch_tile_kernel(dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, 
 template <bool UNIT, int START, typename T>
 void launch_tile_shape(int shape, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const LevelArgs &L,
                        const T *src, T *dst, const T *rhs, const T *coarse, Dims nc, float relaxation,
-                       const double *gate, int tiles_k, int tiles_j) {
+                       const double *gate, FastDiv tiles_k, FastDiv tiles_j) {
 #define PLM_TILE(TI, TJ, THREADS)                                                                                    \
     launch_tile_kernel<TI, TJ, THREADS, UNIT, START, T>(grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation,  \
                                                         gate, tiles_k, tiles_j)
@@ -1171,20 +1260,10 @@ constexpr size_t TAIL_LDS_BYTES = 150 * 1024;
 
 // Division by a level extent through a multiply-high: m = ceil(2^32 / d) gives floor(t / d) exactly for
 // t, d < 2^16 (the error t (m d - 2^32) / (d 2^32) stays below 1 / d).
-struct FastDiv {
-    uint32_t d, m;
-    __device__ __forceinline__ int operator()(int t) const { return d == 1 ? t : (int)__umulhi((uint32_t)t, m); }
-};
 
 // Slots of the colour enumeration on a level: n0 n1 ceil(n2 / 2); a tail thread plans a fixed number of them
 int colour_slots(const plm_grid_info &g) { return g.dims[0] * g.dims[1] * ((g.dims[2] + 1) / 2); }
 
-FastDiv fast_div(int d) {
-    FastDiv f;
-    f.d = (uint32_t)d;
-    f.m = d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d);
-    return f;
-}
 
 // Each tail level lives in LDS zero-bordered: its q, rhs and (for the level being restricted) residual as
 // (n0 + 2) (n1 + 2) (n2 + 2) arrays, cell (i, j, k) at ((i + 1)(n1 + 2) + j + 1)(n2 + 2) + k + 1. The border of
@@ -1756,7 +1835,6 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_norm_finish(double *scratch,
 
 int launch_norms(const LevelArgs &L, const float *q, const float *rhs, double *scratch, const double *gate,
                  double tolerance, int mode, int32_t *cycles_run, hipStream_t s) {
-    const Dims n = cell_dims(L.g);
     if (L.unit)
         k_norm_partials<true><<<norm_grid(L.g), block3(), 0, s>>>(L, q, rhs, scratch + 4, gate);
     else
@@ -1774,6 +1852,7 @@ int launch_tile(const LevelArgs &L, int start, const T *src, T *dst, const T *rh
     const TileShape &ts = TILE_SHAPES[shape];
     const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + ts.tj - 1) / ts.tj, tiles_i = (n.n0 + ts.ti - 1) / ts.ti;
     const dim3 grid((unsigned)(tiles_k * tiles_j * tiles_i));
+    const FastDiv div_k = fast_div(tiles_k), div_j = fast_div(tiles_j);
     // family "rb_sweep_tile_top": the plain FP32 two-colour sweep (q in, q out) on the simulation grid
     TimingScope launch_timing(!UNIT || sizeof(T) != 4 ? "rb_sweep_tile"
                               : start == FROM_Q      ? "rb_sweep_tile_top"
@@ -1782,13 +1861,13 @@ int launch_tile(const LevelArgs &L, int start, const T *src, T *dst, const T *rh
     const hipEvent_t e0 = launch_timing.start(), e1 = launch_timing.stop();
     if (start == FROM_ZERO)
         launch_tile_shape<UNIT, FROM_ZERO, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
-                                             tiles_k, tiles_j);
+                                             div_k, div_j);
     else if (start == FROM_PROLONG)
         launch_tile_shape<UNIT, FROM_PROLONG, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
-                                             tiles_k, tiles_j);
+                                             div_k, div_j);
     else
         launch_tile_shape<UNIT, FROM_Q, T>(shape, grid, s, e0, e1, L, src, dst, rhs, coarse, nc, relaxation, gate,
-                                             tiles_k, tiles_j);
+                                             div_k, div_j);
     return check_launch("rb sweep tile");
 }
 
