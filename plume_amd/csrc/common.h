@@ -142,7 +142,10 @@ template <> __device__ __forceinline__ __half from_f32<__half>(float x) { return
 // Element offset (i n1 + j) n2 + k with full-rate 24-bit multiplies: validate_dims keeps (nx + 1)(ny + 1) below
 // 2^24 and every field below 2^31 / 3 elements, so the products and the byte offsets stay exact in 32 bits.
 __device__ __forceinline__ uint32_t offset_of(Dims n, int i, int j, int k) {
-    return __umul24(__umul24((uint32_t)i, (uint32_t)n.n1) + (uint32_t)j, (uint32_t)n.n2) + (uint32_t)k;
+    uint32_t row = __umul24((uint32_t)i, (uint32_t)n.n1);
+    // kept a 24-bit product: fused with the add it becomes a quarter-rate 64-bit multiply-add
+    asm volatile("" : "+v"(row));
+    return __umul24(row + (uint32_t)j, (uint32_t)n.n2) + (uint32_t)k;
 }
 
 // f at a 32-bit byte offset from the field's base (the base stays scalar; one vector add per tap)
@@ -159,20 +162,31 @@ struct Corner {
     uint32_t o, di, dj, dk;
     float tx, ty, tz;
 };
+// clamp_(g, 0, hi) for hi = n - 1 >= 0 in two instructions: fmax_(g, 0) is g or +0.0, never -0.0 or NaN, and
+// non-negative floats order as their bit patterns do, so the unsigned minimum of the patterns is the select
+// fmin_ makes, bit for bit (and needs no NaN quieting of its operands, as the float minimum would).
+__device__ __forceinline__ float clamp_index(float g, float hi) {
+    return __uint_as_float(min(__float_as_uint(fmax_(g, 0.0f)), __float_as_uint(hi)));
+}
+
+// x - (float)(int)x for a clamped coordinate 0 <= x < 2^24: x - floor(x), exact, which v_fract_f32 computes in
+// one instruction.
+__device__ __forceinline__ float index_weight(float x) { return __builtin_amdgcn_fractf(x); }
+
 template <uint32_t B>
 __device__ __forceinline__ Corner corner_of(Dims n, float gx, float gy, float gz) {
-    const float x = clamp_(gx, 0.0f, (float)(n.n0 - 1));
-    const float y = clamp_(gy, 0.0f, (float)(n.n1 - 1));
-    const float z = clamp_(gz, 0.0f, (float)(n.n2 - 1));
+    const float hx = (float)(n.n0 - 1), hy = (float)(n.n1 - 1), hz = (float)(n.n2 - 1);
+    const float x = clamp_index(gx, hx), y = clamp_index(gy, hy), z = clamp_index(gz, hz);
     const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
     Corner c;
     c.o = offset_of(n, i0, j0, k0) * B;
-    c.di = i0 + 1 < n.n0 ? (uint32_t)(n.n1 * n.n2) * B : 0u;
-    c.dj = j0 + 1 < n.n1 ? (uint32_t)n.n2 * B : 0u;
-    c.dk = k0 + 1 < n.n2 ? B : 0u;
-    c.tx = x - (float)i0;
-    c.ty = y - (float)j0;
-    c.tz = z - (float)k0;
+    // i0 + 1 < n exactly when x < n - 1 (x <= n - 1, and n - 1 is a whole number)
+    c.di = x < hx ? (uint32_t)(n.n1 * n.n2) * B : 0u;
+    c.dj = y < hy ? (uint32_t)n.n2 * B : 0u;
+    c.dk = z < hz ? B : 0u;
+    c.tx = index_weight(x);
+    c.ty = index_weight(y);
+    c.tz = index_weight(z);
     return c;
 }
 

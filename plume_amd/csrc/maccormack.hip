@@ -37,14 +37,15 @@ template <typename B> __device__ __forceinline__ float lerp_zero(float a, B b) {
 // range, so their weight is 0.
 template <int MASK>
 __device__ __forceinline__ float sample_aligned(const float *__restrict__ f, Dims n, float gx, float gy, float gz) {
-    const float x = (MASK & 1) ? gx : clamp_(gx, 0.0f, (float)(n.n0 - 1));
-    const float y = (MASK & 2) ? gy : clamp_(gy, 0.0f, (float)(n.n1 - 1));
-    const float z = (MASK & 4) ? gz : clamp_(gz, 0.0f, (float)(n.n2 - 1));
+    const float hx = (float)(n.n0 - 1), hy = (float)(n.n1 - 1), hz = (float)(n.n2 - 1);
+    const float x = (MASK & 1) ? gx : clamp_index(gx, hx);
+    const float y = (MASK & 2) ? gy : clamp_index(gy, hy);
+    const float z = (MASK & 4) ? gz : clamp_index(gz, hz);
     const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
     const uint32_t o = offset_of(n, i0, j0, k0) * 4u;
-    const uint32_t di = i0 + 1 < n.n0 ? (uint32_t)(n.n1 * n.n2) * 4u : 0u, dj = j0 + 1 < n.n1 ? (uint32_t)n.n2 * 4u : 0u;
-    const uint32_t dk = k0 + 1 < n.n2 ? 4u : 0u;
-    const float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
+    const uint32_t di = x < hx ? (uint32_t)(n.n1 * n.n2) * 4u : 0u, dj = y < hy ? (uint32_t)n.n2 * 4u : 0u;
+    const uint32_t dk = z < hz ? 4u : 0u;
+    const float tx = index_weight(x), ty = index_weight(y), tz = index_weight(z);
     auto xl = [&](uint32_t base) -> float {
         const float a = load_bytes(f, base);
         if (MASK & 1) return lerp_zero(a, [&] { return load_bytes(f, base + di); });
