@@ -299,10 +299,14 @@ def test_fused_sweep_on_large_grids_is_bit_exact(device, oracle32, cells):
                                                   (0, (40, 36, 128), ALL_CLOSED), (1, (90, 90, 127), DEFAULT_FACES),
                                                   (1, (90, 90, 127), ALL_CLOSED), (0, (70, 66, 96), ALL_OPEN),
                                                   (1, (64, 96, 64), DEFAULT_FACES), (1, (96, 64, 160), ALL_OPEN),
-                                                  (2, (128, 64, 128), ALL_CLOSED), (1, (96, 64, 160), ALL_CLOSED)],
+                                                  (2, (128, 64, 128), ALL_CLOSED), (1, (96, 64, 160), ALL_CLOSED),
+                                                  (0, (37, 30, 128), DEFAULT_FACES), (1, (70, 60, 128), DEFAULT_FACES),
+                                                  (1, (40, 34, 128), ALL_CLOSED), (0, (19, 13, 256), ALL_OPEN),
+                                                  (0, (21, 18, 64), ALL_CLOSED)],
                          ids=["top", "coarse", "top-closed-k-full", "coarse-k-full-open", "coarse-k-full-closed",
                               "top-open-k-partial", "flat-k32-default", "flat-k80-open", "flat-corners-closed",
-                              "flat-k80-closed"])
+                              "flat-k80-closed", "k128-top", "k64-flat-ghost", "k64-flat-closed", "k256-open",
+                              "k64-top-closed"])
 def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level, cells, faces):
     """The autotuner may pick any tile shape; each gives the reference's sweep, bit for bit (a coarse level
     exercises the overhang couplings; 128 cells along k with closed faces give tiles whose 64 k cells all
@@ -310,7 +314,8 @@ def test_every_sweep_tile_shape_is_bit_exact(device, oracle32, level, cells, fac
     along k with fill 0.5, so the general-coupling path meets the last cell, its neighbour, and a ghost or
     closed face). Levels with fill (1, 1, 1) below the top take the flat path with ghost couplings other than
     1 on open faces (Σc 4 + 1 / 0.75 and the like: the IEEE division), fewer than 64 cells along k (32) or a
-    last k tile of 16 cells (80), and corner cells whose three closed faces leave Σc = 3."""
+    last k tile of 16 cells (80), and corner cells whose three closed faces leave Σc = 3; levels of whole 64-cell
+    k tiles (64, 128, 256) with tiles that overhang the level's end in i and j, under every face kind."""
     from oracle.sim import level_infos
     from plume_amd._lib import LIB, call, int3, ptr, stream
     from plume_amd.sim.multigrid import build_levels
