@@ -276,15 +276,16 @@ def ensemble_bench(args) -> dict:
     and 128x192x128, emitter radii 4-12 cells, alpha in [0.05, 0.2] and epsilon in [0, 0.5]. The 12 emitting
     frames and one more run eagerly; then the same number of frames is timed twice from the state reached:
     batched (one captured HIP graph launch per two substeps of all 32 members, the members on 8 streams as
-    concurrent graph branches) and plain (every member stepped eagerly, one after the other, as a single-grid
-    run does)."""
+    concurrent graph branches, the solves of the members up to --ensemble-batch-cells cells as one batched
+    multigrid) and plain (every member stepped eagerly, one after the other, as a single-grid run does)."""
     import torch
 
     from plume_amd.sim.device import init_device
     from plume_amd.sim.ensemble import Ensemble, ensemble_tables
 
     device = init_device("cuda")
-    ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device)
+    ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device,
+                   batch_cells=args.ensemble_batch_cells or None)
     ens.autotune()
     cells = ens.cells
     print(f"[bench] ensemble of {len(ens.members)}: {cells} cells", file=sys.stderr, flush=True)
@@ -292,6 +293,7 @@ def ensemble_bench(args) -> dict:
         ens.run_frame(replay=False)
     ens.run_frame(replay=True)  # captures the graph
     torch.cuda.synchronize()
+    print(f"[bench] batched multigrid tiles per depth: {ens.batch_tiles}", file=sys.stderr, flush=True)
 
     def timed(replay: bool) -> tuple[float, int]:
         steps = 0
@@ -318,6 +320,8 @@ def ensemble_bench(args) -> dict:
             "batched": {"cell_substeps_per_s": batched, "seconds": batched_s, "substeps": batched_steps},
             "plain": {"cell_substeps_per_s": plain, "seconds": plain_s, "substeps": plain_steps},
             "speedup": batched / plain, "frames_timed": args.ensemble_frames,
+            "batched_multigrid": {"members": len(ens.batched), "max_cells": args.ensemble_batch_cells or None,
+                                  "tiles_per_depth": [shape for shape, _ in ens.batch_tiles]},
             "max_final_relative_residual": max(residuals),
             "data": "synthetic (seeded puff recipes; frames 13 onwards, after the emitters stop)"}
 
@@ -412,6 +416,8 @@ def main() -> int:
                         help="run the 32-member batched ensemble benchmark instead and exit")
     parser.add_argument("--ensemble-members", type=int, default=32)
     parser.add_argument("--ensemble-frames", type=int, default=4)
+    parser.add_argument("--ensemble-batch-cells", type=int, default=1_500_000,
+                        help="members of at most this many cells share one batched multigrid (0: every member)")
     parser.add_argument("--mixed", action="store_true",
                         help="run the 256x384x256 mixed-precision projection benchmark instead and exit")
     parser.add_argument("--mixed-tolerance", type=float, default=1e-6)

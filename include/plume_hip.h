@@ -258,6 +258,36 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
                double *norm_scratch, int32_t *cycles_run, void *stream);
 int64_t plm_norm_scratch_doubles(const plm_grid_info *info);
 
+/* Batched multigrid for an ensemble of independent solves of uneven size (no reference counterpart: the
+   reference solves one grid per run, sim/run.py:57-58; this serves the batched ensemble). Each member is a
+   plm_vcycle level list (level 0 its simulation grid), with its own norm_scratch (zero-filled doubles,
+   plm_norm_scratch_doubles of its level 0), cycles_run (device int32[2]) and cap on cycles. A solve checks
+   every member's FP64 relative residual, then runs up to `cycles` V(sweeps, sweeps) cycles in lockstep, each
+   stage of a cycle one launch for all members (a persistent grid of blocks over the members' tiles, blocks of
+   one workgroup per member for the LDS tails), and the norms after each cycle. A member stops once its residual
+   is at or below `tolerance` or it has run max_cycles[m] cycles, so each member ends exactly as its own
+   plm_vcycle(tolerance, max_cycles) would (bit for bit) when `cycles` covers what it needs; otherwise its gate
+   (norm_scratch[1]) is still 1 afterwards. create validates, builds the stage tables and copies them to the
+   device (the only allocation: freed by destroy); solve allocates nothing and reads nothing back (graph-safe).
+   Requirements: 1 to 64 members, tolerance > 0, sweeps even, every member's coarse levels form an LDS tail.
+   The arrays must stay where they are for the batch's lifetime. */
+typedef struct plm_mg_batch plm_mg_batch;
+int plm_mg_batch_create(int32_t members, const int32_t *n_levels, const plm_grid_info *const *infos,
+                        float *const *const *q, float *const *const *rhs, float *const *const *residual,
+                        int32_t sweeps, double tolerance, const int32_t *max_cycles, double *const *norm_scratch,
+                        int32_t *const *cycles_run, plm_mg_batch **out);
+int plm_mg_batch_solve(plm_mg_batch *batch, int32_t cycles, void *stream);
+int plm_mg_batch_destroy(plm_mg_batch *batch);
+/* The batch's autotuner (project-written; the sweep is bit-identical under every shape): at each depth of the
+   V-cycle that runs tile sweeps, times every tile shape for a launch over all members (2 * repeats launches of
+   a first sweep from 0 into the members' scratch levels, which are overwritten; q is not touched), writes the
+   microseconds into times_us[depth * shapes + shape] (may be NULL; plm_mg_batch_depth(batch) depths and
+   plm_sweep_tile_shapes shapes), keeps the fastest shape per depth and rebuilds the batch's table.
+   Synchronises the stream; not for graph capture. */
+int plm_mg_batch_autotune(plm_mg_batch *batch, int32_t repeats, float *times_us, void *stream);
+/* Depths of the batch that run tile sweeps (the largest first level of any member's LDS tail); -1 for NULL. */
+int32_t plm_mg_batch_depth(const plm_mg_batch *batch);
+
 /* |rhs - A q|^2 and |rhs|^2 of one level in FP64, deterministic (fixed-order tree), into out[0..1]
    (device). For the convergence tests of sim/multigrid.py's V-cycle. */
 int plm_residual_norms(const plm_grid_info *info, const float *q, const float *rhs, double *scratch, double *out,

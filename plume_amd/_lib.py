@@ -101,6 +101,10 @@ SIGNATURES = {
     "plm_prolong_add": [_P, _P, _P, _P, _P],
     "plm_zero": [_L, _P, _P],
     "plm_vcycle": [_I, _P, _P, _P, _P, _I, _I, _D, _I, _P, _P, _P],
+    "plm_mg_batch_create": [_I, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, ctypes.POINTER(_P)],
+    "plm_mg_batch_solve": [_P, _I, _P],
+    "plm_mg_batch_destroy": [_P],
+    "plm_mg_batch_autotune": [_P, _I, _P, _P],
     "plm_residual_norms": [_INFO, _P, _P, _P, _P, _P],
     "plm_vcycle_lp": [_I, _P, _I, _P, _P, _P, _I, _I, _I, _P],
     "plm_to_storage": [_L, _P, _F, _I, _P, _P],
@@ -145,6 +149,8 @@ def _load() -> ctypes.CDLL:
     lib.plm_norm_scratch_doubles.argtypes = [_INFO]
     lib.plm_adj_raymarch_record_stride.restype = ctypes.c_int32
     lib.plm_adj_raymarch_record_stride.argtypes = [_INFO, _I]
+    lib.plm_mg_batch_depth.restype = ctypes.c_int32
+    lib.plm_mg_batch_depth.argtypes = [_P]
     lib.plm_sweep_tile_shapes.restype = ctypes.c_int32
     lib.plm_sweep_tile_shapes.argtypes = [_P, _P, _P, _I]
     for name, argtypes in SIGNATURES.items():

@@ -14,11 +14,15 @@
 //   - k_mg_tail: every level from the first with few enough cells down to the coarsest, solved by one
 //     1024-thread workgroup in LDS, instead of ~90 launches per cycle at 128 x 192 x 128.
 // Multigrid kernels take an optional device gate (tolerance mode): when gate[0] == 0 they return at once.
+#include <algorithm>
 #include <array>
+#include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -752,12 +756,13 @@ __device__ __forceinline__ void sweep_tile_flat(const LevelArgs &L, float *__res
 // written out. The rhs values both passes need are loaded into registers before the first barrier. T is the
 // stored type of q, rhs and the coarse correction (FP32, or BF16 / FP16 in the mixed-precision solve); the
 // tile and all arithmetic are FP32.
-template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T = float>
-__global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T *__restrict__ q_in,
-                                                           T *__restrict__ q_out, const T *__restrict__ rhs,
-                                                           const T *__restrict__ coarse, Dims nc,
-                                                           float relaxation, const double *gate, FastDiv tiles_k,
-                                                           FastDiv tiles_j) {
+// The tile sweep of tile number `tile` (k fastest, then j, then i) with the block's LDS: s holds (TI + 4) x
+// (TJ + 4) rows of LK floats, cps the coupling table (levels with couplings other than 1).
+template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T>
+__device__ __forceinline__ void sweep_tile(const LevelArgs &L, const T *__restrict__ q_in, T *__restrict__ q_out,
+                                           const T *__restrict__ rhs, const T *__restrict__ coarse, Dims nc,
+                                           float relaxation, int tile, FastDiv tiles_k, FastDiv tiles_j, float *s,
+                                           float *cps) {
     constexpr int WAVES = THREADS / 64;
     constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
     constexpr int RI = TI + 2, RJ = TJ + 2;
@@ -765,9 +770,6 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
     constexpr int BLACK_ITERS = TI * TJ / 2 / WAVES;
     static_assert(THREADS % 64 == 0 && (TI * TJ) % (2 * WAVES) == 0, "tile rows split evenly over wavefronts");
     static_assert(RI * RJ <= THREADS, "one thread per ring row for the k-halo red cells");
-    if (gated_off(gate)) return;
-    __shared__ float s[LI * LJ * LK];
-    __shared__ float cps[UNIT ? 1 : CP_WORDS];  // the coupling table (levels with couplings other than 1)
     if (!UNIT && threadIdx.x == 0) {
         // constant indices: read straight from the kernel arguments
         const float *table = &L.cp.in[0][0][0];
@@ -779,7 +781,6 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
         cps[CP_ZERO] = 0.0f;
     }
     const Dims n = cell_dims(L.g);
-    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
     const int tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d, ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
     const int k0 = tk * TK, j0 = tj * TJ, i0 = ti * TI;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -933,6 +934,21 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
     }
 }
 
+// One two-colour sweep over a TI x TJ x 64 tile per block (sweep_tile), tiles renumbered so that each XCD runs
+// one contiguous range of them.
+template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T = float>
+__global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T *__restrict__ q_in,
+                                                           T *__restrict__ q_out, const T *__restrict__ rhs,
+                                                           const T *__restrict__ coarse, Dims nc,
+                                                           float relaxation, const double *gate, FastDiv tiles_k,
+                                                           FastDiv tiles_j) {
+    if (gated_off(gate)) return;
+    __shared__ float s[(TI + 2 * TH) * (TJ + 2 * TH) * LK];
+    __shared__ float cps[UNIT ? 1 : CP_WORDS];
+    sweep_tile<TI, TJ, THREADS, UNIT, START, T>(L, q_in, q_out, rhs, coarse, nc, relaxation,
+                                                xcd_contiguous(blockIdx.x, gridDim.x), tiles_k, tiles_j, s, cps);
+}
+
 // The tile shapes the autotuner chooses among: TI, TJ (k is 64 wide), threads per block.
 struct TileShape {
     int ti, tj, threads;
@@ -1052,14 +1068,13 @@ __global__ __launch_bounds__(256) void k_restrict(const float *__restrict__ fine
 
 // residual_kernel + restrict_kernel fused: each coarse cell forms its children's residuals in registers.
 // coarse_q (may be NULL) is set to 0, the reference's coarse.q.zero_() (sim/multigrid.py:143).
-template <bool UNIT, typename T = float>
-__global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T *__restrict__ q,
-                                                           const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
-                                                           T *__restrict__ coarse_q, Dims nc, const double *gate) {
-    if (gated_off(gate)) return;
+template <bool UNIT, typename T>
+__device__ __forceinline__ void residual_restrict_cell(const LevelArgs &L, const T *__restrict__ q,
+                                                       const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
+                                                       T *__restrict__ coarse_q, Dims nc, dim3 block) {
     const Dims nf = cell_dims(L.g);
-    int i, j, k;
-    if (!thread_cell(nc, i, j, k)) return;
+    const int k = block.x * BX + threadIdx.x, j = block.y * BY + threadIdx.y, i = block.z;
+    if (!(k < nc.n2 && j < nc.n1 && i < nc.n0)) return;
     float total = 0.0f;
     int count = 0;
     if constexpr (UNIT) {
@@ -1148,6 +1163,15 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
     if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
 }
 
+// residual_kernel + restrict_kernel (sim/multigrid.py:36-60), one thread per coarse cell (grid3 of the coarse level)
+template <bool UNIT, typename T = float>
+__global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T *__restrict__ q,
+                                                           const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
+                                                           T *__restrict__ coarse_q, Dims nc, const double *gate) {
+    if (gated_off(gate)) return;
+    residual_restrict_cell<UNIT, T>(L, q, rhs, coarse_rhs, coarse_q, nc, blockIdx);
+}
+
 // k_residual_restrict for levels with couplings other than 1, with one thread per child: lanes 8 m + 4 di +
 // 2 dj + dk hold the children of coarse cell m of the wavefront. A child whose six neighbours are inside the
 // level and none of them (nor the child) the last cell on an axis has couplings 1, a sum of couplings of 6
@@ -1156,14 +1180,12 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
 // +0.0, which changes no bit) and divides by the count. Eight short threads instead of one thread walking
 // eight children keep the small levels' latency hidden.
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L, const T *__restrict__ q,
-                                                                    const T *__restrict__ rhs,
-                                                                    T *__restrict__ coarse_rhs, Dims nc,
-                                                                    const double *gate) {
-    if (gated_off(gate)) return;
+__device__ __forceinline__ void residual_restrict_children(const LevelArgs &L, const T *__restrict__ q,
+                                                           const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
+                                                           Dims nc, uint32_t block) {
     const Dims nf = cell_dims(L.g);
     const int child = threadIdx.x & 7;
-    const int cell = (int)(blockIdx.x * 32u + (threadIdx.x >> 3));
+    const int cell = (int)(block * 32u + (threadIdx.x >> 3));
     const bool valid = cell < nc.count();
     const int row = cell / nc.n2, k = cell - row * nc.n2, i = row / nc.n1, j = row - i * nc.n1;
     const int fi = 2 * i + (child >> 2), fj = 2 * j + ((child >> 1) & 1), fk = 2 * k + (child & 1);
@@ -1198,6 +1220,15 @@ __global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L,
     if (child == 0 && valid) coarse_rhs[cell] = from_f32<T>(total / (float)count);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L, const T *__restrict__ q,
+                                                                    const T *__restrict__ rhs,
+                                                                    T *__restrict__ coarse_rhs, Dims nc,
+                                                                    const double *gate) {
+    if (gated_off(gate)) return;
+    residual_restrict_children<T>(L, q, rhs, coarse_rhs, nc, blockIdx.x);
+}
+
 inline dim3 children_grid(const Dims &nc) { return dim3((unsigned)((nc.count() + 31) / 32)); }
 
 // prolong_add_kernel, sim/multigrid.py:63-68
@@ -1214,11 +1245,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ c
 // uniform and only the k part varies per lane (the operations of sample_index, so the same values). T is the
 // stored type (the sum is formed in FP32 and rounded once into T).
 template <typename T = float>
-__global__ __launch_bounds__(256) void k_prolong_rows(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine,
-                                                      Dims nf, const double *gate) {
-    if (gated_off(gate)) return;
-    const int k = blockIdx.x * BX + threadIdx.x;
-    const int j = uniform((int)(blockIdx.y * BY + threadIdx.y)), i = blockIdx.z;
+__device__ __forceinline__ void prolong_row_cell(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine, Dims nf,
+                                                 dim3 block) {
+    const int k = block.x * BX + threadIdx.x;
+    const int j = uniform((int)(block.y * BY + threadIdx.y)), i = block.z;
     if (j >= nf.n1 || k >= nf.n2) return;
     uint32_t pk0, pk1;
     float tz;
@@ -1226,6 +1256,13 @@ __global__ __launch_bounds__(256) void k_prolong_rows(const T *__restrict__ coar
     const float p = prolong_at_row(coarse, nc, i, j, pk0, pk1, tz);
     T *cell = reinterpret_cast<T *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, k) * (uint32_t)sizeof(T));
     *cell = from_f32<T>(to_f32(*cell) + p);
+}
+
+template <typename T = float>
+__global__ __launch_bounds__(256) void k_prolong_rows(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine,
+                                                      Dims nf, const double *gate) {
+    if (gated_off(gate)) return;
+    prolong_row_cell<T>(coarse, nc, fine, nf, blockIdx);
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t count, float *__restrict__ f, const double *gate) {
@@ -1602,12 +1639,10 @@ __device__ __forceinline__ int coarsest_sweeps(const TailLevel &L) {
 // ONES: every tail level has fill_hi (1, 1, 1) (its in-domain couplings are all exactly 1). The first
 // KEEP block levels build their plans once, on the way down, and keep them in registers for the way up.
 template <bool ONES>
-__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
-                                                          const float *__restrict__ rhs_top, int sweeps,
-                                                          const double *gate) {
+__device__ __forceinline__ void mg_tail(const TailLevels &args, float *__restrict__ q_top,
+                                        const float *__restrict__ rhs_top, int sweeps) {
     constexpr int K = ONES ? BLOCK_PLAN_ONES : BLOCK_PLAN;
     constexpr int KEEP = ONES ? 2 : 1;
-    if (gated_off(gate)) return;
     extern __shared__ float lds[];
     // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
     // be copied to per-thread scratch, so it is staged in LDS once.
@@ -1702,6 +1737,14 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float
     }
 }
 
+template <bool ONES>
+__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
+                                                          const float *__restrict__ rhs_top, int sweeps,
+                                                          const double *gate) {
+    if (gated_off(gate)) return;
+    mg_tail<ONES>(args, q_top, rhs_top, sweeps);
+}
+
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
 
 constexpr int NORM_THREADS = 256;
@@ -1729,17 +1772,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // (a neighbour outside the level reads the cell itself and is masked), so they issue as one batch; unit levels
 // then need no branch per cell.
 template <bool UNIT>
-__global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
-                                                                const float *__restrict__ rhs,
-                                                                double *__restrict__ partials, const double *gate) {
-    if (gated_off(gate)) return;
+__device__ __forceinline__ void norm_partials(const LevelArgs &L, const float *__restrict__ q,
+                                              const float *__restrict__ rhs, double *__restrict__ partials,
+                                              dim3 block, dim3 grid) {
     const Dims n = cell_dims(L.g);
-    const int k = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int k = block.x * BX + threadIdx.x, j = block.y * BY + threadIdx.y;
     double rr = 0.0, bb = 0.0;
     if (k < n.n2 && j < n.n1) {
         const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m on unit levels
         const uint32_t si = (uint32_t)(n.n1 * n.n2) * 4u, sj = (uint32_t)n.n2 * 4u;
-        const int ib = blockIdx.z * NORM_I;
+        const int ib = block.z * NORM_I;
         const bool jm = j >= 1, jp = j + 1 < n.n1, km = k >= 1, kp = k + 1 < n.n2;
         const int couples_jk = unit_couples(L.g, 1, jm, jp) + unit_couples(L.g, 2, km, kp);
         const uint32_t dj_lo = jm ? sj : 0u, dj_hi = jp ? sj : 0u, dk_lo = km ? 4u : 0u, dk_hi = kp ? 4u : 0u;
@@ -1788,16 +1830,24 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
     }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[2 * block] = ((srr[0] + srr[1]) + srr[2]) + srr[3];
-        partials[2 * block + 1] = ((sbb[0] + sbb[1]) + sbb[2]) + sbb[3];
+        const int64_t b = ((int64_t)block.z * grid.y + block.y) * grid.x + block.x;
+        partials[2 * b] = ((srr[0] + srr[1]) + srr[2]) + srr[3];
+        partials[2 * b + 1] = ((sbb[0] + sbb[1]) + sbb[2]) + sbb[3];
     }
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
+                                                                const float *__restrict__ rhs,
+                                                                double *__restrict__ partials, const double *gate) {
+    if (gated_off(gate)) return;
+    norm_partials<UNIT>(L, q, rhs, partials, blockIdx, gridDim);
 }
 
 // scratch layout: [0] relative residual, [1] gate (1 active / 0 converged), [2] |r|^2, [3] |rhs|^2,
 // [4 ...] 2 per block partials. `mode`: 0 plain norms (no gate change), 1 initial check, 2 after a cycle.
-__global__ __launch_bounds__(FINISH_THREADS) void k_norm_finish(double *scratch, int64_t blocks, double tolerance,
-                                                                int mode, int32_t *cycles_run) {
+__device__ __forceinline__ void norm_finish(double *scratch, int64_t blocks, double tolerance, int mode,
+                                            int32_t *cycles_run) {
     if (mode == 2 && scratch[1] == 0.0) return;
     const double *partials = scratch + 4;
     __shared__ double srr[FINISH_THREADS], sbb[FINISH_THREADS];
@@ -1829,6 +1879,11 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_norm_finish(double *scratch,
         cycles_run[1] = cycles_run[1] + 1;  // running total across solves, never reset here
     }
     scratch[1] = ratio > tolerance ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(FINISH_THREADS) void k_norm_finish(double *scratch, int64_t blocks, double tolerance,
+                                                                int mode, int32_t *cycles_run) {
+    norm_finish(scratch, blocks, tolerance, mode, cycles_run);
 }
 
 // ------------------------------------------------------------------ host orchestration
@@ -2297,10 +2352,645 @@ int vcycle_lp(int32_t n_levels, const plm_grid_info *infos, void *const *q, void
     return 0;
 }
 
+// The level table of one solve: LevelArgs per level, the arrays, and the LDS tail (the first level from which
+// every coarser level fits in LDS together, with its staged table and size). Host only; allocates nothing.
+int build_hierarchy(Hierarchy &h, int32_t n_levels, const plm_grid_info *infos, float *const *q, float *const *rhs,
+                    float *const *residual, int32_t sweeps, const double *gate) {
+    h = Hierarchy{};
+    h.n_levels = n_levels;
+    for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
+    h.q = q;
+    h.rhs = rhs;
+    h.residual = residual;
+    h.sweeps = sweeps;
+    h.gate = gate;
+    // the tail starts at the first level from which every coarser level fits in LDS together
+    // (and whose colour slots fit the tail's per-thread plans; levels are added coarsest first, so the one
+    // just added is the largest)
+    int total = 0;
+    bool ones = true;
+    h.tail_start = n_levels;
+    for (int l = n_levels - 1; l >= 0; l--) {
+        const plm_grid_info &g = infos[l];
+        const int cells = g.dims[0] * g.dims[1] * g.dims[2];
+        const bool with = ones && g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f;
+        const int plan = with ? BLOCK_PLAN_ONES : BLOCK_PLAN;
+        if (total + cells > PLM_LDS_CELLS || n_levels - l > MAX_TAIL || colour_slots(g) > plan * TAIL_THREADS) break;
+        total += cells;
+        ones = with;
+        h.tail_start = l;
+    }
+    // The tail's LDS: each level's q and rhs zero-bordered ((n0 + 2)(n1 + 2)(n2 + 2) floats each) and the padded
+    // residual of the largest level that restricts; a tail that would not fit drops its largest level.
+    auto padded = [&](int l) {
+        return (infos[l].dims[0] + 2) * (infos[l].dims[1] + 2) * (infos[l].dims[2] + 2);
+    };
+    auto tail_floats = [&](int from) {
+        int floats = 0, restricted = 0;
+        for (int l = from; l < n_levels; l++) {
+            floats += 2 * padded(l);
+            if (l + 1 < n_levels) restricted = std::max(restricted, padded(l));
+        }
+        return floats + restricted;
+    };
+    while (h.tail_start < n_levels && sizeof(float) * (size_t)tail_floats(h.tail_start) > TAIL_LDS_BYTES)
+        h.tail_start++;
+    ones = true;
+    for (int l = h.tail_start; l < n_levels; l++)
+        ones = ones && infos[l].fill_hi[0] == 1.0f && infos[l].fill_hi[1] == 1.0f && infos[l].fill_hi[2] == 1.0f;
+    h.tail_ones = ones;
+    if (h.tail_start < n_levels) {
+        h.tail.count = n_levels - h.tail_start;
+        int offset = 0, restricted = 0;
+        for (int m = 0; m < h.tail.count; m++) {
+            h.tail.level[m] = h.level[h.tail_start + m];
+            const plm_grid_info &g = infos[h.tail_start + m];
+            h.tail.qoff[m] = offset;
+            h.tail.roff[m] = offset + padded(h.tail_start + m);
+            offset += 2 * padded(h.tail_start + m);
+            h.tail.div_n1[m] = fast_div(g.dims[1]);
+            h.tail.div_n2[m] = fast_div(g.dims[2]);
+            h.tail.div_half[m] = fast_div((g.dims[2] + 1) / 2);
+            if (m + 1 < h.tail.count) restricted = std::max(restricted, padded(h.tail_start + m));
+        }
+        h.tail.q_floats = offset;
+        h.tail.res_offset = offset;
+        h.tail_bytes = sizeof(float) * (size_t)(offset + restricted);
+        h.tail.wave_from = h.tail.count;
+        for (int m = h.tail.count - 1; m >= 0; m--) {
+            const plm_grid_info &g = infos[h.tail_start + m];
+            if (g.dims[0] * g.dims[1] * g.dims[2] > WAVE_CELLS || colour_slots(g) > WAVE) break;
+            h.tail.wave_from = m;
+        }
+        PLM_TRY(h.tail_ones ? allow_tail_lds<true>(h.tail_bytes) : allow_tail_lds<false>(h.tail_bytes));
+    }
+
+    return 0;
+}
+
+
+// ------------------------------------------------------------------ batched multigrid (ensemble members)
+
+// The V-cycles of many independent solves of uneven size, advanced together: each launch covers one stage of
+// the V-cycle (a pass of sweeps at one depth, a restriction, the tails, a prolongation, the norms) for every
+// member that has it, as a persistent grid of blocks walking a list of (member, tile) work units. A member's
+// own operations and their order are exactly those of plm_vcycle (the same kernel bodies, the prolongation
+// as the row kernel of the large levels, then plain sweeps), so each member's result is bit for bit that of
+// its own solve. Every unit first reads its member's gate: a member whose residual met its tolerance, or that
+// ran its cap of cycles, is skipped while the others go on.
+
+struct MemberGate {
+    const double *gate;     // norm_scratch + 1: 1 while the residual is above tolerance, 0 once it is not
+    const int32_t *cycles;  // cycles_run[0]: the cycles the current solve has run
+    int32_t cap;            // the member's most cycles
+    int32_t pad;
+};
+
+__device__ __forceinline__ bool member_off(const MemberGate &m) { return m.gate[0] == 0.0 || m.cycles[0] >= m.cap; }
+
+struct SweepJob {
+    LevelArgs L;
+    const float *q_in;
+    float *q_out;
+    const float *rhs;
+    FastDiv tiles_k, tiles_j;
+    int member, first;  // its gate, and its first work unit in the launch
+};
+
+// A stage whose work units are blocks of one of the grid-wide kernels: a restriction, a prolongation, norms
+struct UnitJob {
+    LevelArgs L;         // the level worked on (the fine level of a restriction or a prolongation)
+    const float *a;      // q (restriction, norms) or the coarse q (prolongation)
+    const float *b;      // rhs (restriction, norms)
+    float *out;          // the coarse rhs (restriction) or the fine q (prolongation)
+    double *scratch;     // the member's norm scratch (norms)
+    int32_t *cycles_run; // the member's cycle counters (norms)
+    int64_t blocks;      // the member's norm partials
+    Dims nc;             // the coarse level
+    int gx, gy, gz;      // the stage's blocks for this member
+    int member, first;
+};
+
+struct TailJob {
+    TailLevels tail;
+    float *q_top;
+    const float *rhs_top;
+    int member, first;
+};
+
+// The job holding work unit u: `firsts` holds each job's first unit in increasing order (at most 64 jobs), one
+// per lane in a single load, and the jobs that start at or before u are counted by a ballot.
+__device__ __forceinline__ int job_of(const int *firsts, int n_jobs, int u) {
+    const int lane = (int)__lane_id();
+    const int first = lane < n_jobs ? firsts[lane] : 0x7fffffff;
+    return __popcll(__builtin_amdgcn_ballot_w64(first <= u)) - 1;
+}
+
+// This block's units in a persistent launch: u = first, first + step, ... while u < end. The blocks of XCD x
+// (block b runs on XCD b mod 8) own the x-th eighth of the units and take them in turn, so the blocks running
+// together on one XCD work on neighbouring tiles (shared halos stay in that XCD's L2), and every block gets
+// nearly the same number of units.
+__device__ __forceinline__ void unit_range(int units, int &first, int &end, int &step) {
+    const int blocks = (int)gridDim.x, x = (int)blockIdx.x % XCDS;
+    step = blocks / XCDS + (x < blocks % XCDS ? 1 : 0);
+    first = (int)((int64_t)units * x / XCDS) + (int)blockIdx.x / XCDS;
+    end = (int)((int64_t)units * (x + 1) / XCDS);
+}
+
+// The unit before the first one of this block's stride at or past the next job (the loop's `u += step` then
+// lands there): a member that is off is skipped with one gate read per block
+__device__ __forceinline__ int skip_job(int u, int j, const int *firsts, int n_jobs, int end, int step) {
+    const int next = j + 1 < n_jobs ? firsts[j + 1] : end;
+    return u + ((next - u + step - 1) / step - 1) * step;
+}
+
+__device__ __forceinline__ dim3 unit_block(const UnitJob &J, int unit) {
+    const int v = unit - J.first;
+    return dim3((unsigned)(v % J.gx), (unsigned)((v / J.gx) % J.gy), (unsigned)(v / (J.gx * J.gy)));
+}
+
+template <int TI, int TJ, int THREADS, bool UNIT, int START>
+__global__ __launch_bounds__(THREADS) void k_batch_sweep(const SweepJob *jobs, const int *firsts, int n_jobs,
+                                                         int units, const MemberGate *gates) {
+    __shared__ float s[(TI + 2 * TH) * (TJ + 2 * TH) * LK];
+    __shared__ float cps[UNIT ? 1 : CP_WORDS];
+    int first, end, step;
+    unit_range(units, first, end, step);
+    for (int u = first; u < end; u += step) {
+        const int j = job_of(firsts, n_jobs, u);
+        const SweepJob &J = jobs[j];
+        if (member_off(gates[J.member])) {
+            u = skip_job(u, j, firsts, n_jobs, end, step);
+            continue;
+        }
+        sweep_tile<TI, TJ, THREADS, UNIT, START, float>(J.L, J.q_in, J.q_out, J.rhs, nullptr, Dims{1, 1, 1}, 1.0f,
+                                                        u - J.first, J.tiles_k, J.tiles_j, s, cps);
+        __syncthreads();  // the next tile reuses the LDS
+    }
+}
+
+// residual + restriction onto the coarse rhs: unit levels one thread per coarse cell (64 x 4 blocks), the rest
+// eight threads per coarse cell (256-thread blocks), as plm_vcycle launches them
+__global__ __launch_bounds__(256) void k_batch_restrict_unit(const UnitJob *jobs, const int *firsts, int n_jobs, int units,
+                                                             const MemberGate *gates) {
+    int first, end, step;
+    unit_range(units, first, end, step);
+    for (int u = first; u < end; u += step) {
+        const int j = job_of(firsts, n_jobs, u);
+        const UnitJob &J = jobs[j];
+        if (member_off(gates[J.member])) {
+            u = skip_job(u, j, firsts, n_jobs, end, step);
+            continue;
+        }
+        residual_restrict_cell<true, float>(J.L, J.a, J.b, J.out, nullptr, J.nc, unit_block(J, u));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_restrict_children(const UnitJob *jobs, const int *firsts, int n_jobs, int units,
+                                                                 const MemberGate *gates) {
+    int first, end, step;
+    unit_range(units, first, end, step);
+    for (int u = first; u < end; u += step) {
+        const int j = job_of(firsts, n_jobs, u);
+        const UnitJob &J = jobs[j];
+        if (member_off(gates[J.member])) {
+            u = skip_job(u, j, firsts, n_jobs, end, step);
+            continue;
+        }
+        residual_restrict_children<float>(J.L, J.a, J.b, J.out, J.nc, (uint32_t)(u - J.first));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_prolong(const UnitJob *jobs, const int *firsts, int n_jobs, int units,
+                                                       const MemberGate *gates) {
+    int first, end, step;
+    unit_range(units, first, end, step);
+    for (int u = first; u < end; u += step) {
+        const int j = job_of(firsts, n_jobs, u);
+        const UnitJob &J = jobs[j];
+        if (member_off(gates[J.member])) {
+            u = skip_job(u, j, firsts, n_jobs, end, step);
+            continue;
+        }
+        prolong_row_cell<float>(J.a, J.nc, J.out, cell_dims(J.L.g), unit_block(J, u));
+    }
+}
+
+// One member's tail per block
+template <bool ONES>
+__global__ __launch_bounds__(TAIL_THREADS) void k_batch_tail(const TailJob *jobs, const MemberGate *gates, int sweeps) {
+    const TailJob &J = jobs[blockIdx.x];
+    if (member_off(gates[J.member])) return;
+    mg_tail<ONES>(J.tail, J.q_top, J.rhs_top, sweeps);
+}
+
+// The FP64 norm partials of every member's simulation grid (`check`: skip the members that are off)
+__global__ __launch_bounds__(NORM_THREADS) void k_batch_norm_partials(const UnitJob *jobs, const int *firsts, int n_jobs, int units,
+                                                                      const MemberGate *gates, int check) {
+    int first, end, step;
+    unit_range(units, first, end, step);
+    for (int u = first; u < end; u += step) {
+        const int j = job_of(firsts, n_jobs, u);
+        const UnitJob &J = jobs[j];
+        if (check && member_off(gates[J.member])) {
+            u = skip_job(u, j, firsts, n_jobs, end, step);
+            continue;
+        }
+        norm_partials<true>(J.L, J.a, J.b, J.scratch + 4, unit_block(J, u), dim3(J.gx, J.gy, J.gz));
+        __syncthreads();  // the next unit reuses the per-wavefront sums
+    }
+}
+
+// One member's norm reduction per block, in `mode` (1 the solve's first check, 2 after a cycle)
+__global__ __launch_bounds__(FINISH_THREADS) void k_batch_norm_finish(const UnitJob *jobs, const MemberGate *gates,
+                                                                      double tolerance, int mode, int check) {
+    const UnitJob &J = jobs[blockIdx.x];
+    if (check && member_off(gates[J.member])) return;
+    norm_finish(J.scratch, J.blocks, tolerance, mode, J.cycles_run);
+}
+
+enum BatchKind { B_SWEEP, B_RESTRICT_UNIT, B_RESTRICT_CHILDREN, B_PROLONG, B_TAIL, B_NORM, B_FINISH };
+
+// One launch of the batched solve
+struct BatchStage {
+    int kind = B_SWEEP;
+    int shape = 0, unit = 1, start = FROM_Q;  // sweeps: tile shape, unit couplings, start
+    int ones = 1;                             // tails: k_mg_tail<ONES>
+    int mode = 2, check = 1;                  // norms
+    int n_jobs = 0, units = 0;
+    size_t offset = 0;                        // its jobs in the device table
+    size_t firsts = 0;                        // and each job's first unit (int32)
+    size_t lds = 0;                           // tails: dynamic LDS
+};
+
+// Blocks of a persistent launch, at most one per unit: for the tile sweeps as many as are resident on the chip
+// at once (the kernel's occupancy times the CUs: a block that starts only when another ends would run its
+// whole share late), for the kernels without LDS eight per CU (32 wavefronts of 256 threads). A converged
+// member's units then cost each block one gate read, where one block per unit would dispatch them all.
+int device_cus() {
+    int device = 0, cus = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    return cus;
+}
+
+template <typename Kernel>
+unsigned launch_blocks(Kernel kernel, int threads, size_t lds, int units) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess || per < 1) per = 1;
+    return (unsigned)std::max(1, std::min(units, device_cus() * per));
+}
+
+inline unsigned stream_blocks(int units) { return (unsigned)std::max(1, std::min(units, device_cus() * 8)); }
+
+template <bool UNIT, int START>
+int launch_batch_sweep(int shape, const void *jobs, const int *firsts, int n_jobs, int units, const MemberGate *gates,
+                       hipStream_t s) {
+#define PLM_BATCH_TILE(TI, TJ, THREADS)                                                                            \
+    {                                                                                                              \
+        auto kernel = k_batch_sweep<TI, TJ, THREADS, UNIT, START>;                                                 \
+        kernel<<<launch_blocks(kernel, THREADS, 0, units), THREADS, 0, s>>>(                                   \
+            static_cast<const SweepJob *>(jobs), firsts, n_jobs, units, gates);                                    \
+        break;                                                                                                     \
+    }
+    switch (shape) {
+    case 1: PLM_BATCH_TILE(2, 8, 256)
+    case 2: PLM_BATCH_TILE(8, 4, 256)
+    case 3: PLM_BATCH_TILE(4, 8, 512)
+    case 4: PLM_BATCH_TILE(8, 8, 512)
+    case 5: PLM_BATCH_TILE(2, 4, 256)
+    default: PLM_BATCH_TILE(4, 4, 256)
+    }
+#undef PLM_BATCH_TILE
+    return check_launch("batched sweep");
+}
+
+// A batched solve: the members' level tables, their gates, the tile shape per depth, and the device table of
+// every stage's jobs with the launch lists (the first check, and one V-cycle with its norms)
+struct BatchPlan {
+    std::vector<Hierarchy> h;
+    std::vector<MemberGate> gates;
+    std::vector<double *> norm_scratch;
+    std::vector<int32_t *> cycles_run;
+    std::vector<int> shapes;
+    int members = 0, depth = 0, sweeps = 0;
+    double tolerance = 0.0;
+    void *table = nullptr;
+    size_t gate_offset = 0;
+    std::vector<BatchStage> first, cycle;
+};
+
+// Adds `jobs` to the byte table (16-byte aligned) and returns their offset
+template <typename Job>
+size_t append_jobs(std::vector<char> &table, const std::vector<Job> &jobs) {
+    const size_t offset = (table.size() + 15) & ~size_t(15);
+    table.resize(offset + jobs.size() * sizeof(Job));
+    if (!jobs.empty()) std::memcpy(table.data() + offset, jobs.data(), jobs.size() * sizeof(Job));
+    return offset;
+}
+
+// A stage's jobs and the list of their first units
+template <typename Job>
+void append_stage(std::vector<char> &table, BatchStage &st, const std::vector<Job> &jobs) {
+    st.offset = append_jobs(table, jobs);
+    std::vector<int> firsts;
+    for (const Job &J : jobs) firsts.push_back(J.first);
+    st.firsts = append_jobs(table, firsts);
+}
+
+int launch_stage(const void *table, const MemberGate *gates, int sweeps, double tolerance, const BatchStage &st,
+                 hipStream_t s) {
+    const char *base = static_cast<const char *>(table);
+    const void *jobs = base + st.offset;
+    const int *firsts = reinterpret_cast<const int *>(base + st.firsts);
+    if (st.n_jobs == 0) return 0;
+    switch (st.kind) {
+    case B_SWEEP:
+        if (st.unit)
+            return st.start == FROM_ZERO ? launch_batch_sweep<true, FROM_ZERO>(st.shape, jobs, firsts, st.n_jobs, st.units, gates, s)
+                                         : launch_batch_sweep<true, FROM_Q>(st.shape, jobs, firsts, st.n_jobs, st.units, gates, s);
+        return st.start == FROM_ZERO ? launch_batch_sweep<false, FROM_ZERO>(st.shape, jobs, firsts, st.n_jobs, st.units, gates, s)
+                                     : launch_batch_sweep<false, FROM_Q>(st.shape, jobs, firsts, st.n_jobs, st.units, gates, s);
+    case B_RESTRICT_UNIT:
+        k_batch_restrict_unit<<<stream_blocks(st.units), block3(), 0, s>>>(
+            static_cast<const UnitJob *>(jobs), firsts, st.n_jobs, st.units, gates);
+        return check_launch("batched restriction");
+    case B_RESTRICT_CHILDREN:
+        k_batch_restrict_children<<<stream_blocks(st.units), 256, 0, s>>>(
+            static_cast<const UnitJob *>(jobs), firsts, st.n_jobs, st.units, gates);
+        return check_launch("batched restriction");
+    case B_PROLONG:
+        k_batch_prolong<<<stream_blocks(st.units), block3(), 0, s>>>(
+            static_cast<const UnitJob *>(jobs), firsts, st.n_jobs, st.units, gates);
+        return check_launch("batched prolongation");
+    case B_TAIL:
+        if (st.ones)
+            k_batch_tail<true><<<st.n_jobs, TAIL_THREADS, st.lds, s>>>(static_cast<const TailJob *>(jobs), gates, sweeps);
+        else
+            k_batch_tail<false><<<st.n_jobs, TAIL_THREADS, st.lds, s>>>(static_cast<const TailJob *>(jobs), gates, sweeps);
+        return check_launch("batched tail");
+    case B_NORM:
+        k_batch_norm_partials<<<stream_blocks(st.units), block3(), 0, s>>>(
+            static_cast<const UnitJob *>(jobs), firsts, st.n_jobs, st.units, gates, st.check);
+        return check_launch("batched norm partials");
+    case B_FINISH:
+        k_batch_norm_finish<<<st.n_jobs, FINISH_THREADS, 0, s>>>(static_cast<const UnitJob *>(jobs), gates, tolerance,
+                                                                 st.mode, st.check);
+        return check_launch("batched norm finish");
+    default:
+        return fail(PLM_EINVAL, "batched solve: unknown stage");
+    }
+}
+
+// The sweep stage of depth d with tile shape `shape` over every member that has that level: q -> scratch
+// (to_scratch) or scratch -> q, from q or from 0
+BatchStage batch_sweep_stage(const BatchPlan &b, std::vector<char> &table, int d, int shape, int start,
+                             bool to_scratch) {
+    const TileShape &ts = TILE_SHAPES[shape];
+    std::vector<SweepJob> jobs;
+    int units = 0;
+    bool unit = true;
+    for (int m = 0; m < b.members; m++) {
+        const Hierarchy &h = b.h[m];
+        if (d >= h.tail_start) continue;
+        const LevelArgs &L = h.level[d];
+        const Dims n = cell_dims(L.g);
+        const int tk = (n.n2 + TK - 1) / TK, tj = (n.n1 + ts.tj - 1) / ts.tj, ti = (n.n0 + ts.ti - 1) / ts.ti;
+        SweepJob J{};
+        J.L = L;
+        J.q_in = to_scratch ? h.q[d] : h.residual[d];
+        J.q_out = to_scratch ? h.residual[d] : h.q[d];
+        J.rhs = h.rhs[d];
+        J.tiles_k = fast_div(tk);
+        J.tiles_j = fast_div(tj);
+        J.member = m;
+        J.first = units;
+        units += tk * tj * ti;
+        unit = unit && L.unit;
+        jobs.push_back(J);
+    }
+    BatchStage st;
+    st.kind = B_SWEEP;
+    st.shape = shape;
+    st.unit = unit ? 1 : 0;  // a level with other couplings takes the general template, which serves both
+    st.start = start;
+    st.n_jobs = (int)jobs.size();
+    st.units = units;
+    append_stage(table, st, jobs);
+    return st;
+}
+
+UnitJob unit_job(const LevelArgs &L, int m, int &units, int gx, int gy, int gz) {
+    UnitJob J{};
+    J.L = L;
+    J.member = m;
+    J.first = units;
+    J.gx = gx;
+    J.gy = gy;
+    J.gz = gz;
+    units += gx * gy * gz;
+    return J;
+}
+
+// Builds the launch lists and their jobs for b.shapes and puts the table on the device (replacing the old one)
+int build_tables(BatchPlan &b) {
+    std::vector<char> table;
+    b.gate_offset = append_jobs(table, b.gates);
+    std::vector<BatchStage> first, cycle;
+    for (int d = 0; d < b.depth; d++) {
+        // pre-smoothing: q -> scratch -> q ..., the first sweep from q on the simulation grid, from 0 below it
+        for (int sw = 0; sw < b.sweeps; sw++)
+            cycle.push_back(batch_sweep_stage(b, table, d, b.shapes[d], sw > 0 ? FROM_Q : d == 0 ? FROM_Q : FROM_ZERO,
+                                              sw % 2 == 0));
+        // residual + restriction onto level d + 1, by the coupling form of each member's level
+        std::vector<UnitJob> unit_jobs, child_jobs;
+        int unit_units = 0, child_units = 0;
+        for (int m = 0; m < b.members; m++) {
+            const Hierarchy &h = b.h[m];
+            if (d >= h.tail_start) continue;
+            const LevelArgs &L = h.level[d];
+            const Dims nc = cell_dims(h.level[d + 1].g);
+            UnitJob J;
+            if (L.unit) {
+                const dim3 g = grid3(nc);
+                J = unit_job(L, m, unit_units, (int)g.x, (int)g.y, (int)g.z);
+            } else {
+                J = unit_job(L, m, child_units, (int)children_grid(nc).x, 1, 1);
+            }
+            J.a = h.q[d];
+            J.b = h.rhs[d];
+            J.out = h.rhs[d + 1];
+            J.nc = nc;
+            (L.unit ? unit_jobs : child_jobs).push_back(J);
+        }
+        for (int form = 0; form < 2; form++) {
+            BatchStage st;
+            st.kind = form == 0 ? B_RESTRICT_UNIT : B_RESTRICT_CHILDREN;
+            const std::vector<UnitJob> &jobs = form == 0 ? unit_jobs : child_jobs;
+            st.n_jobs = (int)jobs.size();
+            st.units = form == 0 ? unit_units : child_units;
+            append_stage(table, st, jobs);
+            if (st.n_jobs > 0) cycle.push_back(st);
+        }
+    }
+    // every member's LDS tail (one block each), in two launches by coupling form
+    for (int ones = 1; ones >= 0; ones--) {
+        std::vector<TailJob> jobs;
+        size_t lds = 0;
+        for (int m = 0; m < b.members; m++) {
+            const Hierarchy &h = b.h[m];
+            if ((h.tail_ones ? 1 : 0) != ones) continue;
+            TailJob J{};
+            J.tail = h.tail;
+            J.q_top = h.q[h.tail_start];
+            J.rhs_top = h.rhs[h.tail_start];
+            J.member = m;
+            J.first = (int)jobs.size();
+            jobs.push_back(J);
+            lds = std::max(lds, h.tail_bytes);
+        }
+        BatchStage st;
+        st.kind = B_TAIL;
+        st.ones = ones;
+        st.n_jobs = (int)jobs.size();
+        st.units = st.n_jobs;
+        st.lds = lds;
+        append_stage(table, st, jobs);
+        if (st.n_jobs > 0) cycle.push_back(st);
+    }
+    for (int d = b.depth - 1; d >= 0; d--) {
+        // prolong-add from level d + 1 (the row kernel), then post-smoothing from q
+        std::vector<UnitJob> jobs;
+        int units = 0;
+        for (int m = 0; m < b.members; m++) {
+            const Hierarchy &h = b.h[m];
+            if (d >= h.tail_start) continue;
+            const LevelArgs &L = h.level[d];
+            const dim3 g = grid3(cell_dims(L.g));
+            UnitJob J = unit_job(L, m, units, (int)g.x, (int)g.y, (int)g.z);
+            J.a = h.q[d + 1];
+            J.out = h.q[d];
+            J.nc = cell_dims(h.level[d + 1].g);
+            jobs.push_back(J);
+        }
+        BatchStage st;
+        st.kind = B_PROLONG;
+        st.n_jobs = (int)jobs.size();
+        st.units = units;
+        append_stage(table, st, jobs);
+        cycle.push_back(st);
+        for (int sw = 0; sw < b.sweeps; sw++)
+            cycle.push_back(batch_sweep_stage(b, table, d, b.shapes[d], FROM_Q, sw % 2 == 0));
+    }
+    // the FP64 norms of every member's simulation grid: partials over the units of each member's norm_grid, then
+    // one finishing block per member
+    std::vector<UnitJob> partial_jobs, finish_jobs;
+    int units = 0;
+    for (int m = 0; m < b.members; m++) {
+        const LevelArgs &L = b.h[m].level[0];
+        const dim3 g = norm_grid(L.g);
+        UnitJob J = unit_job(L, m, units, (int)g.x, (int)g.y, (int)g.z);
+        J.a = b.h[m].q[0];
+        J.b = b.h[m].rhs[0];
+        J.scratch = b.norm_scratch[m];
+        J.cycles_run = b.cycles_run[m];
+        J.blocks = norm_blocks(L.g);
+        partial_jobs.push_back(J);
+        J.first = m;
+        finish_jobs.push_back(J);
+    }
+    BatchStage partials, finish;
+    partials.kind = B_NORM;
+    partials.n_jobs = b.members;
+    partials.units = units;
+    append_stage(table, partials, partial_jobs);
+    finish.kind = B_FINISH;
+    finish.n_jobs = b.members;
+    finish.units = b.members;
+    append_stage(table, finish, finish_jobs);
+    for (int mode = 1; mode <= 2; mode++) {
+        partials.check = finish.check = mode == 2 ? 1 : 0;
+        finish.mode = mode;
+        std::vector<BatchStage> &list = mode == 1 ? first : cycle;
+        list.push_back(partials);
+        list.push_back(finish);
+    }
+    void *device = nullptr;
+    if (hipMalloc(&device, table.size()) != hipSuccess)
+        return fail(PLM_EINVAL, "batched solve: %zu bytes of device memory refused", table.size());
+    if (hipMemcpy(device, table.data(), table.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(device);
+        return fail(PLM_EINVAL, "batched solve: table copy failed");
+    }
+    if (b.table != nullptr) (void)hipFree(b.table);
+    b.table = device;
+    b.first = std::move(first);
+    b.cycle = std::move(cycle);
+    return 0;
+}
+
+const MemberGate *device_gates(const BatchPlan &b) {
+    return reinterpret_cast<const MemberGate *>(static_cast<const char *>(b.table) + b.gate_offset);
+}
+
+// The time of one FROM_ZERO sweep stage at depth d in shape v, every member on (gates that are always open), writing
+// only the members' scratch levels: microseconds per launch over 2 * repeats launches
+int time_batch_sweep(const BatchPlan &b, int d, int v, int repeats, hipStream_t s, float &us) {
+    std::vector<char> table;
+    const size_t one_offset = append_jobs(table, std::vector<double>{1.0});
+    const size_t zero_offset = append_jobs(table, std::vector<int32_t>{0});
+    std::vector<MemberGate> open((size_t)b.members);
+    const size_t gates_offset = append_jobs(table, open);
+    const BatchStage st = batch_sweep_stage(b, table, d, v, FROM_ZERO, true);
+    void *device = nullptr;
+    if (hipMalloc(&device, table.size()) != hipSuccess) return fail(PLM_EINVAL, "batched autotune: allocation refused");
+    for (int m = 0; m < b.members; m++)
+        open[m] = MemberGate{reinterpret_cast<const double *>(static_cast<char *>(device) + one_offset),
+                             reinterpret_cast<const int32_t *>(static_cast<char *>(device) + zero_offset), 1, 0};
+    std::memcpy(table.data() + gates_offset, open.data(), open.size() * sizeof(MemberGate));
+    int rc = hipMemcpy(device, table.data(), table.size(), hipMemcpyHostToDevice) == hipSuccess
+                 ? 0
+                 : fail(PLM_EINVAL, "batched autotune: table copy failed");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(PLM_EINVAL, "batched autotune: event creation failed");
+    const MemberGate *gates = reinterpret_cast<const MemberGate *>(static_cast<char *>(device) + gates_offset);
+    if (rc == 0) rc = launch_stage(device, gates, b.sweeps, b.tolerance, st, s);  // warm-up
+    if (rc == 0 && hipEventRecord(e0, s) != hipSuccess) rc = fail(PLM_EINVAL, "batched autotune: event record failed");
+    for (int r = 0; r < 2 * repeats && rc == 0; r++) rc = launch_stage(device, gates, b.sweeps, b.tolerance, st, s);
+    if (rc == 0 && hipEventRecord(e1, s) != hipSuccess) rc = fail(PLM_EINVAL, "batched autotune: event record failed");
+    if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = fail(PLM_EINVAL, "batched autotune: synchronise failed");
+    float ms = 0.0f;
+    if (rc == 0) (void)hipEventElapsedTime(&ms, e0, e1);
+    us = ms * 1000.0f / (float)(2 * repeats);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == 0) rc = fail(PLM_EINVAL, "batched autotune: synchronise failed");
+    (void)hipFree(device);
+    return rc;
+}
+
+// k_batch_tail needs the dynamic LDS of its largest tail allowed once
+template <bool ONES>
+int allow_batch_tail_lds(size_t bytes) {
+    static size_t allowed = 0;
+    if (bytes <= 65536 || bytes <= allowed) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_batch_tail<ONES>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+        return fail(PLM_EINVAL, "batched tail: %zu bytes of LDS refused", bytes);
+    allowed = bytes;
+    return 0;
+}
 }  // namespace
 }  // namespace plm
 
 using namespace plm;
+
+// The handle of a batched solve (its plan lives in the library's internal namespace)
+struct plm_mg_batch {
+    void *plan = nullptr;
+};
 
 extern "C" {
 
@@ -2497,74 +3187,8 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
     PLM_REQUIRE(!tolerance_mode || norm_scratch, "plm_vcycle: tolerance mode needs norm_scratch");
     hipStream_t s = as_stream(stream);
 
-    Hierarchy h{};
-    h.n_levels = n_levels;
-    for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
-    h.q = q;
-    h.rhs = rhs;
-    h.residual = residual;
-    h.sweeps = sweeps;
-    h.gate = tolerance_mode ? norm_scratch + 1 : nullptr;
-    // the tail starts at the first level from which every coarser level fits in LDS together
-    // (and whose colour slots fit the tail's per-thread plans; levels are added coarsest first, so the one
-    // just added is the largest)
-    int total = 0;
-    bool ones = true;
-    h.tail_start = n_levels;
-    for (int l = n_levels - 1; l >= 0; l--) {
-        const plm_grid_info &g = infos[l];
-        const int cells = g.dims[0] * g.dims[1] * g.dims[2];
-        const bool with = ones && g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f;
-        const int plan = with ? BLOCK_PLAN_ONES : BLOCK_PLAN;
-        if (total + cells > PLM_LDS_CELLS || n_levels - l > MAX_TAIL || colour_slots(g) > plan * TAIL_THREADS) break;
-        total += cells;
-        ones = with;
-        h.tail_start = l;
-    }
-    // The tail's LDS: each level's q and rhs zero-bordered ((n0 + 2)(n1 + 2)(n2 + 2) floats each) and the padded
-    // residual of the largest level that restricts; a tail that would not fit drops its largest level.
-    auto padded = [&](int l) {
-        return (infos[l].dims[0] + 2) * (infos[l].dims[1] + 2) * (infos[l].dims[2] + 2);
-    };
-    auto tail_floats = [&](int from) {
-        int floats = 0, restricted = 0;
-        for (int l = from; l < n_levels; l++) {
-            floats += 2 * padded(l);
-            if (l + 1 < n_levels) restricted = std::max(restricted, padded(l));
-        }
-        return floats + restricted;
-    };
-    while (h.tail_start < n_levels && sizeof(float) * (size_t)tail_floats(h.tail_start) > TAIL_LDS_BYTES)
-        h.tail_start++;
-    ones = true;
-    for (int l = h.tail_start; l < n_levels; l++)
-        ones = ones && infos[l].fill_hi[0] == 1.0f && infos[l].fill_hi[1] == 1.0f && infos[l].fill_hi[2] == 1.0f;
-    h.tail_ones = ones;
-    if (h.tail_start < n_levels) {
-        h.tail.count = n_levels - h.tail_start;
-        int offset = 0, restricted = 0;
-        for (int m = 0; m < h.tail.count; m++) {
-            h.tail.level[m] = h.level[h.tail_start + m];
-            const plm_grid_info &g = infos[h.tail_start + m];
-            h.tail.qoff[m] = offset;
-            h.tail.roff[m] = offset + padded(h.tail_start + m);
-            offset += 2 * padded(h.tail_start + m);
-            h.tail.div_n1[m] = fast_div(g.dims[1]);
-            h.tail.div_n2[m] = fast_div(g.dims[2]);
-            h.tail.div_half[m] = fast_div((g.dims[2] + 1) / 2);
-            if (m + 1 < h.tail.count) restricted = std::max(restricted, padded(h.tail_start + m));
-        }
-        h.tail.q_floats = offset;
-        h.tail.res_offset = offset;
-        h.tail_bytes = sizeof(float) * (size_t)(offset + restricted);
-        h.tail.wave_from = h.tail.count;
-        for (int m = h.tail.count - 1; m >= 0; m--) {
-            const plm_grid_info &g = infos[h.tail_start + m];
-            if (g.dims[0] * g.dims[1] * g.dims[2] > WAVE_CELLS || colour_slots(g) > WAVE) break;
-            h.tail.wave_from = m;
-        }
-        PLM_TRY(h.tail_ones ? allow_tail_lds<true>(h.tail_bytes) : allow_tail_lds<false>(h.tail_bytes));
-    }
+    Hierarchy h;
+    PLM_TRY(build_hierarchy(h, n_levels, infos, q, rhs, residual, sweeps, tolerance_mode ? norm_scratch + 1 : nullptr));
 
     TimingScope t("multigrid", s);
     if (tolerance_mode && !resume)
@@ -2657,6 +3281,114 @@ int plm_refine_residual(const plm_grid_info *info, const float *x, const void *e
     PLM_REQUIRE(hipMemcpyAsync(out, scratch + 2, 2 * sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess,
                 "plm_refine_residual: copy failed");
     return 0;
+}
+
+
+int plm_mg_batch_create(int32_t members, const int32_t *n_levels, const plm_grid_info *const *infos,
+                        float *const *const *q, float *const *const *rhs, float *const *const *residual,
+                        int32_t sweeps, double tolerance, const int32_t *max_cycles, double *const *norm_scratch,
+                        int32_t *const *cycles_run, plm_mg_batch **out) {
+    PLM_REQUIRE(out != nullptr, "plm_mg_batch_create: NULL out");
+    *out = nullptr;
+    PLM_REQUIRE(members >= 1 && members <= 64 && n_levels && infos && q && rhs && residual && max_cycles &&
+                    norm_scratch && cycles_run,
+                "plm_mg_batch_create: bad member arrays (1 to 64 members)");
+    PLM_REQUIRE(sweeps >= 2 && sweeps % 2 == 0, "plm_mg_batch_create: sweeps must be even and positive (got %d)",
+                sweeps);
+    PLM_REQUIRE(tolerance > 0.0, "plm_mg_batch_create: the batched solve runs in tolerance mode (tolerance > 0)");
+    std::unique_ptr<BatchPlan> plan(new BatchPlan());
+    BatchPlan &bp = *plan;
+    bp.members = members;
+    bp.sweeps = sweeps;
+    bp.tolerance = tolerance;
+    bp.h.resize((size_t)members);
+    for (int m = 0; m < members; m++) {
+        const int n = n_levels[m];
+        PLM_REQUIRE(n >= 1 && n <= 16 && infos[m] && q[m] && rhs[m] && residual[m] && norm_scratch[m] && cycles_run[m],
+                    "plm_mg_batch_create: bad arrays of member %d", m);
+        PLM_REQUIRE(max_cycles[m] >= 0, "plm_mg_batch_create: negative cycle cap of member %d", m);
+        for (int l = 0; l < n; l++) {
+            PLM_TRY(validate_info(&infos[m][l]));
+            PLM_REQUIRE(q[m][l] && rhs[m][l] && residual[m][l], "plm_mg_batch_create: NULL array, member %d level %d",
+                        m, l);
+            if (l > 0)
+                for (int a = 0; a < 3; a++)
+                    PLM_REQUIRE(infos[m][l].dims[a] == (infos[m][l - 1].dims[a] + 1) / 2,
+                                "plm_mg_batch_create: member %d level %d does not halve level %d", m, l, l - 1);
+        }
+        Hierarchy &h = bp.h[m];
+        PLM_TRY(build_hierarchy(h, n, infos[m], q[m], rhs[m], residual[m], sweeps, norm_scratch[m] + 1));
+        PLM_REQUIRE(h.tail_start < n, "plm_mg_batch_create: member %d has no LDS tail", m);
+        PLM_TRY(h.tail_ones ? allow_batch_tail_lds<true>(h.tail_bytes) : allow_batch_tail_lds<false>(h.tail_bytes));
+        // the reference's freshly zeroed coarse q, unless the whole hierarchy is the tail (warm start)
+        h.tail.zero_top = h.tail_start > 0 ? 1 : 0;
+        bp.gates.push_back(MemberGate{norm_scratch[m] + 1, cycles_run[m], max_cycles[m], 0});
+        bp.norm_scratch.push_back(norm_scratch[m]);
+        bp.cycles_run.push_back(cycles_run[m]);
+        bp.depth = std::max(bp.depth, h.tail_start);
+    }
+    // each depth starts with the shape pinned for its largest member level (plm_mg_batch_autotune retunes)
+    for (int d = 0; d < bp.depth; d++) {
+        int largest = -1, cells = -1;
+        for (int m = 0; m < members; m++)
+            if (d < bp.h[m].tail_start && cell_dims(bp.h[m].level[d].g).count() > cells) {
+                cells = cell_dims(bp.h[m].level[d].g).count();
+                largest = m;
+            }
+        bp.shapes.push_back(tile_shape_for(cell_dims(bp.h[largest].level[d].g)));
+    }
+    PLM_TRY(build_tables(bp));
+    plm_mg_batch *b = new plm_mg_batch();
+    b->plan = plan.release();
+    *out = b;
+    return 0;
+}
+
+int plm_mg_batch_autotune(plm_mg_batch *batch, int32_t repeats, float *times_us, void *stream) {
+    PLM_REQUIRE(batch != nullptr && batch->plan != nullptr && repeats >= 1, "plm_mg_batch_autotune: bad arguments");
+    BatchPlan &bp = *static_cast<BatchPlan *>(batch->plan);
+    hipStream_t s = as_stream(stream);
+    for (int d = 0; d < bp.depth; d++) {
+        int best = 0;
+        float best_us = 0.0f;
+        for (int v = 0; v < N_TILE_SHAPES; v++) {
+            float us = 0.0f;
+            PLM_TRY(time_batch_sweep(bp, d, v, repeats, s, us));
+            if (times_us) times_us[d * N_TILE_SHAPES + v] = us;
+            if (v == 0 || us < best_us) {
+                best = v;
+                best_us = us;
+            }
+        }
+        bp.shapes[d] = best;
+    }
+    return build_tables(bp);
+}
+
+int32_t plm_mg_batch_depth(const plm_mg_batch *batch) {
+    if (batch == nullptr || batch->plan == nullptr) return -1;
+    return static_cast<const BatchPlan *>(batch->plan)->depth;
+}
+
+int plm_mg_batch_solve(plm_mg_batch *batch, int32_t cycles, void *stream) {
+    PLM_REQUIRE(batch != nullptr && batch->plan != nullptr && cycles >= 0, "plm_mg_batch_solve: bad arguments");
+    const BatchPlan &bp = *static_cast<const BatchPlan *>(batch->plan);
+    hipStream_t s = as_stream(stream);
+    const MemberGate *gates = device_gates(bp);
+    TimingScope t("multigrid_batch", s);
+    for (const BatchStage &st : bp.first) PLM_TRY(launch_stage(bp.table, gates, bp.sweeps, bp.tolerance, st, s));
+    for (int c = 0; c < cycles; c++)
+        for (const BatchStage &st : bp.cycle) PLM_TRY(launch_stage(bp.table, gates, bp.sweeps, bp.tolerance, st, s));
+    return 0;
+}
+
+int plm_mg_batch_destroy(plm_mg_batch *batch) {
+    if (batch == nullptr) return 0;
+    BatchPlan *plan = static_cast<BatchPlan *>(batch->plan);
+    const hipError_t e = plan != nullptr && plan->table != nullptr ? hipFree(plan->table) : hipSuccess;
+    delete plan;
+    delete batch;
+    return e == hipSuccess ? 0 : fail(PLM_EINVAL, "plm_mg_batch_destroy: free failed");
 }
 
 }  // extern "C"

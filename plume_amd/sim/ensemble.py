@@ -8,17 +8,21 @@ reach the GPU:
 - after the emitting frames, two substeps of every member are captured once into one HIP graph (through
   torch.cuda.CUDAGraph) and replayed: one launch from the host per two substeps of the whole ensemble,
   instead of ~100 host-side launches per member and substep;
+- the pressure solves of the members up to `batch_cells` cells run as one batched multigrid
+  (plm_mg_batch, multigrid.BatchedMultigrid): each stage of a V-cycle is one launch for all of them, persistent
+  blocks walking the members' tiles (one workgroup per member for the LDS tails), with tile shapes from the
+  batch's own autotuner. Those members' steps split around it: up to the divergence on the side streams, the
+  batch on the capture stream, the gradient and the rest on the side streams again. Larger members step whole
+  on their side streams, beside the batch (their solves in tolerance mode with a fixed number of device-gated
+  V-cycles each);
 - members are spread over side streams inside the graph (8 by default, the largest members first onto the
-  least loaded stream by cells), so the graph's branches run concurrently: the members' latency-bound kernels
-  (coarse multigrid levels, the one-workgroup tails, reductions) overlap with each other and with the large
-  members' bandwidth-bound ones;
-- the pressure solves run their tolerance mode with a fixed number of device-gated V-cycles (the most any
-  member needed in the eager frames, plus one), so nothing in the graph reads back to the host. Each solve in
-  the graph adds 1 to its member's device counter when it ends with the residual still above tolerance; the
-  frame loop reads the counters once per replayed frame (beside the CFL plan's read). A frame in which any
-  solve stayed open is undone from a snapshot of the fields taken at its start and run again eagerly, where
-  every solve continues to its target, and the graph for that step length is captured again with more cycles.
-  A replayed frame therefore equals the eager frame bit for bit.
+  least loaded stream by cells), so the graph's branches run concurrently;
+- nothing in the graph reads back to the host: every tolerance-mode solve, batched or not, runs a fixed budget
+  of gated cycles (the most its members needed lately, plus one) and adds 1 to its member's device counter when
+  it ends with the residual still above tolerance. The frame loop reads the counters once per replayed frame
+  (beside the CFL plan's read). A frame in which any solve stayed open is undone from a snapshot of the fields
+  taken at its start and run again eagerly, where every solve continues to its target, and the graph is captured
+  again with more cycles. A replayed frame therefore equals the eager frame bit for bit.
 
 Two substeps are captured because AdvectedField.swap exchanges buffers by reference: after two substeps every
 field is back in the buffer the graph was captured with. The substep count of a frame is therefore common to
@@ -35,7 +39,8 @@ import torch
 from plume_amd.recipe.derived import derive
 from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
 from plume_amd.sim.grid import make_grid
-from plume_amd.sim.multigrid import Multigrid
+from plume_amd.sim.multigrid import BatchedMultigrid, Multigrid
+from plume_amd.sim.pressure import compute_divergence, subtract_gradient, zero_closed_faces
 from plume_amd.sim.scenes import puff_table
 from plume_amd.sim.step_cap import StepCap
 from plume_amd.sim.stepper import Stepper
@@ -84,7 +89,7 @@ def balanced_streams(cells: list[int], streams: int) -> list[int]:
 
 
 class Ensemble:
-    def __init__(self, tables: list[dict], device: str, streams: int = 8):
+    def __init__(self, tables: list[dict], device: str, streams: int = 8, batch_cells: int | None = 1_500_000):
         self.device = device
         self.members = []
         for table in tables:
@@ -111,6 +116,13 @@ class Ensemble:
         self.frame_cycles = torch.zeros(len(self.members), dtype=torch.int32, device=device)
         self.budgets: dict[float, list[int | None]] = {}  # per graph: each member's captured cycle budget
         self._snapshot: list[list[torch.Tensor]] | None = None
+        # the members whose pressure solves run as one batched multigrid inside the graph (tolerance-mode
+        # multigrid with common sweeps and tolerance, at most `batch_cells` cells: None batches every size), and
+        # that batch (built on the first capture). Larger members step whole on the side streams, beside it.
+        self.batch_cells = batch_cells
+        self.batched = self._batchable()
+        self.batch: BatchedMultigrid | None = None
+        self.batch_tiles: list[tuple[int, list[float]]] = []  # the batch autotuner's shape and times per depth
 
     @property
     def cells(self) -> int:
@@ -160,6 +172,15 @@ class Ensemble:
             for dst, src in zip(self._state(m), saved):
                 dst.copy_(src)
 
+    def _batchable(self) -> list[int]:
+        solves = [m.stepper.pressure_solve for m in self.members]
+        chosen = [i for i, solve in enumerate(solves) if isinstance(solve, Multigrid) and solve.tolerance > 0.0
+                  and (self.batch_cells is None or self.members[i].cells <= self.batch_cells)]
+        keys = {(solves[i].sweeps, solves[i].tolerance) for i in chosen}
+        if len(keys) != 1 or next(iter(keys))[0] % 2 != 0:
+            return []
+        return chosen
+
     def step_eager(self, time_s: float, dt: float) -> None:
         for m in self.members:
             m.stepper.step(time_s, dt)
@@ -167,41 +188,82 @@ class Ensemble:
     def capture(self, time_s: float, dt: float, extra_cycles: int = 1) -> torch.cuda.CUDAGraph:
         """Captures two substeps of length `dt` of every member, from a time at which no emitter is active (the
         captured launches do not depend on the time then). Call after at least one eager frame, which sets each
-        solve's cycle count and allocates every scratch array."""
+        solve's cycle count and allocates every scratch array.
+
+        Members whose solves batch (self.batched) run each substep in three parts: their steps up to the
+        projection's divergence on the side streams, one batched multigrid solve of them all on the capture stream
+        (every stage of a V-cycle one launch for all members), then the gradient and the rest of their steps on
+        the side streams again. The batch runs as many cycles as the most any of them needed lately plus the
+        margin, each member stopping on its own gate. Other members step whole on the side streams."""
         if self.emitting(time_s, time_s + 2.0 * dt):
             raise ValueError("an emitter is active: its launches depend on the time and cannot be replayed")
         if self.turbulent:
             raise ValueError("turbulence is on: its pattern depends on the time and cannot be replayed")
         solves = []
         budgets: list[int | None] = []
-        for m in self.members:
+        for index, m in enumerate(self.members):
             solve = m.stepper.pressure_solve
             budget = None
             if isinstance(solve, Multigrid) and solve.tolerance > 0.0:
                 solve.settle()
-                budget = solve.graph_cycles = max(1, solve.recent_max + extra_cycles)
+                budget = max(1, solve.recent_max + extra_cycles)
+                if index not in self.batched:
+                    solve.graph_cycles = budget
                 solves.append(solve)
             budgets.append(budget)
+        batch_cycles = max((budgets[i] for i in self.batched), default=0)
+        for i in self.batched:
+            budgets[i] = batch_cycles
+        if self.batched and self.batch is None:
+            self.batch = BatchedMultigrid([self.members[i].stepper.pressure_solve for i in self.batched],
+                                          [self.members[i].grid for i in self.batched])
+            self.batch_tiles = self.batch.autotune()
         torch.cuda.synchronize(self.device)
+        order = sorted(range(len(self.members)), key=lambda i: -self.members[i].cells)
         graph = torch.cuda.CUDAGraph()
+
+        def note(index: int) -> None:
+            solve = self.members[index].stepper.pressure_solve
+            if solve in solves:  # its gate: 1 while above tolerance
+                count = self.open_solves[index:index + 1]
+                count.add_(solve.norm_scratch[1:2].ne(0.0).to(torch.int32))
+                most = self.frame_cycles[index:index + 1]
+                torch.maximum(most, solve.cycles_run[0:1], out=most)
+
         try:
             with torch.cuda.graph(graph):
                 main = torch.cuda.current_stream()
-                for s in self.streams:
-                    s.wait_stream(main)
-                for index in sorted(range(len(self.members)), key=lambda i: -self.members[i].cells):
-                    m = self.members[index]
-                    with torch.cuda.stream(self.streams[self.stream_of[index]]):
-                        for substep in range(2):
-                            m.stepper.step(time_s + substep * dt, dt)
-                            solve = m.stepper.pressure_solve
-                            if solve in solves:  # its gate: 1 while above tolerance (plm_vcycle)
-                                count = self.open_solves[index:index + 1]
-                                count.add_(solve.norm_scratch[1:2].ne(0.0).to(torch.int32))
-                                most = self.frame_cycles[index:index + 1]
-                                torch.maximum(most, solve.cycles_run[0:1], out=most)
-                for s in self.streams:
-                    main.wait_stream(s)
+                for substep in range(2):
+                    t = time_s + substep * dt
+                    for s in self.streams:
+                        s.wait_stream(main)
+                    for index in order:
+                        m = self.members[index]
+                        with torch.cuda.stream(self.streams[self.stream_of[index]]):
+                            if index in self.batched:
+                                m.stepper.before_projection(t, dt)
+                                zero_closed_faces(m.grid)
+                                compute_divergence(m.grid)
+                            else:
+                                m.stepper.step(t, dt)
+                                note(index)
+                    if self.batched:
+                        own = [self.streams[k] for k in sorted({self.stream_of[i] for i in self.batched})]
+                        for s in own:
+                            main.wait_stream(s)
+                        self.batch(batch_cycles)
+                        for index in self.batched:
+                            note(index)
+                        for s in own:
+                            s.wait_stream(main)
+                        for index in order:
+                            if index in self.batched:
+                                m = self.members[index]
+                                with torch.cuda.stream(self.streams[self.stream_of[index]]):
+                                    subtract_gradient(m.grid)
+                                    m.stepper.after_projection(dt)
+                    for s in self.streams:
+                        main.wait_stream(s)
         finally:
             for solve in solves:
                 solve.graph_cycles = None
@@ -229,10 +291,14 @@ class Ensemble:
                 # the cycles the replayed solves ran set the members' predictions; a budget two or more cycles
                 # above what a member now needs (plus the margin) wastes gated launches: capture again
                 stale = False
-                for m, count, budget in zip(self.members, most, self.budgets[dt]):
+                for index, (m, count, budget) in enumerate(zip(self.members, most, self.budgets[dt])):
                     if budget is not None:
                         m.stepper.pressure_solve.note_cycles(count)
-                        stale = stale or budget > m.stepper.pressure_solve.recent_max + self.extra_cycles
+                        if index not in self.batched:
+                            stale = stale or budget > m.stepper.pressure_solve.recent_max + self.extra_cycles
+                if self.batched:  # the batch's budget follows the member that needs the most cycles
+                    need = max(self.members[i].stepper.pressure_solve.recent_max for i in self.batched)
+                    stale = stale or self.budgets[dt][self.batched[0]] > need + self.extra_cycles
                 if stale:
                     self.graphs.pop(dt, None)
             if any(opened):

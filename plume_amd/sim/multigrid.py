@@ -193,6 +193,80 @@ class Multigrid:
         return int(self.cycles_run[1].item())
 
 
+class BatchedMultigrid:
+    """The tolerance-mode solves of many grids (ensemble members of uneven size) advanced together by
+    plm_mg_batch: each stage of a V-cycle is one launch for every member (a persistent grid of blocks over all
+    members' tiles; one workgroup per member for the LDS tails), each member stopping on its own gate or its
+    max_cycles. A member's pressure ends bit for bit as its own `Multigrid` would leave it, given enough cycles.
+    The members' level arrays (and each grid's pressure and divergence) must keep their storage while the batch
+    lives: the device table holds their addresses."""
+
+    def __init__(self, solves: list[Multigrid], grids):
+        if not solves:
+            raise ValueError("an empty batch")
+        sweeps = {s.sweeps for s in solves}
+        if len(sweeps) != 1:
+            raise ValueError("batched members must share the smoothing sweeps")
+        if any(s.tolerance <= 0.0 for s in solves):
+            raise ValueError("the batched solve runs in tolerance mode")
+        self.solves = list(solves)
+        self.grids = list(grids)
+        tolerance = {s.tolerance for s in solves}
+        if len(tolerance) != 1:
+            raise ValueError("batched members must share the tolerance")
+        self.tolerance = tolerance.pop()
+        count = len(solves)
+        self._keep = []  # ctypes arrays the table was built from (the device table copies their values)
+        infos, qs, rhss, residuals = [], [], [], []
+        for solve, grid in zip(solves, grids):
+            solve.levels[0].q = grid.pressure
+            solve.levels[0].rhs = grid.divergence
+            q, rhs, residual = solve._pointers()
+            infos.append(ctypes.cast(solve._infos, ctypes.c_void_p))
+            qs.append(ctypes.cast(q, ctypes.c_void_p))
+            rhss.append(ctypes.cast(rhs, ctypes.c_void_p))
+            residuals.append(ctypes.cast(residual, ctypes.c_void_p))
+            self._keep += [q, rhs, residual]
+        array = ctypes.c_void_p * count
+        n_levels = (ctypes.c_int32 * count)(*(len(s.levels) for s in solves))
+        caps = (ctypes.c_int32 * count)(*(s.max_cycles for s in solves))
+        scratch = array(*(ptr(s.norm_scratch, torch.float64) for s in solves))
+        cycles = array(*(ptr(s.cycles_run, torch.int32) for s in solves))
+        handle = ctypes.c_void_p()
+        call("plm_mg_batch_create", count, n_levels, array(*infos), array(*qs), array(*rhss), array(*residuals),
+             int(sweeps.pop()), self.tolerance, caps, scratch, cycles, ctypes.byref(handle))
+        self._handle = handle
+
+    def __call__(self, cycles: int) -> None:
+        """Checks every member's residual, then runs up to `cycles` lockstep V-cycles on the current stream."""
+        call("plm_mg_batch_solve", self._handle, int(cycles), stream())
+
+    def autotune(self, repeats: int = 3) -> list[tuple[int, list[float]]]:
+        """Picks the tile shape of each depth's sweep launches by timing every shape over all members (writes
+        only the members' scratch levels; synchronises). Returns (chosen shape, microseconds per shape) per
+        depth."""
+        depth = int(LIB.plm_mg_batch_depth(self._handle))
+        shapes = int(LIB.plm_sweep_tile_shapes(None, None, None, 0))
+        times = (ctypes.c_float * max(1, depth * shapes))()
+        call("plm_mg_batch_autotune", self._handle, int(repeats), times, stream())
+        out = []
+        for d in range(depth):
+            row = [round(times[d * shapes + v], 2) for v in range(shapes)]
+            out.append((min(range(shapes), key=lambda v: row[v]), row))
+        return out
+
+    def close(self) -> None:
+        if self._handle is not None and self._handle.value:
+            call("plm_mg_batch_destroy", self._handle)
+        self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown: the library may be gone
+            pass
+
+
 def residual(level: Level) -> torch.Tensor:
     """residual_kernel (sim/multigrid.py:36-43) into `level.residual`."""
     call("plm_residual", ctypes.byref(level.info), ptr(level.q), ptr(level.rhs), ptr(level.residual), stream())

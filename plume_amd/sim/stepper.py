@@ -29,6 +29,12 @@ class Stepper:
 
     def step(self, time_s: float, dt: float) -> None:
         """Advances from `time_s` to `time_s + dt` (sim/stepper.py:28-44)."""
+        self.before_projection(time_s, dt)
+        project(self.grid, self.pressure_solve)
+        self.after_projection(dt)
+
+    def before_projection(self, time_s: float, dt: float) -> None:
+        """The step up to the projection: emit, forces, velocity advection (sim/stepper.py:30-41)."""
         grid, recipe = self.grid, self.recipe
         ambient = recipe.domain.ambient_k
         if self._any_emitter_active(time_s):
@@ -42,9 +48,14 @@ class Stepper:
                          time_s, dt)
         apply_drag(grid, forces.wind_mps, forces.drag_per_s, dt)
         self._advect_velocity(dt)
-        project(grid, self.pressure_solve)
+
+    def after_projection(self, dt: float) -> None:
+        """The step after the projection: scalar advection with the smoke budget, dissipation (sim/stepper.py:
+        42-44)."""
+        recipe = self.recipe
         self._advect_scalars(dt)
-        dissipate(grid, recipe.dissipation.smoke_per_s, recipe.dissipation.cooling_per_s, ambient, dt)
+        dissipate(self.grid, recipe.dissipation.smoke_per_s, recipe.dissipation.cooling_per_s,
+                  recipe.domain.ambient_k, dt)
 
     def _any_emitter_active(self, time_s: float) -> bool:
         """is_active (sim/emitters.py:80-82) of any emitter, compared in float32 as the kernels compare; when

@@ -37,7 +37,7 @@ def test_every_declared_entry_point_has_a_binding():
     import plume_amd._lib as lib
 
     bound = set(lib.SIGNATURES) | {"plm_last_error", "plm_abi_version", "plm_norm_scratch_doubles",
-                                    "plm_sweep_tile_shapes", "plm_adj_raymarch_record_stride"}
+                                    "plm_sweep_tile_shapes", "plm_adj_raymarch_record_stride", "plm_mg_batch_depth"}
     assert set(declared_functions()) == bound
 
 

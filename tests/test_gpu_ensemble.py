@@ -37,14 +37,19 @@ def fields(grid):
         "pressure": grid.pressure.cpu().numpy()}
 
 
-@pytest.mark.parametrize("extra_cycles", [None, -1000], ids=["default-budget", "starved-budget"])
-def test_graph_replay_matches_each_member_stepped_alone(device, extra_cycles):
+@pytest.mark.parametrize("extra_cycles, batch_cells", [(None, None), (-1000, None), (None, 20000)],
+                         ids=["default-budget", "starved-budget", "larger-members-beside-the-batch"])
+def test_graph_replay_matches_each_member_stepped_alone(device, extra_cycles, batch_cells):
     """With the default cycle budget and with a budget of one cycle per solve (every replayed frame then ends
-    with an open gate and is run again eagerly from its snapshot), replayed frames equal eager ones."""
+    with an open gate and is run again eagerly from its snapshot), replayed frames equal eager ones; every
+    member's solve in one batched multigrid, or only the members of at most 20000 cells in it and the larger
+    ones stepped whole on their streams beside it."""
     from plume_amd.sim.ensemble import Ensemble
 
     tabs = tables()
-    ens = Ensemble(tabs, device, streams=3)
+    ens = Ensemble(tabs, device, streams=3, batch_cells=batch_cells)
+    if batch_cells is not None:
+        assert 0 < len(ens.batched) < len(ens.members)
     if extra_cycles is not None:
         ens.extra_cycles = extra_cycles
     alone = [alone_stepper(t, device) for t in tabs]
