@@ -240,7 +240,8 @@ def mixed_bench(args) -> dict:
     """Mixed-precision pressure projection at 256x384x256: the seeded divergent field
     (64 Gaussian vortex blobs plus a radial emitter source), solved from q = 0 to a common relative residual
     by the plain FP32 multigrid (tolerance mode) and by FP32 iterative refinement around BF16 and FP16
-    multigrid with an FP64 coarsest 8x12x8 level, every form with the same V(sweeps, sweeps) cycle. The
+    multigrid with an FP64 coarsest 8x12x8 level (f16x2: the FP16 form with the simulation level swept in
+    packed FP16 arithmetic), every form with the same V(sweeps, sweeps) cycle. The
     common target is max(tolerance, 2 x the FP32 floor),
     the floor being where plain FP32 V-cycles stall (below it no FP32 pressure gets; the FP64 oracle's
     solution rounded to FP32 sits at the same level). Solve time, FP64 relative residual and
@@ -260,10 +261,11 @@ def mixed_bench(args) -> dict:
         print(f"[bench] mixed {name}: {entry['seconds'] * 1e3:.2f} ms, residual {entry['relative_residual']:.3g}, "
               f"post div {entry['post_projection_divergence']:.3g}", file=sys.stderr, flush=True)
     plain = results["f32"]["cell_solves_per_s"]
-    for name in ("bf16", "f16"):
+    for name in ("bf16", "f16", "f16x2"):
         results[name]["speedup_vs_f32"] = results[name]["cell_solves_per_s"] / plain
     return {"metric": "pressure-solve cells/s at 256x384x256 to a common relative residual (FP32 vs BF16/FP16 "
-                      "multigrid with FP32 iterative refinement)", "grid": list(MIXED_GRID),
+                      "multigrid with FP32 iterative refinement; f16x2: FP16 with the simulation level swept in "
+                      "packed FP16 arithmetic)", "grid": list(MIXED_GRID),
             "tolerance": args.mixed_tolerance, "fp32_floor": floor, "target": target,
             "target_rule": "max(tolerance, 2 x FP32 floor); floor = where plain FP32 V-cycles stall",
             "sweeps": args.mixed_sweeps,

@@ -298,6 +298,13 @@ int plm_residual_norms(const plm_grid_info *info, const float *q, const float *r
 #define PLM_STORAGE_F32 0
 #define PLM_STORAGE_BF16 1
 #define PLM_STORAGE_F16 2
+/* FP16 storage with the sweeps of flat levels (the simulation level, and coarser levels whose fill_hi is
+   (1, 1, 1)) in packed FP16 arithmetic (two cells per lane: v_pk_add_f16, v_pk_fma_f16; the per-cell factors
+   1 / t1 and -cell_m^2 rhs / t1 formed in FP32 and rounded once). Other levels, residuals and refinement are
+   those of PLM_STORAGE_F16; the entry points below that only store take it as PLM_STORAGE_F16. A level sweeps
+   packed where its k extent is a multiple of 4 and every extent is at least 2 (the packed tile's aligned
+   4-cell groups); elsewhere as PLM_STORAGE_F16. */
+#define PLM_STORAGE_F16_PACKED 3
 
 /* `cycles` V(sweeps, sweeps) cycles of Multigrid._v_cycle (sim/multigrid.py:130-147) with q, rhs and scratch of
    every level stored as `storage` (FP32 arithmetic in the sweeps and the FP32 residual inside the fused
@@ -306,6 +313,23 @@ int plm_residual_norms(const plm_grid_info *info, const float *q, const float *r
    scratch: device arrays of the storage type; the coarsest level needs no scratch. sweeps >= 1. */
 int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage, void *const *q, void *const *rhs,
                   void *const *scratch, int32_t cycles, int32_t sweeps, int32_t zero_start, void *stream);
+/* `sweeps` two-colour sweeps at relaxation 1 of one level in `storage` (red_black_sweeps, sim/pressure.py:80-92,
+   as the V-cycle of plm_vcycle_lp runs them on that level): q and scratch ping-pong, the result in q (an odd
+   count ends with a copy back). zero_start: the first sweep starts from q = 0 (q is not read). */
+int plm_rb_sweeps_lp(const plm_grid_info *info, int32_t storage, void *q, const void *rhs, void *scratch,
+                     int32_t sweeps, int32_t zero_start, void *stream);
+/* residual_kernel + restrict_kernel (sim/multigrid.py:36-60) of one level in `storage`, as the V-cycle of
+   plm_vcycle_lp forms the next level's right-hand side: coarse_rhs (the level's dims halved, rounded up) =
+   restrict(rhs - A q), the residual in FP32 and stored rounded to nearest even. */
+int plm_residual_restrict_lp(const plm_grid_info *info, int32_t storage, const void *q, const void *rhs,
+                             void *coarse_rhs, void *stream);
+/* q += prolongation(coarse) (prolongation_kernel, sim/multigrid.py:63-80) on one level in `storage`, as the
+   V-cycle of plm_vcycle_lp applies the coarse correction: the sum in FP32, stored rounded to nearest even.
+   coarse has the level's dims halved, rounded up. */
+int plm_prolong_add_lp(const plm_grid_info *info, int32_t storage, const void *coarse, void *q, void *stream);
+/* The tile shape PLM_STORAGE_F16_PACKED sweeps launch with, an index into {8 x 8 on 256 threads, 8 x 8 on 512,
+   4 x 8 on 256} (i x j, 64 cells along k); process-wide. */
+int plm_set_f16x2_shape(int32_t shape);
 /* dst = storage(src * scale), rounded to nearest even. */
 int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage, void *dst, void *stream);
 /* x = x + float(e) * inv_scale. */

@@ -107,6 +107,10 @@ SIGNATURES = {
     "plm_mg_batch_autotune": [_P, _I, _P, _P],
     "plm_residual_norms": [_INFO, _P, _P, _P, _P, _P],
     "plm_vcycle_lp": [_I, _P, _I, _P, _P, _P, _I, _I, _I, _P],
+    "plm_rb_sweeps_lp": [_P, _I, _P, _P, _P, _I, _I, _P],
+    "plm_set_f16x2_shape": [_I],
+    "plm_residual_restrict_lp": [_P, _I, _P, _P, _P, _P],
+    "plm_prolong_add_lp": [_P, _I, _P, _P, _P],
     "plm_to_storage": [_L, _P, _F, _I, _P, _P],
     "plm_add_storage": [_L, _P, _P, _I, _F, _P],
     "plm_refine_residual": [_INFO, _P, _P, _F, _I, _P, _P, _P, _F, _P, _P, _P],

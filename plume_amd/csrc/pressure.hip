@@ -2256,6 +2256,440 @@ int launch_refine(const LevelArgs &L, const float *x, const void *e, float inv_s
     return check_launch("norm finish");
 }
 
+// ------------------------------------------------------------------ packed FP16 sweep (simulation level)
+//
+// The two-colour tile sweep of an FP16 correction solve on a flat level (the simulation level and the coarser
+// levels of even-sized grids) with its arithmetic in packed FP16: a lane updates two cells of one colour at once (v_pk_add_f16, v_pk_fma_f16), which halves the
+// instructions, the LDS accesses and the LDS footprint of the FP32-arithmetic tile sweep, whose cost at large
+// sizes is its issue and latency, not its HBM bytes. The update is red_black_sweep_kernel's at relaxation 1
+// (sim/pressure.py:86-92), q = (sum of the six neighbours - cell_m² rhs) / t1, evaluated as
+// sum * (1 / t1) + (-cell_m² rhs / t1) with both per-cell factors formed in FP32 and rounded to FP16 once. A
+// neighbour outside the level is +0 in the tile, and t1 counts the level's neighbours plus the open boundary
+// faces' ghost couplings (1 on the simulation level, 1 / (0.5 + ghost_gap) below it). The results are FP16 roundings of the same sweep, not the FP32-arithmetic sweep's
+// bits.
+//
+// LDS rows, (TI + 4) x (TJ + 4) of them, hold a row's 68 cells k0 - 2 .. k0 + 65 as halves, colour-split: the
+// cell kk = k - k0 + 2 at half (kk & 1) PK_HP + (kk >> 1) + 1. The 32 cells of one parity inside the tile
+// (kk = 2 .. 65) are then dwords 1 .. 16 of their part, a lane's pair one dword: its i and j neighbour pairs
+// are the same dword of the rows around, one k-neighbour pair the other part's dword of the same index and the
+// other a half-dword off (v_alignbit of two dwords). Rows are PK_RW halves apart, so that the four rows a
+// wavefront updates together (j, j + 2, j + 4, j + 6 of one i and parity) start 16 banks apart.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+constexpr int PK_HP = 36, PK_RW = 80;
+
+__device__ __forceinline__ half2_t as_half2(uint32_t v) { return __builtin_bit_cast(half2_t, v); }
+__device__ __forceinline__ uint32_t as_bits(half2_t v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t pack_half2(float lo, float hi) {
+    return as_bits(half2_t{(_Float16)lo, (_Float16)hi});
+}
+__device__ __forceinline__ float half_lo(uint32_t v) { return (float)as_half2(v).x; }
+__device__ __forceinline__ float half_hi(uint32_t v) { return (float)as_half2(v).y; }
+__device__ __forceinline__ int pk_slot(int kk) { return (kk & 1) * PK_HP + (kk >> 1) + 1; }
+
+// levels the packed sweep takes: flat levels (fill_hi (1, 1, 1): every coupling between two cells and every
+// volume 1, an open boundary face coupling by the level's ghost value; the simulation level is one), rows of
+// whole 4-cell groups (so a row's 8-byte loads are aligned and a group lies wholly inside or outside the
+// level) and every cell coupled to something
+bool packed_f16_level(const LevelArgs &L) {
+    const Dims n = cell_dims(L.g);
+    return (L.unit || L.flat == 1) && n.n2 % 4 == 0 && n.n0 >= 2 && n.n1 >= 2 && n.n2 >= 4;
+}
+
+template <int TI, int TJ, int THREADS, int START, bool EDGE>
+__device__ __forceinline__ void sweep_tile_f16x2(const LevelArgs &L, uint32_t *__restrict__ s,
+                                                 const __half *__restrict__ q_in, __half *__restrict__ q_out,
+                                                 const __half *__restrict__ rhs, int i0, int j0, int k0) {
+    constexpr int WAVES = THREADS / 64, LI = TI + 2 * TH, LJ = TJ + 2 * TH, RI = TI + 2, RJ = TJ + 2;
+    constexpr int ROWD = PK_RW / 2, PARTD = PK_HP / 2;  // dwords per row, dwords from a row's part 0 to part 1
+    constexpr int RED_GROUPS = RI * RJ / 4, BLACK_GROUPS = TI * TJ / 4;
+    constexpr int RED_ITERS = (RED_GROUPS + WAVES - 1) / WAVES, BLACK_ITERS = (BLACK_GROUPS + WAVES - 1) / WAVES;
+    constexpr int LOAD_GROUPS = LI * LJ / 4, LOAD_ITERS = (LOAD_GROUPS + WAVES - 1) / WAVES;
+    constexpr int STORE_ITERS = (BLACK_GROUPS + WAVES - 1) / WAVES;
+    constexpr int HALO = LI * LJ * 2, HALO_ITERS = (HALO + THREADS - 1) / THREADS;
+    static_assert(TJ == 8 && RI % 2 == 0 && RI * RJ <= THREADS,
+                  "rows group by four; one thread per ring row for the k-halo red cells");
+    uint16_t *s16 = reinterpret_cast<uint16_t *>(s);
+    const Dims n = cell_dims(L.g);
+    const int lane = threadIdx.x & 63, wave = uniform((int)(threadIdx.x >> 6));
+    const int g = lane >> 4, t = lane & 15;  // the lane's row of a group of four, its pair (dword) in the row
+    const float cm2 = uniform(L.g.cell_m) * uniform(L.g.cell_m);
+    const float ghost = uniform(L.ghost);
+    bool open_lo[3], open_hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        open_lo[a] = uniform(L.g.open_lo[a]) == 1;
+        open_hi[a] = uniform(L.g.open_hi[a]) == 1;
+    }
+    auto row_inside = [&](int i, int j) { return (unsigned)i < (unsigned)n.n0 && (unsigned)j < (unsigned)n.n1; };
+    auto couple = [&](bool in, bool open) { return in ? 1.0f : open ? ghost : 0.0f; };
+    auto t1_xy = [&](int i, int j) {
+        if (!EDGE) return 4.0f;  // a ring row of an inner tile lies strictly inside the level
+        return ((couple(i > 0, open_lo[0]) + couple(i < n.n0 - 1, open_hi[0])) + couple(j > 0, open_lo[1])) +
+               couple(j < n.n1 - 1, open_hi[1]);
+    };
+    auto t1_z = [&](int k) { return couple(k > 0, open_lo[2]) + couple(k < n.n2 - 1, open_hi[2]); };
+    auto row_at = [&](int i, int j) { return (uint32_t)((i * n.n1 + j) * n.n2); };
+    // The lane's row in group G of a colour: its pair's dword `base` (dword 1 + t of part p), the row's parity
+    // p and its cells' (i, j). Black group G: tile row i0 + (G >> 1), j = j0 + 2 g + (G & 1), so a group's four
+    // rows are j, j + 2, j + 4, j + 6 of one i and parity. Red groups cover the (TI + 2) x 10 ring rows: the
+    // same for G < 2 RI (ring i = i0 - 1 + (G >> 1), j = j0 - 1 + 2 g + (G & 1)), then the ring's last two
+    // columns (j = j0 + 7 + c) in groups of four (i, c) pairs. The wave-uniform part of each index is a scalar
+    // (G is), the lane's part a constant.
+    struct Row {
+        int base, p, i, j;
+    };
+    const int lane_pair = 2 * g * ROWD + 1 + t;
+    const int left_row = (g >> 1) * LJ + (g & 1), left_p = ((g >> 1) + (g & 1)) & 1;
+    auto row_of = [&](int colour, int G) {
+        Row r;
+        if (colour == 1 || G < 2 * RI) {
+            const int ii = (G >> 1) + (colour == 1 ? TH : 1), jj = (G & 1) + (colour == 1 ? TH : 1);
+            r.p = (ii + jj + colour) & 1;
+            r.base = (ii * LJ + jj) * ROWD + r.p * PARTD + lane_pair;
+            r.i = i0 - TH + ii;
+            r.j = j0 - TH + jj + 2 * g;
+        } else {
+            const int ii = 2 * (G - 2 * RI) + 1, jj = TJ + 1;
+            r.p = left_p;
+            r.base = (ii * LJ + jj + left_row) * ROWD + left_p * PARTD + 1 + t;
+            r.i = i0 - TH + ii + (g >> 1);
+            r.j = j0 - TH + jj + (g & 1);
+        }
+        return r;
+    };
+    // 1 / t1 of the lane's two cells by parity on an inner tile's rows (t1 = 4 + its z couplings)
+    float lane_rcp[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+    if (!EDGE) {
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) lane_rcp[p][c] = __builtin_amdgcn_rcpf(4.0f + t1_z(k0 + 4 * t + p + 2 * c));
+    }
+    // Per-pair factors (1 / t1, -cell_m² rhs / t1) of each iteration's pair, formed before the first barrier;
+    // a pair that is not updated (outside the level) keeps write false.
+    auto pair_factors = [&](int colour, int group, uint32_t &rcp2, uint32_t &nsr2, bool &write) {
+        const Row r = row_of(colour, group);
+        write = !EDGE || (row_inside(r.i, r.j) && k0 + 4 * t < n.n2);
+        float ra = 0.0f, rb = 0.0f, ca = 0.0f, cb = 0.0f;
+        if (write) {
+            const uint2 r4 = *reinterpret_cast<const uint2 *>(rhs + row_at(r.i, r.j) + (uint32_t)(k0 + 4 * t));
+            ra = r.p ? half_hi(r4.x) : half_lo(r4.x);
+            rb = r.p ? half_hi(r4.y) : half_lo(r4.y);
+            if (EDGE) {
+                const float xy = t1_xy(r.i, r.j);
+                ca = __builtin_amdgcn_rcpf(xy + t1_z(k0 + 4 * t + r.p));
+                cb = __builtin_amdgcn_rcpf(xy + t1_z(k0 + 4 * t + r.p + 2));
+            } else {
+                ca = r.p ? lane_rcp[1][0] : lane_rcp[0][0];
+                cb = r.p ? lane_rcp[1][1] : lane_rcp[0][1];
+            }
+        }
+        rcp2 = pack_half2(ca, cb);
+        nsr2 = pack_half2((ra * -cm2) * ca, (rb * -cm2) * cb);
+    };
+    uint32_t red_rcp[RED_ITERS], red_nsr[RED_ITERS], black_rcp[BLACK_ITERS], black_nsr[BLACK_ITERS];
+    bool red_write[RED_ITERS], black_write[BLACK_ITERS];
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++) {
+        red_write[u] = false;
+        if (wave + u * WAVES < RED_GROUPS) pair_factors(0, wave + u * WAVES, red_rcp[u], red_nsr[u], red_write[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++) {
+        black_write[u] = false;
+        if (wave + u * WAVES < BLACK_GROUPS)
+            pair_factors(1, wave + u * WAVES, black_rcp[u], black_nsr[u], black_write[u]);
+    }
+    // the red cell of ring row threadIdx.x in the k halo: k0 - 1 when i + j is odd, else k0 + 64
+    int e_i = 0, e_j = 0, e_kk = 0;
+    bool e_in = false;
+    float e_rhs = 0.0f;
+    if (threadIdx.x < RI * RJ) {
+        e_i = i0 - 1 + (int)threadIdx.x / RJ;
+        e_j = j0 - 1 + (int)threadIdx.x % RJ;
+        e_kk = ((e_i + e_j) & 1) ? 1 : TK + 2;
+        const int k = k0 - TH + e_kk;
+        e_in = row_inside(e_i, e_j) && k >= 0 && k < n.n2;
+        if (e_in) e_rhs = (float)*reinterpret_cast<const _Float16 *>(rhs + row_at(e_i, e_j) + (uint32_t)k);
+    }
+    if (START != FROM_ZERO) {
+        // the rows' 64 cells, four a lane (one 8-byte load), split into the two parts; then the k halo, two
+        // cells (one dword) per load
+        uint2 v[LOAD_ITERS];
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; u++) {
+            v[u] = make_uint2(0u, 0u);
+            const int r = 4 * (wave + u * WAVES) + g, i = i0 - TH + r / LJ, j = j0 - TH + r % LJ;
+            if (wave + u * WAVES < LOAD_GROUPS && (!EDGE || (row_inside(i, j) && k0 + 4 * t < n.n2)))
+                v[u] = *reinterpret_cast<const uint2 *>(q_in + row_at(i, j) + (uint32_t)(k0 + 4 * t));
+        }
+        uint32_t h[HALO_ITERS];
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int tt = min((int)threadIdx.x + u * THREADS, HALO - 1), r = tt >> 1;
+            const int i = i0 - TH + r / LJ, j = j0 - TH + r % LJ, k = (tt & 1) ? k0 + TK : k0 - TH;
+            h[u] = 0u;
+            if ((!EDGE || row_inside(i, j)) && k >= 0 && k + 1 < n.n2)
+                h[u] = *reinterpret_cast<const uint32_t *>(q_in + row_at(i, j) + (uint32_t)k);
+        }
+#pragma unroll
+        for (int u = 0; u < LOAD_ITERS; u++) {
+            if (wave + u * WAVES >= LOAD_GROUPS) continue;
+            const int r = 4 * (wave + u * WAVES) + g;
+            s[r * ROWD + 1 + t] = __builtin_amdgcn_perm(v[u].y, v[u].x, 0x05040100u);
+            s[r * ROWD + PARTD + 1 + t] = __builtin_amdgcn_perm(v[u].y, v[u].x, 0x07060302u);
+        }
+#pragma unroll
+        for (int u = 0; u < HALO_ITERS; u++) {
+            const int tt = threadIdx.x + u * THREADS, r = tt >> 1, kk = (tt & 1) ? TK + 2 : 0;
+            if (tt >= HALO) continue;
+            s16[r * PK_RW + pk_slot(kk)] = (uint16_t)(h[u] & 0xffffu);
+            s16[r * PK_RW + pk_slot(kk + 1)] = (uint16_t)(h[u] >> 16);
+        }
+        __syncthreads();
+    }
+    // one pass of colour c over its rows: a lane's pair of row r (parity p) is dword 1 + t of part p; the
+    // other part's dwords t + p and t + p + 1 hold its k neighbours, one aligned, one across the two
+    auto pass = [&](int colour, int group, uint32_t rcp2, uint32_t nsr2, bool write) {
+        const Row r = row_of(colour, group);
+        const int p = r.p, base = r.base, other = base + (p ? -PARTD : PARTD - 1);
+        uint32_t value = 0u;
+        if (colour == 0 && START == FROM_ZERO) {
+            value = nsr2;  // every neighbour of a red cell is black, and black starts at 0
+        } else {
+            const half2_t im = as_half2(s[base - LJ * ROWD]), ip = as_half2(s[base + LJ * ROWD]);
+            const half2_t jm = as_half2(s[base - ROWD]), jp = as_half2(s[base + ROWD]);
+            const uint32_t x = s[other], y = s[other + 1];
+            const half2_t aligned = as_half2(p ? x : y), across = as_half2(__builtin_amdgcn_alignbit(y, x, 16));
+            const half2_t sum = ((im + ip) + (jm + jp)) + (aligned + across);
+            value = as_bits(__builtin_elementwise_fma(sum, as_half2(rcp2), as_half2(nsr2)));
+        }
+        // a red row outside the level keeps +0 (from zero: written, since LDS was not filled)
+        if (!EDGE || write) s[base] = value;
+        else if (colour == 0 && START == FROM_ZERO) s[base] = 0u;
+    };
+#pragma unroll
+    for (int u = 0; u < RED_ITERS; u++)
+        if (wave + u * WAVES < RED_GROUPS) pass(0, wave + u * WAVES, red_rcp[u], red_nsr[u], red_write[u]);
+    if (threadIdx.x < RI * RJ) {
+        const int lr = (e_i - i0 + TH) * LJ + (e_j - j0 + TH), at = lr * PK_RW + pk_slot(e_kk);
+        float value = 0.0f;
+        if (e_in) {
+            float sum = 0.0f;
+            if (START != FROM_ZERO) {
+                auto h16 = [&](int index) { return (float)*reinterpret_cast<const _Float16 *>(s16 + index); };
+                sum = ((((h16(at - LJ * PK_RW) + h16(at + LJ * PK_RW)) + h16(at - PK_RW)) + h16(at + PK_RW)) +
+                       h16(lr * PK_RW + pk_slot(e_kk - 1))) +
+                      h16(lr * PK_RW + pk_slot(e_kk + 1));
+            }
+            const float t1 = t1_xy(e_i, e_j) + t1_z(k0 - TH + e_kk);
+            value = (sum - cm2 * e_rhs) * __builtin_amdgcn_rcpf(t1);
+        }
+        if (e_in || START == FROM_ZERO) {
+            const _Float16 hv = (_Float16)value;
+            s16[at] = __builtin_bit_cast(uint16_t, hv);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BLACK_ITERS; u++)
+        if (wave + u * WAVES < BLACK_GROUPS) pass(1, wave + u * WAVES, black_rcp[u], black_nsr[u], black_write[u]);
+    __syncthreads();
+    // the tile's rows, four cells a lane: parts 0 and 1 interleaved back into one 8-byte store
+#pragma unroll
+    for (int u = 0; u < STORE_ITERS; u++) {
+        if (wave + u * WAVES >= BLACK_GROUPS) continue;
+        const int r = 4 * (wave + u * WAVES) + g, i = i0 + r / TJ, j = j0 + r % TJ;
+        const int lr = (r / TJ + TH) * LJ + (r % TJ + TH);
+        const uint32_t a = s[lr * ROWD + 1 + t], b = s[lr * ROWD + PARTD + 1 + t];
+        if (EDGE && !(row_inside(i, j) && k0 + 4 * t < n.n2)) continue;
+        *reinterpret_cast<uint2 *>(q_out + row_at(i, j) + (uint32_t)(k0 + 4 * t)) =
+            make_uint2(__builtin_amdgcn_perm(b, a, 0x05040100u), __builtin_amdgcn_perm(b, a, 0x07060302u));
+    }
+}
+
+template <int TI, int TJ, int THREADS, int START>
+__global__ __launch_bounds__(THREADS) void k_rb_sweep_f16x2(LevelArgs L, const __half *__restrict__ q_in,
+                                                            __half *__restrict__ q_out,
+                                                            const __half *__restrict__ rhs, FastDiv tiles_k,
+                                                            FastDiv tiles_j) {
+    __shared__ uint32_t s[(TI + 2 * TH) * (TJ + 2 * TH) * PK_RW / 2];
+    const Dims n = cell_dims(L.g);
+    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d, ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
+    const int k0 = tk * TK, j0 = tj * TJ, i0 = ti * TI;
+    if (i0 >= TH && i0 + TI + TH <= n.n0 && j0 >= TH && j0 + TJ + TH <= n.n1 && k0 + TK <= n.n2)
+        sweep_tile_f16x2<TI, TJ, THREADS, START, false>(L, s, q_in, q_out, rhs, i0, j0, k0);
+    else
+        sweep_tile_f16x2<TI, TJ, THREADS, START, true>(L, s, q_in, q_out, rhs, i0, j0, k0);
+}
+
+// The prolongation of a packed level in FP16 storage (the values of k_prolong_rows<__half>, bit for bit):
+// q = half(float(q) + p) with p the trilinear interpolation of the coarse correction
+// (prolongation_kernel, sim/multigrid.py:63-80). A thread takes four fine cells along k, k = 4t .. 4t + 3,
+// whose interpolation reads coarse k 2t - 1 .. 2t + 2: three dwords of each of the four coarse rows around
+// (i, j). Each coarse k's x-then-y interpolation is formed once and shared by the fine cells that read it;
+// the z step clamps at the level's two ends as the scalar form does. Needs the fine k extent a multiple of 4.
+__global__ __launch_bounds__(256) void k_prolong_f16x2(const __half *__restrict__ coarse, Dims nc,
+                                                       __half *__restrict__ fine, Dims nf) {
+    const int t = blockIdx.x * BX + threadIdx.x;
+    const int j = uniform((int)(blockIdx.y * BY + threadIdx.y)), i = blockIdx.z;
+    if (j >= nf.n1 || 4 * t >= nf.n2) return;
+    const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nc.n0 - 1));
+    const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nc.n1 - 1));
+    const int ic0 = (int)x, jc0 = (int)y, ic1 = min(ic0 + 1, nc.n0 - 1), jc1 = min(jc0 + 1, nc.n1 - 1);
+    const float tx = x - (float)ic0, ty = y - (float)jc0;
+    // coarse dwords 2t - 2 .. 2t + 3 of a row, clamped into the row (a clamped value is never selected)
+    const int last_dword = nc.n2 / 2 - 1;
+    const int d0 = max(t - 1, 0), d2 = min(t + 1, last_dword);
+    auto row = [&](int ic, int jc, float (&v)[6]) {
+        const uint32_t *r = reinterpret_cast<const uint32_t *>(coarse + (uint32_t)((ic * nc.n1 + jc) * nc.n2));
+        const uint32_t a = r[d0], b = r[t], c = r[d2];
+        v[0] = half_lo(a), v[1] = half_hi(a), v[2] = half_lo(b), v[3] = half_hi(b), v[4] = half_lo(c);
+        v[5] = half_hi(c);
+    };
+    float A[6], B[6], C[6], D[6];
+    row(ic0, jc0, A);
+    row(ic1, jc0, B);
+    row(ic0, jc1, C);
+    row(ic1, jc1, D);
+    // Y[m]: coarse k 2t - 2 + m interpolated in x, then y (prolong_at_row's c00 / c10 then their y step)
+    float Y[5];
+#pragma unroll
+    for (int m = 1; m < 5; m++) Y[m] = lerp_(lerp_(A[m], B[m], tx), lerp_(C[m], D[m], tx), ty);
+    // fine k = 4t + s: coarse (2t - 1, 2t) at 0.75, (2t, 2t + 1) at 0.25 and 0.75, (2t + 1, 2t + 2) at 0.25;
+    // at k = 0 the clamped z = 0 takes (0, 1) at 0, at the last coarse k (2t + 1 = n2c - 1) z clamps to it
+    const bool first = t == 0, last = 2 * t + 1 == nc.n2 - 1;
+    const float p0 = first ? lerp_(Y[2], Y[3], 0.0f) : lerp_(Y[1], Y[2], 0.75f);
+    const float p1 = lerp_(Y[2], Y[3], 0.25f), p2 = lerp_(Y[2], Y[3], 0.75f);
+    const float p3 = last ? lerp_(Y[3], Y[3], 0.0f) : lerp_(Y[3], Y[4], 0.25f);
+    uint2 *cell = reinterpret_cast<uint2 *>(fine + (uint32_t)((i * nf.n1 + j) * nf.n2 + 4 * t));
+    const uint2 q = *cell;
+    *cell = make_uint2(pack_half2(half_lo(q.x) + p0, half_hi(q.x) + p1),
+                       pack_half2(half_lo(q.y) + p2, half_hi(q.y) + p3));
+}
+
+// residual_kernel + restrict_kernel (sim/multigrid.py:36-60) on a packed level in FP16 storage, one thread per
+// coarse cell: the values of k_residual_restrict<true, __half> on the simulation level and of
+// k_residual_restrict_children<__half> on coarser flat levels, bit for bit. The children's rows are read as
+// dwords: the four centre rows at fine k 2k - 2 .. 2k + 3, the eight side rows and the four rhs rows at 2k,
+// 2k + 1. As in the scalar simulation-level kernel every read is unconditional (rows clamped into the level,
+// neighbours outside it masked to +0.0, an overhang child adding +0.0 uncounted), so no wavefront runs two
+// paths. A child's couplings on a flat level are 1 to a neighbour in the level, the ghost value across an open
+// face and 0 across a closed one, summed in the reference's order; its volume is 1.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_residual_restrict_f16x2(LevelArgs L, const __half *__restrict__ q,
+                                                                 const __half *__restrict__ rhs,
+                                                                 __half *__restrict__ coarse_rhs, Dims nc) {
+    const Dims nf = cell_dims(L.g);
+    const int k = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, i = blockIdx.z;
+    if (!(k < nc.n2 && j < nc.n1 && i < nc.n0)) return;
+    const float scale = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m
+    const float ghost = UNIT ? 1.0f : uniform(L.ghost);
+    bool open_lo[3], open_hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        open_lo[a] = uniform(L.g.open_lo[a]) == 1;
+        open_hi[a] = uniform(L.g.open_hi[a]) == 1;
+    }
+    // fine rows x = 2i - 1 + m and y = 2j - 1 + m (m = 0 .. 3), clamped; fine k 2k - 1 + m, the middle two
+    // always in the level (its k extent is even)
+    int ri[4], rj[4];
+    bool in_i[4], in_j[4], in_k[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int x = 2 * i - 1 + m, y = 2 * j - 1 + m;
+        in_i[m] = x >= 0 && x < nf.n0;
+        in_j[m] = y >= 0 && y < nf.n1;
+        ri[m] = max(0, min(x, nf.n0 - 1));
+        rj[m] = max(0, min(y, nf.n1 - 1));
+    }
+    in_k[0] = k >= 1;
+    in_k[1] = in_k[2] = true;
+    in_k[3] = 2 * k + 2 < nf.n2;
+    const int dk_lo = max(k - 1, 0), dk_hi = min(k + 1, nf.n2 / 2 - 1);
+    auto row = [&](const __half *f, int mi, int mj) {
+        return reinterpret_cast<const uint32_t *>(f + (uint32_t)((ri[mi] * nf.n1 + rj[mj]) * nf.n2));
+    };
+    // q at child-block coordinates (a, b, c) in [-1, 2]^3 on the cross around the block: the centre rows' four
+    // k values and the side rows' two
+    float z[2][2][4], xs[2][2][2], ys[2][2][2], rc[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const uint32_t *r = row(q, a + 1, b + 1);
+            const uint32_t lo = r[dk_lo], mid = r[k], hi = r[dk_hi];
+            z[a][b][0] = half_hi(lo), z[a][b][1] = half_lo(mid), z[a][b][2] = half_hi(mid), z[a][b][3] = half_lo(hi);
+            const uint32_t h = row(rhs, a + 1, b + 1)[k];
+            rc[a][b][0] = half_lo(h), rc[a][b][1] = half_hi(h);
+            const uint32_t x = row(q, a == 0 ? 0 : 3, b + 1)[k];  // the x side beyond child a of row b
+            xs[a][b][0] = half_lo(x), xs[a][b][1] = half_hi(x);
+            const uint32_t y = row(q, a + 1, b == 0 ? 0 : 3)[k];  // the y side beyond child b of row a
+            ys[a][b][0] = half_lo(y), ys[a][b][1] = half_hi(y);
+        }
+    auto Q = [&](int a, int b, int c) {
+        if (a < 0 || a > 1) return xs[a < 0 ? 0 : 1][b][c];
+        if (b < 0 || b > 1) return ys[a][b < 0 ? 0 : 1][c];
+        return z[a][b][c + 1];
+    };
+    auto couple = [&](bool in, bool open) { return in ? 1.0f : open ? ghost : 0.0f; };
+    float total = 0.0f;
+    int count = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float qn[6] = {Q(a - 1, b, c), Q(a + 1, b, c), Q(a, b - 1, c),
+                                     Q(a, b + 1, c), Q(a, b, c - 1), Q(a, b, c + 1)};
+                const bool in[6] = {in_i[a], in_i[a + 2], in_j[b], in_j[b + 2], in_k[c], in_k[c + 2]};
+                float t0 = 0.0f, t1 = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 6; m++) {
+                    t0 = t0 + (in[m] ? qn[m] : 0.0f);
+                    t1 = t1 + couple(in[m], (m & 1) ? open_hi[m >> 1] : open_lo[m >> 1]);
+                }
+                const float r = rc[a][b][c] - (t0 - t1 * Q(a, b, c)) / scale;
+                const bool exists = in_i[a + 1] && in_j[b + 1];
+                total = total + (exists ? r : 0.0f);
+                count += exists ? 1 : 0;
+            }
+    coarse_rhs[nc.at(i, j, k)] = from_f32<__half>(total / (float)count);
+}
+
+// The packed shapes: TI, TJ (k is 64 wide), threads per block.
+constexpr TileShape F16X2_SHAPES[] = {{8, 8, 256}, {8, 8, 512}, {4, 8, 256}};
+constexpr int N_F16X2_SHAPES = 3;
+int f16x2_shape = 1;  // the shape packed sweeps launch with (plm_set_f16x2_shape; 8 x 8 on 512 measured fastest)
+
+template <int START>
+int launch_f16x2(const LevelArgs &L, const __half *src, __half *dst, const __half *rhs, hipStream_t s) {
+    const Dims n = cell_dims(L.g);
+    const TileShape &ts = F16X2_SHAPES[f16x2_shape];
+    const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + ts.tj - 1) / ts.tj, tiles_i = (n.n0 + ts.ti - 1) / ts.ti;
+    const dim3 grid((unsigned)(tiles_k * tiles_j * tiles_i));
+    const FastDiv div_k = fast_div(tiles_k), div_j = fast_div(tiles_j);
+    TimingScope launch_timing("rb_sweep_f16x2", s, true);
+    const hipEvent_t e0 = launch_timing.start(), e1 = launch_timing.stop();
+#define PLM_F16X2(TI, TJ, THREADS)                                                                                   \
+    do {                                                                                                             \
+        if (e0 != nullptr)                                                                                           \
+            hipExtLaunchKernelGGL(k_rb_sweep_f16x2<TI, TJ, THREADS, START>, grid, dim3(THREADS), 0, s, e0, e1, 0u,  \
+                                  L, src, dst, rhs, div_k, div_j);                                                   \
+        else                                                                                                         \
+            k_rb_sweep_f16x2<TI, TJ, THREADS, START><<<grid, THREADS, 0, s>>>(L, src, dst, rhs, div_k, div_j);     \
+    } while (0)
+    switch (f16x2_shape) {
+    case 1: PLM_F16X2(8, 8, 512); break;
+    case 2: PLM_F16X2(4, 8, 256); break;
+    default: PLM_F16X2(8, 8, 256); break;
+    }
+#undef PLM_F16X2
+    return check_launch("packed FP16 sweep");
+}
+
 template <typename T>
 struct LpHierarchy {
     int n_levels;
@@ -2264,17 +2698,53 @@ struct LpHierarchy {
     T *const *rhs;
     T *const *scratch;
     int sweeps;
+    bool packed;  // the simulation level's sweeps in packed FP16 arithmetic (T = __half)
 };
+
+// q += prolongation(coarse) on one level in storage T (k_prolong_f16x2 on a packed level)
+template <typename T>
+int lp_prolong(const LevelArgs &L, T *q, const T *coarse, Dims nc, hipStream_t s, bool packed) {
+    const Dims n = cell_dims(L.g);
+    if constexpr (std::is_same<T, __half>::value) {
+        if (packed && packed_f16_level(L)) {
+            const dim3 grid((unsigned)((n.n2 / 4 + BX - 1) / BX), (unsigned)((n.n1 + BY - 1) / BY), (unsigned)n.n0);
+            k_prolong_f16x2<<<grid, block3(), 0, s>>>(coarse, nc, q, n);
+            return check_launch("packed prolongation");
+        }
+    }
+    k_prolong_rows<T><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, nullptr);
+    return check_launch("prolong rows");
+}
+
+// coarse_rhs = restrict(rhs - A q) of one level in storage T (k_residual_restrict_f16x2 on a packed level)
+template <typename T>
+int lp_residual_restrict(const LevelArgs &L, const T *q, const T *rhs, T *coarse_rhs, Dims nc, hipStream_t s,
+                         bool packed) {
+    if constexpr (std::is_same<T, __half>::value) {
+        // large levels (the per-child kernel keeps small levels' latency lower)
+        if (packed && packed_f16_level(L) && cell_dims(L.g).count() >= PROLONG_ROWS_CELLS) {
+            if (L.unit) k_residual_restrict_f16x2<true><<<grid3(nc), block3(), 0, s>>>(L, q, rhs, coarse_rhs, nc);
+            else k_residual_restrict_f16x2<false><<<grid3(nc), block3(), 0, s>>>(L, q, rhs, coarse_rhs, nc);
+            return check_launch("packed residual restrict");
+        }
+    }
+    if (L.unit)
+        k_residual_restrict<true, T><<<grid3(nc), block3(), 0, s>>>(L, q, rhs, coarse_rhs, nullptr, nc, nullptr);
+    else
+        k_residual_restrict_children<T><<<children_grid(nc), 256, 0, s>>>(L, q, rhs, coarse_rhs, nc, nullptr);
+    return check_launch("residual restrict");
+}
 
 // `sweeps` (>= 1) two-colour tile sweeps of one level in storage T, ping-ponging q and scratch
 template <typename T>
 int lp_sweeps(const LevelArgs &L, T *q, const T *rhs, T *scratch, int sweeps, int start, const T *coarse, Dims nc,
-              hipStream_t s) {
-    // large levels: the prolongation as its own row kernel (q + p rounded into T), as in launch_sweeps
-    if (start == FROM_PROLONG && cell_dims(L.g).count() >= PROLONG_ROWS_CELLS) {
-        const Dims n = cell_dims(L.g);
-        k_prolong_rows<T><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, nullptr);
-        PLM_TRY(check_launch("prolong rows"));
+              hipStream_t s, bool packed = false) {
+    // packed FP16 arithmetic where the level allows it (the unit-coupling simulation level)
+    packed = packed && std::is_same<T, __half>::value && packed_f16_level(L);
+    // large levels, and every packed level: the prolongation as its own row kernel (q + p rounded into T), as
+    // in launch_sweeps
+    if (start == FROM_PROLONG && (packed || cell_dims(L.g).count() >= PROLONG_ROWS_CELLS)) {
+        PLM_TRY(lp_prolong<T>(L, q, coarse, nc, s, packed));
         start = FROM_Q;
     }
     for (int sw = 0; sw < sweeps; sw++) {
@@ -2282,6 +2752,13 @@ int lp_sweeps(const LevelArgs &L, T *q, const T *rhs, T *scratch, int sweeps, in
         const T *src = to_scratch ? q : scratch;
         T *dst = to_scratch ? scratch : q;
         const int first = sw == 0 ? start : FROM_Q;
+        if constexpr (std::is_same<T, __half>::value) {
+            if (packed) {
+                if (first == FROM_ZERO) PLM_TRY(launch_f16x2<FROM_ZERO>(L, src, dst, rhs, s));
+                else PLM_TRY(launch_f16x2<FROM_Q>(L, src, dst, rhs, s));
+                continue;
+            }
+        }
         if (L.unit)
             PLM_TRY((launch_tile<true, T>(L, first, src, dst, rhs, coarse, nc, 1.0f, nullptr, s)));
         else
@@ -2314,24 +2791,20 @@ int v_cycle_lp(const LpHierarchy<T> &h, int index, bool cleared, hipStream_t s) 
     }
     const Dims none{1, 1, 1};
     PLM_TRY(lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, cleared ? FROM_ZERO : FROM_Q,
-                         nullptr, none, s));
+                         nullptr, none, s, h.packed));
     const Dims nc = cell_dims(h.level[index + 1].g);
-    if (L.unit)
-        k_residual_restrict<true, T><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
-                                                                   nullptr, nc, nullptr);
-    else
-        k_residual_restrict_children<T><<<children_grid(nc), 256, 0, s>>>(L, h.q[index], h.rhs[index],
-                                                                         h.rhs[index + 1], nc, nullptr);
-    PLM_TRY(check_launch("residual restrict"));
+    PLM_TRY(lp_residual_restrict<T>(L, h.q[index], h.rhs[index], h.rhs[index + 1], nc, s, h.packed));
     PLM_TRY(v_cycle_lp(h, index + 1, true, s));
-    return lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, FROM_PROLONG, h.q[index + 1], nc, s);
+    return lp_sweeps<T>(L, h.q[index], h.rhs[index], h.scratch[index], h.sweeps, FROM_PROLONG, h.q[index + 1], nc, s,
+                        h.packed);
 }
 
 template <typename T>
 int vcycle_lp(int32_t n_levels, const plm_grid_info *infos, void *const *q, void *const *rhs, void *const *scratch,
-              int32_t cycles, int32_t sweeps, int32_t zero_start, hipStream_t s) {
+              int32_t cycles, int32_t sweeps, int32_t zero_start, hipStream_t s, bool packed = false) {
     LpHierarchy<T> h{};
     h.n_levels = n_levels;
+    h.packed = packed;
     for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
     h.q = reinterpret_cast<T *const *>(q);
     h.rhs = reinterpret_cast<T *const *>(rhs);
@@ -3205,7 +3678,7 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
 int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage, void *const *q, void *const *rhs,
                   void *const *scratch, int32_t cycles, int32_t sweeps, int32_t zero_start, void *stream) {
     PLM_REQUIRE(n_levels >= 2 && n_levels <= 16 && infos && q && rhs && scratch, "plm_vcycle_lp: bad level arrays");
-    PLM_REQUIRE(storage >= PLM_STORAGE_F32 && storage <= PLM_STORAGE_F16, "plm_vcycle_lp: storage %d unknown",
+    PLM_REQUIRE(storage >= PLM_STORAGE_F32 && storage <= PLM_STORAGE_F16_PACKED, "plm_vcycle_lp: storage %d unknown",
                 storage);
     PLM_REQUIRE(cycles >= 0 && sweeps >= 1, "plm_vcycle_lp: cycles must be >= 0 and sweeps >= 1");
     for (int l = 0; l < n_levels; l++) {
@@ -3222,12 +3695,72 @@ int plm_vcycle_lp(int32_t n_levels, const plm_grid_info *infos, int32_t storage,
     hipStream_t s = as_stream(stream);
     if (storage == PLM_STORAGE_BF16)
         return vcycle_lp<__hip_bfloat16>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
-    if (storage == PLM_STORAGE_F16)
-        return vcycle_lp<__half>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
+    if (storage == PLM_STORAGE_F16 || storage == PLM_STORAGE_F16_PACKED)
+        return vcycle_lp<__half>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s,
+                                 storage == PLM_STORAGE_F16_PACKED);
     return vcycle_lp<float>(n_levels, infos, q, rhs, scratch, cycles, sweeps, zero_start, s);
 }
 
+int plm_rb_sweeps_lp(const plm_grid_info *info, int32_t storage, void *q, const void *rhs, void *scratch,
+                     int32_t sweeps, int32_t zero_start, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(q && rhs && scratch && sweeps >= 1, "plm_rb_sweeps_lp: NULL array or sweeps < 1");
+    const LevelArgs L = level_args(*info);
+    hipStream_t s = as_stream(stream);
+    const int start = zero_start ? FROM_ZERO : FROM_Q;
+    const Dims none{1, 1, 1};
+    if (storage == PLM_STORAGE_BF16)
+        return lp_sweeps<__hip_bfloat16>(L, (__hip_bfloat16 *)q, (const __hip_bfloat16 *)rhs, (__hip_bfloat16 *)scratch,
+                                         sweeps, start, nullptr, none, s);
+    if (storage == PLM_STORAGE_F16 || storage == PLM_STORAGE_F16_PACKED)
+        return lp_sweeps<__half>(L, (__half *)q, (const __half *)rhs, (__half *)scratch, sweeps, start, nullptr, none, s,
+                                 storage == PLM_STORAGE_F16_PACKED);
+    if (storage == PLM_STORAGE_F32)
+        return lp_sweeps<float>(L, (float *)q, (const float *)rhs, (float *)scratch, sweeps, start, nullptr, none, s);
+    return fail(PLM_EINVAL, "plm_rb_sweeps_lp: storage %d unknown", storage);
+}
+
+int plm_residual_restrict_lp(const plm_grid_info *info, int32_t storage, const void *q, const void *rhs,
+                             void *coarse_rhs, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(q && rhs && coarse_rhs, "plm_residual_restrict_lp: NULL array");
+    const LevelArgs L = level_args(*info);
+    const Dims n = cell_dims(*info), nc{(n.n0 + 1) / 2, (n.n1 + 1) / 2, (n.n2 + 1) / 2};
+    hipStream_t s = as_stream(stream);
+    if (storage == PLM_STORAGE_BF16)
+        return lp_residual_restrict<__hip_bfloat16>(L, (const __hip_bfloat16 *)q, (const __hip_bfloat16 *)rhs,
+                                                    (__hip_bfloat16 *)coarse_rhs, nc, s, false);
+    if (storage == PLM_STORAGE_F16 || storage == PLM_STORAGE_F16_PACKED)
+        return lp_residual_restrict<__half>(L, (const __half *)q, (const __half *)rhs, (__half *)coarse_rhs, nc, s,
+                                            storage == PLM_STORAGE_F16_PACKED);
+    if (storage == PLM_STORAGE_F32)
+        return lp_residual_restrict<float>(L, (const float *)q, (const float *)rhs, (float *)coarse_rhs, nc, s, false);
+    return fail(PLM_EINVAL, "plm_residual_restrict_lp: storage %d unknown", storage);
+}
+
+int plm_prolong_add_lp(const plm_grid_info *info, int32_t storage, const void *coarse, void *q, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(q && coarse, "plm_prolong_add_lp: NULL array");
+    const LevelArgs L = level_args(*info);
+    const Dims n = cell_dims(*info), nc{(n.n0 + 1) / 2, (n.n1 + 1) / 2, (n.n2 + 1) / 2};
+    hipStream_t s = as_stream(stream);
+    if (storage == PLM_STORAGE_BF16)
+        return lp_prolong<__hip_bfloat16>(L, (__hip_bfloat16 *)q, (const __hip_bfloat16 *)coarse, nc, s, false);
+    if (storage == PLM_STORAGE_F16 || storage == PLM_STORAGE_F16_PACKED)
+        return lp_prolong<__half>(L, (__half *)q, (const __half *)coarse, nc, s, storage == PLM_STORAGE_F16_PACKED);
+    if (storage == PLM_STORAGE_F32)
+        return lp_prolong<float>(L, (float *)q, (const float *)coarse, nc, s, false);
+    return fail(PLM_EINVAL, "plm_prolong_add_lp: storage %d unknown", storage);
+}
+
+int plm_set_f16x2_shape(int32_t shape) {
+    PLM_REQUIRE(shape >= 0 && shape < N_F16X2_SHAPES, "plm_set_f16x2_shape: shape %d out of range", shape);
+    f16x2_shape = shape;
+    return 0;
+}
+
 int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage, void *dst, void *stream) {
+    if (storage == PLM_STORAGE_F16_PACKED) storage = PLM_STORAGE_F16;  // the same FP16 storage
     PLM_REQUIRE(count >= 0 && src && dst, "plm_to_storage: bad arguments");
     hipStream_t s = as_stream(stream);
     if (count == 0) return 0;
@@ -3243,6 +3776,7 @@ int plm_to_storage(int64_t count, const float *src, float scale, int32_t storage
 }
 
 int plm_add_storage(int64_t count, float *x, const void *e, int32_t storage, float inv_scale, void *stream) {
+    if (storage == PLM_STORAGE_F16_PACKED) storage = PLM_STORAGE_F16;  // the same FP16 storage
     PLM_REQUIRE(count >= 0 && x && e, "plm_add_storage: bad arguments");
     hipStream_t s = as_stream(stream);
     if (count == 0) return 0;
@@ -3270,6 +3804,7 @@ int plm_refine_residual(const plm_grid_info *info, const float *x, const void *e
     hipStream_t s = as_stream(stream);
     const LevelArgs L = level_args(*info);
     TimingScope t("refine_residual", s);
+    if (storage == PLM_STORAGE_F16_PACKED) storage = PLM_STORAGE_F16;  // the same FP16 storage
     if (storage == PLM_STORAGE_BF16)
         PLM_TRY(launch_refine<__hip_bfloat16>(L, x, e, inv_scale, rhs, x_out, r_out, r_scale, scratch, s));
     else if (storage == PLM_STORAGE_F16)

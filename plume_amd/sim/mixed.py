@@ -8,8 +8,9 @@ until |b - A x| / |b| <= tolerance, the norms reduced in FP64 on the device. Eac
 2. scales r by a power of two (exact) that brings its RMS near 1, so FP16 neither overflows nor underflows,
    and rounds it into the storage type (BF16 or FP16, round to nearest even);
 3. runs `inner_cycles` V(sweeps, sweeps) cycles of the reference's V-cycle on A e = r from e = 0, every
-   level's q and rhs stored in that type, FP32 arithmetic in the sweeps, FP32 residuals inside the fused
-   residual + restrict, and the coarsest level solved in FP64 (plm_vcycle_lp);
+   level's q and rhs stored in that type, FP32 arithmetic in the sweeps (form "f16x2": the flat levels' sweeps
+   in packed FP16, two cells per lane), FP32 residuals inside the fused residual + restrict, and the
+   coarsest level solved in FP64 (plm_vcycle_lp);
 4. adds e back into x in FP32.
 
 Steps 4, 1 and 2 of consecutive rounds and the FP64 norms run as one kernel (plm_refine_residual), which
@@ -37,9 +38,11 @@ import torch
 from plume_amd._lib import LIB, GridInfo, call, ptr, stream
 from plume_amd.sim.multigrid import MAX_LEVELS, build_levels
 
-STORAGE = {"f32": 0, "bf16": 1, "f16": 2}  # PLM_STORAGE_* in include/plume_hip.h
+# PLM_STORAGE_* in include/plume_hip.h; "f16x2" stores FP16 and sweeps the flat levels (the simulation level
+# and the coarser levels of even-sized grids) in packed FP16 arithmetic (two cells per lane)
+STORAGE = {"f32": 0, "bf16": 1, "f16": 2, "f16x2": 3}
 STALL = 0.5  # a round that does not at least halve the residual ends the solve (V(2,2) and stronger cycles)
-DTYPES = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}
+DTYPES = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16, "f16x2": torch.float16}
 
 
 def _vptr(t: torch.Tensor) -> int:
@@ -254,10 +257,10 @@ def divergent_problem(cells, device: str, seed: int = 20261208) -> Problem:
     return Problem(grid, [f.data.clone() for f in grid.velocity_fields()], grid.divergence.clone())
 
 
-def solve_forms(problem: Problem, target: float, forms=("f32", "bf16", "f16"), max_levels: int = 6,
+def solve_forms(problem: Problem, target: float, forms=("f32", "bf16", "f16", "f16x2"), max_levels: int = 6,
                 repeats: int = 2, sweeps: int = 2) -> dict:
     """Solves the problem from q = 0 to `target` with each form: "f32" the plain FP32 multigrid in tolerance
-    mode (autotuned tiles), "bf16" / "f16" iterative refinement around that storage, every form with
+    mode (autotuned tiles), "bf16" / "f16" / "f16x2" iterative refinement around that storage, every form with
     V(sweeps, sweeps) cycles. The last of `repeats`
     solves is timed (the first warms the code paths). Returns {form: (entry, pressure)} where entry holds the
     seconds, cells per second, FP64 relative residual and post-projection divergence |div u| / |div u0|."""

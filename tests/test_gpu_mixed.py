@@ -8,7 +8,7 @@ solution rounded to FP32: no solve that keeps the pressure in FP32 gets below it
 import numpy as np
 import pytest
 
-from support import DEFAULT_FACES, gpu_grid, oracle_grid
+from support import ALL_CLOSED, ALL_OPEN, DEFAULT_FACES, gpu_grid, oracle_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,115 @@ def test_fused_refinement_step_equals_the_unfused_sequence(device, oracle64, sto
              1.0, ptr(scratch, torch.float64), ptr(norms, torch.float64), stream())
 
 
+PACKED_CASES = [((48, 64, 40), DEFAULT_FACES), ((36, 36, 128), ALL_CLOSED), ((26, 34, 68), ALL_OPEN),
+                ((30, 20, 64), DEFAULT_FACES)]
+
+
+@pytest.mark.parametrize("cells, faces", PACKED_CASES, ids=["48x64x40", "36x36x128-closed", "26x34x68-open",
+                                                            "30x20x64"])
+@pytest.mark.parametrize("sweeps, zero_start", [(1, 0), (2, 0), (2, 1)], ids=["one", "two", "two-from-zero"])
+@pytest.mark.parametrize("scale", [1.0, 1e-4], ids=["unit", "small"])
+@pytest.mark.parametrize("level", [0, 1])
+def test_packed_fp16_sweeps_match_the_fp64_sweeps(device, oracle64, cells, faces, sweeps, zero_start, scale, level):
+    """plm_rb_sweeps_lp in packed FP16 arithmetic (PLM_STORAGE_F16_PACKED) against the FP64 oracle's
+    red_black_sweeps (sim/pressure.py:80-92, relaxation 1) from the same FP16 values: within 16 roundings to FP16
+    (11 significant bits, and 2^-25 absolute where the values are subnormal) of the largest value, as is the
+    FP32-arithmetic FP16 sweep. A wrong neighbour, parity, coupling or cell_m² is off by the values themselves.
+    The grids hold inner tiles (all eight rows and columns around them in the level, 64 whole cells along k)
+    and boundary tiles, k extents of 40 to 128 cells (partial and whole 64-cell tiles) and every face kind; the
+    "small" scale puts cell_m² rhs / t1 among FP16's subnormals, as in the correction solve. Level 1 is the
+    grid's first coarser level (flat, open faces coupling by 1 / (0.5 + ghost_gap), 2 cm cells; 13 x 17 x 34
+    sweeps unpacked, its k extent not a multiple of 4)."""
+    import ctypes
+
+    import torch
+
+    from oracle.sim import level_infos
+    from plume_amd._lib import call, stream
+    from plume_amd.sim.mixed import STORAGE
+    from plume_amd.sim.multigrid import build_levels
+
+    cm = 0.01
+    gpu = gpu_grid(cells, cm, faces, device=device)
+    orc = oracle_grid(oracle64, cells, cm, faces)
+    info = build_levels(gpu, 8)[level].info
+    cells, orc_info = level_infos(oracle64, cells, cm, orc.origin_m, orc.open_lo, orc.open_hi)[level]
+    assert tuple(info.dims) == tuple(cells)
+    r = np.random.default_rng(20261203 + 41)
+    q0 = (r.uniform(-1, 1, cells) * scale).astype(np.float16)
+    rhs = (r.uniform(-1, 1, cells) * (2.0 * scale / cm**2)).astype(np.float16)  # cell_m² rhs up to 2 |q|
+    expected = np.zeros(cells) if zero_start else q0.astype(np.float64)
+    oracle64.rb_sweeps(orc_info, expected, rhs.astype(np.float64), sweeps, 1.0)
+    # cell_m² rhs up to 2 |q| on level 0; (2 cell_m)² rhs up to 8 |q| on level 1
+    bound = 16.0 * (2.0 ** -11 * float(np.abs(expected).max()) + 2.0 ** -25)
+    results = {}
+    rhs_d = torch.from_numpy(rhs).to(device)
+    for storage in ("f16", "f16x2"):
+        q = torch.from_numpy(q0).to(device)
+        scratch = torch.full_like(q, float("nan"))
+        call("plm_rb_sweeps_lp", ctypes.byref(info), STORAGE[storage], q.data_ptr(), rhs_d.data_ptr(),
+             scratch.data_ptr(), sweeps, zero_start, stream())
+        got = q.cpu().numpy().astype(np.float64)
+        assert np.all(np.isfinite(got)), storage
+        results[storage] = float(np.abs(got - expected).max())
+    print(cells, faces, sweeps, zero_start, scale, level, "max |diff| vs FP64:", results, "bound", bound)
+    for storage, worst in results.items():
+        assert worst <= bound, (storage, worst, bound)
+
+
+@pytest.mark.parametrize("cells, faces", PACKED_CASES, ids=["48x64x40", "36x36x128-closed", "26x34x68-open",
+                                                            "30x20x64"])
+@pytest.mark.parametrize("level", [0, 1])
+def test_packed_transfers_equal_the_fp16_kernels(device, oracle64, cells, faces, level):
+    """The packed level's residual + restrict and prolongation (PLM_STORAGE_F16_PACKED: dword rows, each coarse
+    k's interpolation shared by the fine cells that read it) give the FP16 kernels' values bit for bit, and
+    those match the FP64 oracle's residual_kernel + restrict_kernel and prolongation_kernel
+    (sim/multigrid.py:36-80) within FP16 rounding. Level 1 of 30 x 20 x 64 has an odd i extent (15); level 1
+    of 26 x 34 x 68 is not packed (k extent 34) and takes the FP16 kernels either way."""
+    import ctypes
+
+    import torch
+
+    from oracle.sim import level_infos
+    from plume_amd._lib import call, stream
+    from plume_amd.sim.mixed import STORAGE
+    from plume_amd.sim.multigrid import build_levels
+
+    cm = 0.01
+    gpu = gpu_grid(cells, cm, faces, device=device)
+    orc = oracle_grid(oracle64, cells, cm, faces)
+    info = build_levels(gpu, 8)[level].info
+    fine, orc_info = level_infos(oracle64, cells, cm, orc.origin_m, orc.open_lo, orc.open_hi)[level]
+    coarse = tuple((n + 1) // 2 for n in fine)
+    r = np.random.default_rng(20261203 + 43)
+    h2 = (cm * 2**level) ** 2  # the level's cell_m²: q of that size keeps A q / cell_m² near 1
+    q = (r.uniform(-1, 1, fine) * h2).astype(np.float16)
+    rhs = (r.uniform(-1, 1, fine) * 10.0).astype(np.float16)
+    correction = (r.uniform(-1, 1, coarse) * h2).astype(np.float16)
+    restricted, prolonged = {}, {}
+    q_d, rhs_d, correction_d = (torch.from_numpy(a).to(device) for a in (q, rhs, correction))
+    for storage in ("f16", "f16x2"):
+        out = torch.full(coarse, float("nan"), dtype=torch.float16, device=device)
+        call("plm_residual_restrict_lp", ctypes.byref(info), STORAGE[storage], q_d.data_ptr(), rhs_d.data_ptr(),
+             out.data_ptr(), stream())
+        restricted[storage] = out.cpu().numpy()
+        target = q_d.clone()
+        call("plm_prolong_add_lp", ctypes.byref(info), STORAGE[storage], correction_d.data_ptr(), target.data_ptr(),
+             stream())
+        prolonged[storage] = target.cpu().numpy()
+    assert np.array_equal(restricted["f16"].view(np.uint16), restricted["f16x2"].view(np.uint16))
+    assert np.array_equal(prolonged["f16"].view(np.uint16), prolonged["f16x2"].view(np.uint16))
+    residual = np.zeros(fine)
+    oracle64.residual(orc_info, q.astype(np.float64), rhs.astype(np.float64), residual)
+    expected = np.zeros(coarse)
+    oracle64.restrict(residual, expected)
+    got = restricted["f16x2"].astype(np.float64)
+    assert np.abs(got - expected).max() <= 2.0 ** -9 * np.abs(expected).max(), np.abs(got - expected).max()
+    expected_q = q.astype(np.float64)
+    oracle64.prolong_add(correction.astype(np.float64), expected_q)
+    assert np.abs(prolonged["f16x2"] - expected_q).max() <= 2.0 ** -10 * np.abs(expected_q).max()
+
+
 def divergent_pair(device, oracle64):
     from oracle.sim import compute_divergence as oracle_divergence
     from plume_amd.sim.mixed import divergent_velocity
@@ -166,7 +275,7 @@ def test_mixed_precision_solves_match_the_fp64_solution(device, oracle64):
     target = max(TOLERANCE, 2.0 * floor)
     velocity0 = [f.data.clone() for f in gpu.velocity_fields()]
     errors, divergences = {}, {}
-    for storage in ("f32-plain", "f32", "bf16", "f16"):
+    for storage in ("f32-plain", "f32", "bf16", "f16", "f16x2"):
         gpu.pressure.zero_()
         if storage == "f32-plain":
             solve = Multigrid(gpu, 2, 2, tolerance=target, max_cycles=60)
