@@ -219,9 +219,11 @@ int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *
    (NULL skips an array) and returns how many shapes there are. */
 int32_t plm_sweep_tile_shapes(int32_t *ti, int32_t *tj, int32_t *threads, int32_t capacity);
 /* The project's autotuner for the sweep tile (no reference counterpart; the sweep is bit-identical under
-   every shape): times each shape on this level (2 * repeats sweeps ping-ponging q <-> scratch, which are
-   overwritten), writes the mean microseconds per sweep into times_us (may be NULL), pins the fastest shape
-   for this level size for the rest of the process and returns it in *chosen. Synchronises the stream. */
+   every shape): times each shape on this level (2 * repeats sweeps per shape, min(2 * repeats, 64), the
+   shapes taking turns, ping-ponging q <-> scratch, which are overwritten; each sweep timed by its own
+   dispatch's begin and end), writes each shape's median microseconds per sweep into times_us (may be NULL),
+   pins the fastest shape for this level size for the rest of the process and returns it in *chosen.
+   Synchronises the stream. */
 int plm_autotune_sweep(const plm_grid_info *info, float *q, const float *rhs, float *scratch, int32_t repeats,
                        float *times_us, int32_t *chosen, void *stream);
 /* Pins a tile shape for a level size (-1 restores the default). */

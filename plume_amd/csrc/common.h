@@ -115,6 +115,66 @@ __device__ __forceinline__ bool thread_face(const plm_grid_info &g, int &axis, D
     return k < d.n2 && j < d.n1;
 }
 
+// t / d for any 32-bit t: the multiply-high by m = ceil(2^32 / d) is floor(t / d) or one more (its error
+// t (m d - 2^32) / (d 2^32) is below 1), corrected once
+struct ExtentDiv {
+    uint32_t d, m;
+    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
+        if (d == 1) return t;
+        const uint32_t q = __umulhi(t, m);
+        return q * d > t ? q - 1 : q;
+    }
+};
+
+inline ExtentDiv extent_div(int d) {
+    return ExtentDiv{(uint32_t)d, d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d)};
+}
+
+// The three face arrays of a grid in one 1D launch, each enumerated in its own C order (one thread per face,
+// 256 per block, a block never spanning two arrays): every wavefront holds 64 faces, where the (k, j) blocks
+// of grid_faces leave most lanes of a 129-face row's third wavefront idle
+constexpr int FACE_BLOCK = 256;
+struct FaceLaunch {
+    uint32_t block_end[3];  // cumulative blocks: u in [0, block_end[0]), v, then w
+    uint32_t count[3];
+    ExtentDiv div_n2[3], div_n1[3];
+};
+
+inline FaceLaunch face_launch(const plm_grid_info &g) {
+    FaceLaunch f{};
+    uint32_t blocks = 0;
+    for (int a = 0; a < 3; a++) {
+        const int n0 = g.dims[0] + (a == 0), n1 = g.dims[1] + (a == 1), n2 = g.dims[2] + (a == 2);
+        f.count[a] = (uint32_t)n0 * (uint32_t)n1 * (uint32_t)n2;
+        blocks += (f.count[a] + FACE_BLOCK - 1) / FACE_BLOCK;
+        f.block_end[a] = blocks;
+        f.div_n2[a] = extent_div(n2);
+        f.div_n1[a] = extent_div(n1);
+    }
+    return f;
+}
+
+// The block's face array (uniform) and the thread's face in it; false past the array's end. The per-array
+// values are picked with constant indices (a runtime index into the kernel arguments would copy them to
+// scratch).
+__device__ __forceinline__ bool packed_face(const plm_grid_info &g, const FaceLaunch &f, int &axis, int &i, int &j,
+                                            int &k) {
+    const uint32_t b = blockIdx.x;
+    axis = b < f.block_end[0] ? 0 : b < f.block_end[1] ? 1 : 2;
+    const uint32_t first = axis == 0 ? 0u : axis == 1 ? f.block_end[0] : f.block_end[1];
+    const uint32_t count = axis == 0 ? f.count[0] : axis == 1 ? f.count[1] : f.count[2];
+    const ExtentDiv dn2 = axis == 0 ? f.div_n2[0] : axis == 1 ? f.div_n2[1] : f.div_n2[2];
+    const ExtentDiv dn1 = axis == 0 ? f.div_n1[0] : axis == 1 ? f.div_n1[1] : f.div_n1[2];
+    const uint32_t t = (b - first) * FACE_BLOCK + threadIdx.x;
+    if (t >= count) return false;
+    const uint32_t r = dn2(t), c = dn1(r);
+    k = (int)(t - r * dn2.d);
+    i = (int)c;
+    j = (int)(r - c * dn1.d);
+    (void)g;
+    return true;
+}
+
 template <typename T> struct Tri {
     T *p[3];
 };

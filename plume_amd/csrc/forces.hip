@@ -25,11 +25,11 @@ __global__ __launch_bounds__(256) void k_buoyancy(plm_grid_info g, float *__rest
 }
 
 // drag_kernel, sim/forces.py:34-37, on all three face arrays
-__global__ __launch_bounds__(256) void k_drag(plm_grid_info g, Tri<float> f, float wx, float wy, float wz,
+__global__ __launch_bounds__(256) void k_drag(plm_grid_info g, FaceLaunch fl, Tri<float> f, float wx, float wy, float wz,
                                               float keep) {
     int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
+    const Dims d = face_dims(g, axis);
     const float wind = axis == 0 ? wx : (axis == 1 ? wy : wz);
     float *p = f.p[axis] + d.at(i, j, k);
     *p = wind + (*p - wind) * keep;
@@ -138,21 +138,32 @@ __global__ __launch_bounds__(256) void k_confinement_push(plm_grid_info g, const
     push[3 * c + 2] = scale * (nx * wy - ny * wx);
 }
 
-// add_face_push_kernel, sim/vorticity.py:74-84, on all three face arrays
+// add_face_push_kernel, sim/vorticity.py:74-84, one thread per cell: the thread adds to the three faces on its
+// cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, to that axis's face on the
+// high boundary. Each face's value is the face kernel's, 0.5 (push[below] + push[above]) dt with the clamped
+// neighbours; the cell's own push vector is read once, contiguously, and the lower neighbours' from L2.
 __global__ __launch_bounds__(256) void k_add_face_push(plm_grid_info g, Tri<float> f,
                                                        const float *__restrict__ push, float dt) {
-    int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
     const Dims nc = cell_dims(g);
-    int below[3] = {i, j, k}, above[3] = {i, j, k};
-    const int face[3] = {i, j, k};
-    below[axis] = max(face[axis] - 1, 0);
-    above[axis] = min(face[axis], g.dims[axis] - 1);
-    const float average =
-        0.5f * (push[3 * nc.at(below[0], below[1], below[2]) + axis] + push[3 * nc.at(above[0], above[1], above[2]) + axis]);
-    float *p = f.p[axis] + d.at(i, j, k);
-    *p = *p + average * dt;
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    const int cell[3] = {i, j, k};
+    const int stride[3] = {nc.n1 * nc.n2, nc.n2, 1};
+    const int c = nc.at(i, j, k);
+    const float own[3] = {push[3 * c], push[3 * c + 1], push[3 * c + 2]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const Dims d = face_dims(g, a);
+        const float below = cell[a] > 0 ? push[3 * (c - stride[a]) + a] : own[a];
+        float *p = f.p[a] + d.at(i, j, k);
+        *p = *p + 0.5f * (below + own[a]) * dt;
+        if (cell[a] == g.dims[a] - 1) {
+            int hi[3] = {i, j, k};
+            hi[a] += 1;
+            float *q = f.p[a] + d.at(hi[0], hi[1], hi[2]);
+            *q = *q + 0.5f * (own[a] + own[a]) * dt;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ noise, sim/noise.py
@@ -280,12 +291,12 @@ __global__ __launch_bounds__(256) void k_emit_scalars(plm_grid_info g, float *__
 }
 
 // emit_velocity_kernel, sim/emitters.py:105-121, on all three face arrays
-__global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, Tri<float> f,
+__global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, FaceLaunch fl, Tri<float> f,
                                                        const plm_emitter *__restrict__ emitters, int n_emitters,
                                                        int32_t seed, float time_s, float dt) {
     int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
+    const Dims d = face_dims(g, axis);
     const float x = g.origin_m[0] + ((float)i + (axis == 0 ? 0.0f : 0.5f)) * g.cell_m;
     const float y = g.origin_m[1] + ((float)j + (axis == 1 ? 0.0f : 0.5f)) * g.cell_m;
     const float z = g.origin_m[2] + ((float)k + (axis == 2 ? 0.0f : 0.5f)) * g.cell_m;
@@ -423,11 +434,11 @@ __device__ __forceinline__ void turbulence_paired(const plm_grid_info &g, float 
 // cell is well under half a lattice unit, cell_m / scale_m < 0.99) evaluates each potential's two points with
 // one shared lattice block; otherwise every point's noise is evaluated on its own. Both are bit-identical.
 template <bool PAIRED>
-__global__ __launch_bounds__(256) void k_turbulence(plm_grid_info g, Tri<float> f, int32_t seed, float scale_m,
+__global__ __launch_bounds__(256) void k_turbulence(plm_grid_info g, FaceLaunch fl, Tri<float> f, int32_t seed, float scale_m,
                                                     float strength, float pattern_time, float dt) {
     int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
+    const Dims d = face_dims(g, axis);
     const float to_lattice = g.cell_m / scale_m;
     if constexpr (PAIRED) {
         if (axis == 0) turbulence_paired<0>(g, f.p[0], d, i, j, k, seed, to_lattice, strength, pattern_time, dt);
@@ -494,7 +505,8 @@ int plm_drag(const plm_grid_info *info, float *u, float *v, float *w, const floa
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && wind, "plm_drag: NULL argument");
     hipStream_t s = as_stream(stream);
-    k_drag<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, wind[0], wind[1], wind[2], keep);
+    const FaceLaunch fl = face_launch(*info);
+    k_drag<<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, Tri<float>{{u, v, w}}, wind[0], wind[1], wind[2], keep);
     return check_launch("plm_drag");
 }
 
@@ -526,7 +538,7 @@ int plm_add_face_push(const plm_grid_info *info, float *u, float *v, float *w, c
                       void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && push, "plm_add_face_push: NULL array");
-    k_add_face_push<<<grid_faces(*info), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    k_add_face_push<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, push, dt);
     return check_launch("plm_add_face_push");
 }
 
@@ -549,7 +561,7 @@ int plm_confinement(const plm_grid_info *info, float *u, float *v, float *w, flo
     PLM_TRY(check_launch("plm_confinement curl"));
     k_confinement_push<<<grid3(nc), block3(), 0, s>>>(*info, curl, push, strength);
     PLM_TRY(check_launch("plm_confinement push"));
-    k_add_face_push<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    k_add_face_push<<<grid3(nc), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, push, dt);
     return check_launch("plm_confinement faces");
 }
 
@@ -564,7 +576,8 @@ int plm_emit(const plm_grid_info *info, float *smoke, float *temperature, float 
     k_emit_scalars<<<grid3(cell_dims(*info)), block3(), 0, s>>>(*info, smoke, temperature, emitters, n_emitters,
                                                                seed, time_s, dt);
     PLM_TRY(check_launch("plm_emit scalars"));
-    k_emit_velocity<<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, emitters, n_emitters, seed,
+    const FaceLaunch fl = face_launch(*info);
+    k_emit_velocity<<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, Tri<float>{{u, v, w}}, emitters, n_emitters, seed,
                                                           time_s, dt);
     return check_launch("plm_emit velocity");
 }
@@ -585,11 +598,12 @@ int plm_turbulence(const plm_grid_info *info, float *u, float *v, float *w, int3
     if (strength == 0.0f) return 0;
     hipStream_t s = as_stream(stream);
     TimingScope t("turbulence", s);
+    const FaceLaunch fl = face_launch(*info);
     if (info->cell_m / scale_m < 0.99f)
-        k_turbulence<true><<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
+        k_turbulence<true><<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, Tri<float>{{u, v, w}}, seed, scale_m, strength,
                                                        pattern_time, dt);
     else
-        k_turbulence<false><<<grid_faces(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, seed, scale_m, strength,
+        k_turbulence<false><<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, Tri<float>{{u, v, w}}, seed, scale_m, strength,
                                                        pattern_time, dt);
     return check_launch("plm_turbulence");
 }

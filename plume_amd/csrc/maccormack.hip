@@ -3,9 +3,11 @@
 //
 //  1. The trace's first velocity sample sits on the grid: a field sample at index + offset reads each
 //     velocity component at index + (offset - component offset), which is an exact integer on every axis
-//     where the two offsets agree. There the trilinear weight is exactly 0 and lerp(a, b, 0) = a for
-//     finite a != -0, so those taps are skipped (a -0.0 operand still loads its partner, because
-//     -0 + b*0 takes b's sign). A cell centre reads 2 taps per component instead of 8.
+//     where the two offsets agree. There the trilinear weight is exactly 0 and lerp(a, b, 0) = a + b*0,
+//     which is a for finite a and b except that a = -0 gives +0 when b >= +0. That sample is used only in
+//     the midpoint p - (dt/2) v, and p - (+-0) = p for every p the trace starts from (p >= +0, and
+//     +0 - (+-0) = +0), so those taps are skipped outright: the midpoint, and everything after it, is the
+//     same bits. A cell centre reads 2 taps per component instead of 8, all issued together.
 //  2. The forward pass samples `data` at the traced source, which is exactly where the correction takes
 //     its clamp range, so it keeps the min and max of those 8 taps (`bounds`, 2 floats per sample) and the
 //     correction skips its own forward trace.
@@ -27,14 +29,9 @@ __host__ __device__ constexpr int aligned_mask(int kind, int component) {
            (kind_offset(kind, 2) == (component == 2 ? 0.0f : 0.5f) ? 4 : 0);
 }
 
-// lerp_(a, b, 0) without loading b unless a is -0.0 (then a + b*0 carries b's sign).
-template <typename B> __device__ __forceinline__ float lerp_zero(float a, B b) {
-    if (__float_as_uint(a) == 0x80000000u) return lerp_(a, b(), 0.0f);
-    return a;
-}
-
 // sample_index_space (sim/sample.py:10-29) where the axes in MASK are known to sit exactly on an index in
-// range, so their weight is 0.
+// range, so their weight is 0 and their second taps are not read. The result may differ from the full sample
+// only in the sign of a zero (see 1. above), so it serves the trace's midpoint alone.
 template <int MASK>
 __device__ __forceinline__ float sample_aligned(const float *__restrict__ f, Dims n, float gx, float gy, float gz) {
     const float hx = (float)(n.n0 - 1), hy = (float)(n.n1 - 1), hz = (float)(n.n2 - 1);
@@ -47,18 +44,15 @@ __device__ __forceinline__ float sample_aligned(const float *__restrict__ f, Dim
     const uint32_t dk = z < hz ? 4u : 0u;
     const float tx = index_weight(x), ty = index_weight(y), tz = index_weight(z);
     auto xl = [&](uint32_t base) -> float {
-        const float a = load_bytes(f, base);
-        if (MASK & 1) return lerp_zero(a, [&] { return load_bytes(f, base + di); });
-        return lerp_(a, load_bytes(f, base + di), tx);
+        if (MASK & 1) return load_bytes(f, base);
+        return lerp_(load_bytes(f, base), load_bytes(f, base + di), tx);
     };
     auto yl = [&](uint32_t base) -> float {
-        const float a = xl(base);
-        if (MASK & 2) return lerp_zero(a, [&] { return xl(base + dj); });
-        return lerp_(a, xl(base + dj), ty);
+        if (MASK & 2) return xl(base);
+        return lerp_(xl(base), xl(base + dj), ty);
     };
-    const float a = yl(o);
-    if (MASK & 4) return lerp_zero(a, [&] { return yl(o + dk); });
-    return lerp_(a, yl(o + dk), tz);
+    if (MASK & 4) return yl(o);
+    return lerp_(yl(o), yl(o + dk), tz);
 }
 
 // velocity_at (sim/sample.py:74-82) at the kind-K sample point of index (i, j, k)
@@ -128,18 +122,20 @@ __device__ __forceinline__ void forward_sample(const plm_grid_info &g, const Fie
     trace_from_grid<KIND>(g, u, v, w, px, py, pz, dto, sx, sy, sz);
     const int c = n.at(i, j, k), count = n.count();
     const bool outside = outside_open_face(g, sx, sy, sz);
+    // every field's taps are read before anything is stored (the clamped taps lie in range wherever the
+    // source is), so their loads are in flight together; a source beyond an open face then takes the
+    // field's outside value, as the branch of sample_field does
+    float value[NF], lo[NF], hi[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        value[f] = sample_and_range(fields[f].data, n, sx - ox, sy - oy, sz - oz, lo[f], hi[f]);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         const Field &fd = fields[f];
-        float value, lo, hi;
-        if (fd.uses_outside == 1 && outside) {
-            value = lo = hi = fd.outside;
-        } else {
-            value = sample_and_range(fd.data, n, sx - ox, sy - oy, sz - oz, lo, hi);
-        }
-        fd.hat[c] = value;
-        fd.bounds[c] = lo;
-        fd.bounds[count + c] = hi;
+        if (fd.uses_outside == 1 && outside) value[f] = lo[f] = hi[f] = fd.outside;
+        fd.hat[c] = value[f];
+        fd.bounds[c] = lo[f];
+        fd.bounds[count + c] = hi[f];
     }
 }
 
@@ -155,79 +151,29 @@ __device__ __forceinline__ void correct_sample(const plm_grid_info &g, const Fie
     trace_from_grid<KIND>(g, u, v, w, px, py, pz, -dto, sx, sy, sz);
     const int c = n.at(i, j, k), count = n.count();
     const bool outside = outside_open_face(g, sx, sy, sz);
+    // every load before the first store, as in forward_sample
+    float back[NF], hat[NF], data[NF], lo[NF], hi[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         const Field &fd = fields[f];
-        const float back =
-            (fd.uses_outside == 1 && outside) ? fd.outside : sample_index(fd.hat, n, sx - ox, sy - oy, sz - oz);
-        const float corrected = fd.hat[c] + 0.5f * (fd.data[c] - back);
-        fd.next[c] = clamp_(corrected, fd.bounds[c], fd.bounds[count + c]);
+        back[f] = sample_index(fd.hat, n, sx - ox, sy - oy, sz - oz);
+        hat[f] = fd.hat[c];
+        data[f] = fd.data[c];
+        lo[f] = fd.bounds[c];
+        hi[f] = fd.bounds[count + c];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const Field &fd = fields[f];
+        if (fd.uses_outside == 1 && outside) back[f] = fd.outside;
+        const float corrected = hat[f] + 0.5f * (data[f] - back[f]);
+        fd.next[c] = clamp_(corrected, lo[f], hi[f]);
     }
 }
 
 struct Faces3 {
     Field f[3];
 };
-
-// t / d for any 32-bit t: the multiply-high by m = ceil(2^32 / d) is floor(t / d) or one more (its error
-// t (m d - 2^32) / (d 2^32) is below 1), corrected once
-struct ExtentDiv {
-    uint32_t d, m;
-    __device__ __forceinline__ uint32_t operator()(uint32_t t) const {
-        if (d == 1) return t;
-        const uint32_t q = __umulhi(t, m);
-        return q * d > t ? q - 1 : q;
-    }
-};
-
-ExtentDiv extent_div(int d) {
-    return ExtentDiv{(uint32_t)d, d == 1 ? 0u : (uint32_t)(((uint64_t(1) << 32) + (uint64_t)d - 1) / (uint64_t)d)};
-}
-
-// The three face arrays of a grid in one 1D launch, each enumerated in its own C order (one thread per face,
-// 256 per block, a block never spanning two arrays): every wavefront holds 64 faces, where the (k, j) blocks
-// of grid_faces leave most lanes of a 129-face row's third wavefront idle
-constexpr int FACE_BLOCK = 256;
-struct FaceLaunch {
-    uint32_t block_end[3];  // cumulative blocks: u in [0, block_end[0]), v, then w
-    uint32_t count[3];
-    ExtentDiv div_n2[3], div_n1[3];
-};
-
-FaceLaunch face_launch(const plm_grid_info &g) {
-    FaceLaunch f{};
-    uint32_t blocks = 0;
-    for (int a = 0; a < 3; a++) {
-        const int n0 = g.dims[0] + (a == 0), n1 = g.dims[1] + (a == 1), n2 = g.dims[2] + (a == 2);
-        f.count[a] = (uint32_t)n0 * (uint32_t)n1 * (uint32_t)n2;
-        blocks += (f.count[a] + FACE_BLOCK - 1) / FACE_BLOCK;
-        f.block_end[a] = blocks;
-        f.div_n2[a] = extent_div(n2);
-        f.div_n1[a] = extent_div(n1);
-    }
-    return f;
-}
-
-// The block's face array (uniform) and the thread's face in it; false past the array's end. The per-array
-// values are picked with constant indices (a runtime index into the kernel arguments would copy them to
-// scratch).
-__device__ __forceinline__ bool packed_face(const plm_grid_info &g, const FaceLaunch &f, int &axis, int &i, int &j,
-                                            int &k) {
-    const uint32_t b = blockIdx.x;
-    axis = b < f.block_end[0] ? 0 : b < f.block_end[1] ? 1 : 2;
-    const uint32_t first = axis == 0 ? 0u : axis == 1 ? f.block_end[0] : f.block_end[1];
-    const uint32_t count = axis == 0 ? f.count[0] : axis == 1 ? f.count[1] : f.count[2];
-    const ExtentDiv dn2 = axis == 0 ? f.div_n2[0] : axis == 1 ? f.div_n2[1] : f.div_n2[2];
-    const ExtentDiv dn1 = axis == 0 ? f.div_n1[0] : axis == 1 ? f.div_n1[1] : f.div_n1[2];
-    const uint32_t t = (b - first) * FACE_BLOCK + threadIdx.x;
-    if (t >= count) return false;
-    const uint32_t r = dn2(t), c = dn1(r);
-    k = (int)(t - r * dn2.d);
-    i = (int)c;
-    j = (int)(r - c * dn1.d);
-    (void)g;
-    return true;
-}
 
 __global__ __launch_bounds__(FACE_BLOCK) void k_velocity_forward(plm_grid_info g, FaceLaunch fl, Faces3 faces,
                                                                  float dto) {

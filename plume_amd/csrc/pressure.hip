@@ -1008,30 +1008,35 @@ int tile_shape_for(const Dims &n) {
 
 // ------------------------------------------------------------------ gradient subtraction
 
-// subtract_gradient_kernel, sim/pressure.py:95-116, on all three face arrays
+// subtract_gradient_kernel, sim/pressure.py:95-116, one thread per cell: the thread updates the three faces on
+// its cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, that axis's face on the
+// high boundary, so q is read once, coalesced (the lower neighbours from L1/L2). Each face's update is the face
+// kernel's: f -= (q_above - q_below) / cell_m, a ghost q of 0 outside an open face, closed boundary faces
+// untouched.
 __global__ __launch_bounds__(256) void k_subtract_gradient(plm_grid_info g, Tri<float> f,
                                                            const float *__restrict__ q) {
-    int axis, i, j, k;
-    Dims d;
-    if (!thread_face(g, axis, d, i, j, k)) return;
     const Dims n = cell_dims(g);
-    const int face[3] = {i, j, k};
-    const int index = face[axis];
-    int below[3] = {i, j, k};
-    below[axis] = index - 1;
-    float q_above = 0.0f, q_below = 0.0f;
-    if (index == 0) {
-        if (g.open_lo[axis] == 0) return;
-        q_above = q[n.at(i, j, k)];
-    } else if (index == g.dims[axis]) {
-        if (g.open_hi[axis] == 0) return;
-        q_below = q[n.at(below[0], below[1], below[2])];
-    } else {
-        q_above = q[n.at(i, j, k)];
-        q_below = q[n.at(below[0], below[1], below[2])];
+    int i, j, k;
+    if (!thread_cell(n, i, j, k)) return;
+    const int cell[3] = {i, j, k};
+    const int stride[3] = {n.n1 * n.n2, n.n2, 1};
+    const int c = n.at(i, j, k);
+    const float q_cell = q[c];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const Dims d = face_dims(g, a);
+        if (cell[a] > 0 || g.open_lo[a] == 1) {
+            const float q_below = cell[a] > 0 ? q[c - stride[a]] : 0.0f;
+            float *p = f.p[a] + d.at(i, j, k);
+            *p = *p - (q_cell - q_below) / g.cell_m;
+        }
+        if (cell[a] == g.dims[a] - 1 && g.open_hi[a] == 1) {
+            int hi[3] = {i, j, k};
+            hi[a] += 1;
+            float *p = f.p[a] + d.at(hi[0], hi[1], hi[2]);
+            *p = *p - (0.0f - q_cell) / g.cell_m;
+        }
     }
-    float *p = f.p[axis] + d.at(i, j, k);
-    *p = *p - (q_above - q_below) / g.cell_m;
 }
 
 // ------------------------------------------------------------------ multigrid transfers
@@ -1975,6 +1980,61 @@ int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch
         PLM_TRY(check_launch("copy back"));
     }
     return 0;
+}
+
+// Times every tile shape of the plain two-colour sweep on one level and pins the fastest. Each launch carries
+// its own begin and end timestamps (events bound to the dispatch through hipExtLaunchKernelGGL), so a shape is
+// judged by its kernel time, not by the gaps between back-to-back launches; the shapes take turns round by
+// round (drift in clocks or caches touches all of them alike), and each one's figure is the median of its
+// 2 x repeats launches after one untimed round.
+template <bool UNIT>
+int autotune_sweep(const LevelArgs &L, float *q, const float *rhs, float *scratch, int repeats, float *times_us,
+                   int32_t *chosen, hipStream_t s) {
+    const Dims n = cell_dims(L.g), none{1, 1, 1};
+    const int launches = std::min(2 * repeats, 64);
+    std::vector<hipEvent_t> events((size_t)2 * N_TILE_SHAPES * launches, nullptr);
+    int rc = 0;
+    for (hipEvent_t &e : events)
+        if (rc == 0 && hipEventCreate(&e) != hipSuccess) rc = fail(PLM_EINVAL, "plm_autotune_sweep: event creation failed");
+    for (int r = 0; r <= launches && rc == 0; r++) {
+        for (int v = 0; v < N_TILE_SHAPES && rc == 0; v++) {
+            const TileShape &ts = TILE_SHAPES[v];
+            const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + ts.tj - 1) / ts.tj;
+            const int tiles_i = (n.n0 + ts.ti - 1) / ts.ti;
+            const bool even = (r * N_TILE_SHAPES + v) % 2 == 0;
+            const size_t slot = 2 * ((size_t)v * launches + (size_t)(r - 1));
+            const hipEvent_t e0 = r > 0 ? events[slot] : nullptr, e1 = r > 0 ? events[slot + 1] : nullptr;
+            launch_tile_shape<UNIT, FROM_Q, float>(v, dim3((unsigned)(tiles_k * tiles_j * tiles_i)), s, e0, e1, L,
+                                                   even ? q : scratch, even ? scratch : q, rhs, nullptr, none, 1.0f,
+                                                   nullptr, fast_div(tiles_k), fast_div(tiles_j));
+            rc = check_launch("plm_autotune_sweep");
+        }
+    }
+    if (rc == 0 && hipStreamSynchronize(s) != hipSuccess) rc = fail(PLM_EINVAL, "plm_autotune_sweep: synchronise failed");
+    int best = DEFAULT_TILE_SHAPE;
+    float best_us = 0.0f;
+    for (int v = 0; v < N_TILE_SHAPES && rc == 0; v++) {
+        std::vector<float> us((size_t)launches);
+        for (int r = 0; r < launches && rc == 0; r++) {
+            const size_t slot = 2 * ((size_t)v * launches + (size_t)r);
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, events[slot], events[slot + 1]) != hipSuccess)
+                rc = fail(PLM_EINVAL, "plm_autotune_sweep: event timing failed");
+            us[(size_t)r] = ms * 1000.0f;
+        }
+        std::nth_element(us.begin(), us.begin() + launches / 2, us.end());
+        const float median = us[(size_t)launches / 2];
+        if (times_us) times_us[v] = median;
+        if (v == 0 || median < best_us) {
+            best = v;
+            best_us = median;
+        }
+    }
+    for (hipEvent_t e : events)
+        if (e != nullptr) (void)hipEventDestroy(e);
+    PLM_TRY(rc);
+    *chosen = best;
+    return plm_set_sweep_tile(L.g.dims, best);
 }
 
 struct Hierarchy {
@@ -3517,7 +3577,7 @@ int plm_rb_sweeps(const plm_grid_info *info, float *q, const float *rhs, float *
 int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *w, const float *q, void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && q, "plm_subtract_gradient: NULL array");
-    k_subtract_gradient<<<grid_faces(*info), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, q);
+    k_subtract_gradient<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, q);
     return check_launch("plm_subtract_gradient");
 }
 
@@ -3590,41 +3650,8 @@ int plm_autotune_sweep(const plm_grid_info *info, float *q, const float *rhs, fl
     PLM_REQUIRE(repeats >= 1, "plm_autotune_sweep: repeats must be >= 1");
     hipStream_t s = as_stream(stream);
     const LevelArgs L = level_args(*info);
-    const Dims n = cell_dims(*info), none{1, 1, 1};
-    hipEvent_t start, stop;
-    PLM_REQUIRE(hipEventCreate(&start) == hipSuccess && hipEventCreate(&stop) == hipSuccess,
-                "plm_autotune_sweep: event creation failed");
-    int best = DEFAULT_TILE_SHAPE;
-    float best_ms = 0.0f;
-    int rc = 0;
-    for (int v = 0; v < N_TILE_SHAPES && rc == 0; v++) {
-        auto sweep = [&](int r) {
-            const bool even = r % 2 == 0;
-            return L.unit ? launch_tile<true, float>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
-                                              1.0f, nullptr, s, v)
-                          : launch_tile<false, float>(L, FROM_Q, even ? q : scratch, even ? scratch : q, rhs, nullptr, none,
-                                               1.0f, nullptr, s, v);
-        };
-        rc = sweep(0);  // warm-up
-        if (rc == 0 && hipEventRecord(start, s) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: event record failed");
-        for (int r = 0; r < 2 * repeats && rc == 0; r++) rc = sweep(r + 1);
-        if (rc == 0 && hipEventRecord(stop, s) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: event record failed");
-        if (rc == 0 && hipEventSynchronize(stop) != hipSuccess) rc = fail(PLM_EINVAL, "autotune: synchronise failed");
-        float ms = 0.0f;
-        if (rc == 0) (void)hipEventElapsedTime(&ms, start, stop);
-        ms /= (float)(2 * repeats);
-        if (times_us) times_us[v] = ms * 1000.0f;
-        if (rc == 0 && (v == 0 || ms < best_ms)) {
-            best = v;
-            best_ms = ms;
-        }
-    }
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
-    PLM_TRY(rc);
-    *chosen = best;
-    (void)n;
-    return plm_set_sweep_tile(info->dims, best);
+    return L.unit ? autotune_sweep<true>(L, q, rhs, scratch, repeats, times_us, chosen, s)
+                  : autotune_sweep<false>(L, q, rhs, scratch, repeats, times_us, chosen, s);
 }
 
 int64_t plm_norm_scratch_doubles(const plm_grid_info *info) {

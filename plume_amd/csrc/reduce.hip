@@ -80,8 +80,19 @@ __global__ __launch_bounds__(256) void k_scale_by_total(int64_t count, float *__
     const double factor = totals[3];
     if (factor == 1.0) return;
     const float scale = (float)factor;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x)
-        f[c] = f[c] * scale;
+    // four values per thread (one 16-byte access) over the aligned bulk, then the tail one by one
+    const int64_t quads = (reinterpret_cast<uintptr_t>(f) & 15u) == 0 ? count / 4 : 0;
+    const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+    float4 *f4 = reinterpret_cast<float4 *>(f);
+    for (int64_t c = first; c < quads; c += step) {
+        float4 x = f4[c];
+        x.x = x.x * scale;
+        x.y = x.y * scale;
+        x.z = x.z * scale;
+        x.w = x.w * scale;
+        f4[c] = x;
+    }
+    for (int64_t c = 4 * quads + first; c < count; c += step) f[c] = f[c] * scale;
 }
 
 __global__ __launch_bounds__(256) void k_scale(int64_t count, float *__restrict__ f, float factor) {
@@ -184,7 +195,7 @@ int plm_smoke_budget(const int32_t shape[3], const float *data, const float *hat
     PLM_TRY(check_launch("plm_smoke_budget line sums"));
     k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, totals, nullptr);
     PLM_TRY(check_launch("plm_smoke_budget totals"));
-    k_scale_by_total<<<grid1(n.count()), 256, 0, s>>>(n.count(), next, totals);
+    k_scale_by_total<<<grid1((n.count() + 3) / 4), 256, 0, s>>>(n.count(), next, totals);
     return check_launch("plm_smoke_budget scale");
 }
 
@@ -207,7 +218,7 @@ int plm_smoke_budget_noalloc(const int32_t shape[3], const float *data, const fl
     PLM_TRY(check_launch("plm_smoke_budget_noalloc line sums"));
     k_budget_totals<<<1, BUDGET_THREADS, 0, s>>>(n.n1 * n.n2, lines, held, totals);
     PLM_TRY(check_launch("plm_smoke_budget_noalloc totals"));
-    k_scale_by_total<<<grid1(n.count()), 256, 0, s>>>(n.count(), next, held);
+    k_scale_by_total<<<grid1((n.count() + 3) / 4), 256, 0, s>>>(n.count(), next, held);
     return check_launch("plm_smoke_budget_noalloc scale");
 }
 
