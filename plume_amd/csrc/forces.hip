@@ -72,29 +72,30 @@ __global__ __launch_bounds__(256) void k_curl(plm_grid_info g, const float *__re
     if (!thread_cell(nc, i, j, k)) return;
     const int cell[3] = {i, j, k};
     float d[3][3];  // d[axis][component]
+    // every axis's neighbours are read before any arithmetic (their clamped indices lie in the grid), so the
+    // loads are in flight together; an axis of one cell (span 0) then takes the reference's 0
+    float h[3][3], l[3][3], span[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         int lo[3] = {i, j, k}, hi[3] = {i, j, k};
         lo[a] = max(cell[a] - 1, 0);
         hi[a] = min(cell[a] + 1, g.dims[a] - 1);
-        const float span = (float)(hi[a] - lo[a]);
-        if (span == 0.0f) {
-            d[a][0] = d[a][1] = d[a][2] = 0.0f;
-            continue;
-        }
-        float h[3], l[3];
+        span[a] = (float)(hi[a] - lo[a]);
         if (CENTER_FROM_FACES) {
-            center_velocity(g, u, v, w, hi[0], hi[1], hi[2], h[0], h[1], h[2]);
-            center_velocity(g, u, v, w, lo[0], lo[1], lo[2], l[0], l[1], l[2]);
+            center_velocity(g, u, v, w, hi[0], hi[1], hi[2], h[a][0], h[a][1], h[a][2]);
+            center_velocity(g, u, v, w, lo[0], lo[1], lo[2], l[a][0], l[a][1], l[a][2]);
         } else {
             const int ch = nc.at(hi[0], hi[1], hi[2]), cl = nc.at(lo[0], lo[1], lo[2]);
             for (int q = 0; q < 3; q++) {
-                h[q] = center[3 * ch + q];
-                l[q] = center[3 * cl + q];
+                h[a][q] = center[3 * ch + q];
+                l[a][q] = center[3 * cl + q];
             }
         }
-        const float denom = span * g.cell_m;
-        for (int q = 0; q < 3; q++) d[a][q] = (h[q] - l[q]) / denom;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float denom = span[a] * g.cell_m;
+        for (int q = 0; q < 3; q++) d[a][q] = span[a] == 0.0f ? 0.0f : (h[a][q] - l[a][q]) / denom;
     }
     const int c = nc.at(i, j, k);
     curl[3 * c] = d[1][2] - d[2][1];
@@ -113,25 +114,30 @@ __global__ __launch_bounds__(256) void k_confinement_push(plm_grid_info g, const
     int i, j, k;
     if (!thread_cell(nc, i, j, k)) return;
     const int cell[3] = {i, j, k};
-    float toward[3] = {0.0f, 0.0f, 0.0f};
+    const int c = nc.at(i, j, k);
+    // the six neighbours' and the cell's own curl are read first (clamped indices, in the grid), so the loads
+    // are in flight together; an axis of one cell (span 0) keeps the reference's 0
+    const float wx = curl[3 * c], wy = curl[3 * c + 1], wz = curl[3 * c + 2];
+    float spin_hi[3], spin_lo[3], span[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         int lo[3] = {i, j, k}, hi[3] = {i, j, k};
         lo[a] = max(cell[a] - 1, 0);
         hi[a] = min(cell[a] + 1, g.dims[a] - 1);
-        const float span = (float)(hi[a] - lo[a]);
-        if (span > 0.0f)
-            toward[a] = (spin_at(curl, nc.at(hi[0], hi[1], hi[2])) - spin_at(curl, nc.at(lo[0], lo[1], lo[2]))) /
-                        (span * g.cell_m);
+        span[a] = (float)(hi[a] - lo[a]);
+        spin_hi[a] = spin_at(curl, nc.at(hi[0], hi[1], hi[2]));
+        spin_lo[a] = spin_at(curl, nc.at(lo[0], lo[1], lo[2]));
     }
-    const int c = nc.at(i, j, k);
+    float toward[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        toward[a] = span[a] > 0.0f ? (spin_hi[a] - spin_lo[a]) / (span[a] * g.cell_m) : 0.0f;
     const float length = length3(toward[0], toward[1], toward[2]);
     if (length < 1.0e-12f) {
         push[3 * c] = push[3 * c + 1] = push[3 * c + 2] = 0.0f;
         return;
     }
     const float nx = toward[0] / length, ny = toward[1] / length, nz = toward[2] / length;
-    const float wx = curl[3 * c], wy = curl[3 * c + 1], wz = curl[3 * c + 2];
     const float scale = strength * g.cell_m;
     push[3 * c] = scale * (ny * wz - nz * wy);
     push[3 * c + 1] = scale * (nz * wx - nx * wz);
@@ -141,7 +147,9 @@ __global__ __launch_bounds__(256) void k_confinement_push(plm_grid_info g, const
 // add_face_push_kernel, sim/vorticity.py:74-84, one thread per cell: the thread adds to the three faces on its
 // cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, to that axis's face on the
 // high boundary. Each face's value is the face kernel's, 0.5 (push[below] + push[above]) dt with the clamped
-// neighbours; the cell's own push vector is read once, contiguously, and the lower neighbours' from L2.
+// neighbours; the cell's own push vector is read once, contiguously, and the lower neighbours' from L2. Every
+// load is issued before the first store (loads at clamped addresses whose values a lane may not use), so they
+// are in flight together.
 __global__ __launch_bounds__(256) void k_add_face_push(plm_grid_info g, Tri<float> f,
                                                        const float *__restrict__ push, float dt) {
     const Dims nc = cell_dims(g);
@@ -150,19 +158,26 @@ __global__ __launch_bounds__(256) void k_add_face_push(plm_grid_info g, Tri<floa
     const int cell[3] = {i, j, k};
     const int stride[3] = {nc.n1 * nc.n2, nc.n2, 1};
     const int c = nc.at(i, j, k);
-    const float own[3] = {push[3 * c], push[3 * c + 1], push[3 * c + 2]};
+    float own[3], below[3], low_face[3], high_face[3];
+    float *low_at[3], *high_at[3];
+    bool high[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const Dims d = face_dims(g, a);
-        const float below = cell[a] > 0 ? push[3 * (c - stride[a]) + a] : own[a];
-        float *p = f.p[a] + d.at(i, j, k);
-        *p = *p + 0.5f * (below + own[a]) * dt;
-        if (cell[a] == g.dims[a] - 1) {
-            int hi[3] = {i, j, k};
-            hi[a] += 1;
-            float *q = f.p[a] + d.at(hi[0], hi[1], hi[2]);
-            *q = *q + 0.5f * (own[a] + own[a]) * dt;
-        }
+        int hi[3] = {i, j, k};
+        hi[a] += 1;
+        high[a] = cell[a] == g.dims[a] - 1;
+        own[a] = push[3 * c + a];
+        below[a] = push[3 * (cell[a] > 0 ? c - stride[a] : c) + a];
+        low_at[a] = f.p[a] + d.at(i, j, k);
+        high_at[a] = high[a] ? f.p[a] + d.at(hi[0], hi[1], hi[2]) : low_at[a];
+        low_face[a] = *low_at[a];
+        high_face[a] = *high_at[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        *low_at[a] = low_face[a] + 0.5f * (below[a] + own[a]) * dt;
+        if (high[a]) *high_at[a] = high_face[a] + 0.5f * (own[a] + own[a]) * dt;
     }
 }
 

@@ -1012,7 +1012,8 @@ int tile_shape_for(const Dims &n) {
 // its cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, that axis's face on the
 // high boundary, so q is read once, coalesced (the lower neighbours from L1/L2). Each face's update is the face
 // kernel's: f -= (q_above - q_below) / cell_m, a ghost q of 0 outside an open face, closed boundary faces
-// untouched.
+// untouched. Every load is issued before the first store (at clamped addresses whose values a lane may not
+// use), so they are in flight together.
 __global__ __launch_bounds__(256) void k_subtract_gradient(plm_grid_info g, Tri<float> f,
                                                            const float *__restrict__ q) {
     const Dims n = cell_dims(g);
@@ -1022,20 +1023,27 @@ __global__ __launch_bounds__(256) void k_subtract_gradient(plm_grid_info g, Tri<
     const int stride[3] = {n.n1 * n.n2, n.n2, 1};
     const int c = n.at(i, j, k);
     const float q_cell = q[c];
+    float q_lower[3], low_face[3], high_face[3];
+    float *low_at[3], *high_at[3];
+    bool low[3], high[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const Dims d = face_dims(g, a);
-        if (cell[a] > 0 || g.open_lo[a] == 1) {
-            const float q_below = cell[a] > 0 ? q[c - stride[a]] : 0.0f;
-            float *p = f.p[a] + d.at(i, j, k);
-            *p = *p - (q_cell - q_below) / g.cell_m;
-        }
-        if (cell[a] == g.dims[a] - 1 && g.open_hi[a] == 1) {
-            int hi[3] = {i, j, k};
-            hi[a] += 1;
-            float *p = f.p[a] + d.at(hi[0], hi[1], hi[2]);
-            *p = *p - (0.0f - q_cell) / g.cell_m;
-        }
+        int hi[3] = {i, j, k};
+        hi[a] += 1;
+        low[a] = cell[a] > 0 || g.open_lo[a] == 1;
+        high[a] = cell[a] == g.dims[a] - 1 && g.open_hi[a] == 1;
+        q_lower[a] = q[cell[a] > 0 ? c - stride[a] : c];
+        low_at[a] = f.p[a] + d.at(i, j, k);
+        high_at[a] = high[a] ? f.p[a] + d.at(hi[0], hi[1], hi[2]) : low_at[a];
+        low_face[a] = *low_at[a];
+        high_face[a] = *high_at[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float q_below = cell[a] > 0 ? q_lower[a] : 0.0f;
+        if (low[a]) *low_at[a] = low_face[a] - (q_cell - q_below) / g.cell_m;
+        if (high[a]) *high_at[a] = high_face[a] - (0.0f - q_cell) / g.cell_m;
     }
 }
 
