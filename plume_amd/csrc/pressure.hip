@@ -1401,7 +1401,9 @@ __device__ __forceinline__ float fill_at(const TailLevel &L, int a, int index) {
 
 // the padded index of cell (i, j, k) of a level of extents n
 __device__ __forceinline__ int pad_at(const int (&n)[3], int i, int j, int k) {
-    return ((i + 1) * (n[1] + 2) + j + 1) * (n[2] + 2) + k + 1;
+    // a tail level holds at most PLM_LDS_CELLS cells, so every product fits the full-rate 24-bit multiply
+    return (int)__umul24(__umul24((uint32_t)(i + 1), (uint32_t)(n[1] + 2)) + (uint32_t)(j + 1), (uint32_t)(n[2] + 2)) +
+           k + 1;
 }
 
 // What one tail pass needs of one cell, fixed for a visit of its level (neighbor_sum's couplings and their
@@ -1657,6 +1659,19 @@ __device__ __forceinline__ void mg_tail(const TailLevels &args, float *__restric
     constexpr int K = ONES ? BLOCK_PLAN_ONES : BLOCK_PLAN;
     constexpr int KEEP = ONES ? 2 : 1;
     extern __shared__ float lds[];
+    // The top level's rhs (and q) are read from global memory first, so their latency overlaps the staging of
+    // the level table and the clearing of LDS below (a tail holds at most PLM_LDS_CELLS cells).
+    constexpr int TOP_PER_THREAD = PLM_LDS_CELLS / TAIL_THREADS;
+    static_assert(PLM_LDS_CELLS % TAIL_THREADS == 0, "the top level's cells split evenly over the threads");
+    const int top_cells = args.level[0].g.dims[0] * args.level[0].g.dims[1] * args.level[0].g.dims[2];
+    const bool top_q = args.zero_top == 0;
+    float top_rhs[TOP_PER_THREAD], top_qv[TOP_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < TOP_PER_THREAD; u++) {
+        const int c = (int)threadIdx.x + u * TAIL_THREADS;
+        top_rhs[u] = c < top_cells ? rhs_top[c] : 0.0f;
+        top_qv[u] = c < top_cells && top_q ? q_top[c] : 0.0f;
+    }
     // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
     // be copied to per-thread scratch, so it is staged in LDS once.
     __shared__ TailLevels t;
@@ -1673,13 +1688,16 @@ __device__ __forceinline__ void mg_tail(const TailLevels &args, float *__restric
     const int tid = threadIdx.x;
     {
         const TailDims T = load_dims(t, 0);
-        const int cells = T.n[0] * T.n[1] * T.n[2];
-        for (int c = tid; c < cells; c += TAIL_THREADS) {
-            const int r = T.div_n2(c), k = c - r * T.n[2];
-            const int i = T.div_n1(r), j = r - i * T.n[1];
-            const int p = pad_at(T.n, i, j, k);
-            if (!t.zero_top) lds[t.qoff[0] + p] = q_top[c];
-            lds[t.roff[0] + p] = rhs_top[c];
+#pragma unroll
+        for (int u = 0; u < TOP_PER_THREAD; u++) {
+            const int c = tid + u * TAIL_THREADS;
+            if (c < top_cells) {
+                const int r = T.div_n2(c), k = c - r * T.n[2];
+                const int i = T.div_n1(r), j = r - i * T.n[1];
+                const int p = pad_at(T.n, i, j, k);
+                if (top_q) lds[t.qoff[0] + p] = top_qv[u];
+                lds[t.roff[0] + p] = top_rhs[u];
+            }
         }
     }
     __syncthreads();
@@ -1863,21 +1881,27 @@ __device__ __forceinline__ void norm_finish(double *scratch, int64_t blocks, dou
                                             int32_t *cycles_run) {
     if (mode == 2 && scratch[1] == 0.0) return;
     const double *partials = scratch + 4;
-    __shared__ double srr[FINISH_THREADS], sbb[FINISH_THREADS];
+    // fixed order: each thread's blocks in turn, a butterfly per wavefront, then the wavefronts in order (two
+    // block barriers where an LDS tree over every thread took ten)
+    constexpr int WAVES = FINISH_THREADS / 64;
+    __shared__ double srr[WAVES], sbb[WAVES];
     double rr = 0.0, bb = 0.0;
     for (int64_t b = threadIdx.x; b < blocks; b += FINISH_THREADS) {
         rr += partials[2 * b];
         bb += partials[2 * b + 1];
     }
-    srr[threadIdx.x] = rr;
-    sbb[threadIdx.x] = bb;
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) {
+        srr[threadIdx.x / 64] = rr;
+        sbb[threadIdx.x / 64] = bb;
+    }
     __syncthreads();
-    for (int half = FINISH_THREADS / 2; half > 0; half /= 2) {
-        if ((int)threadIdx.x < half) {
-            srr[threadIdx.x] += srr[threadIdx.x + half];
-            sbb[threadIdx.x] += sbb[threadIdx.x + half];
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < WAVES; w++) {
+            srr[0] += srr[w];
+            sbb[0] += sbb[w];
         }
-        __syncthreads();
     }
     if (threadIdx.x != 0) return;
     const double r2 = srr[0], b2 = sbb[0];
