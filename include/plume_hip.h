@@ -200,6 +200,10 @@ int plm_zero_closed_faces(const plm_grid_info *info, float *u, float *v, float *
 /* divergence_kernel, sim/pressure.py:31-36. */
 int plm_divergence(const plm_grid_info *info, const float *u, const float *v, const float *w, float *divergence,
                    void *stream);
+/* project's first two steps, sim/pressure.py:146-149 (zero_closed_faces_kernel, :21-28, then divergence_kernel,
+   :31-36, on the same faces) in one launch: the faces and the divergence end bit for bit as the two calls above
+   leave them. */
+int plm_close_divergence(const plm_grid_info *info, float *u, float *v, float *w, float *divergence, void *stream);
 /* The per-cell boundary flags behind neighbor_term, sim/pressure.py:39-69: for neighbours x-,x+,y-,y+,z-,z+
    two bits each (0 in-domain, 1 open-face ghost, 2 closed), bits 12..14 = last cell on x, y, z. */
 int plm_build_flags(const plm_grid_info *info, uint16_t *flags, void *stream);

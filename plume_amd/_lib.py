@@ -90,6 +90,7 @@ SIGNATURES = {
     "plm_dissipate": [_L, _P, _P, _F, _F, _F, _P],
     "plm_zero_closed_faces": [_INFO, _P, _P, _P, _P],
     "plm_divergence": [_INFO, _P, _P, _P, _P, _P],
+    "plm_close_divergence": [_INFO, _P, _P, _P, _P, _P],
     "plm_build_flags": [_INFO, _P, _P],
     "plm_rb_sweep": [_INFO, _P, _P, _I, _F, _P],
     "plm_rb_sweeps": [_INFO, _P, _P, _P, _I, _F, _P],

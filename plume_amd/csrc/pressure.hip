@@ -63,6 +63,33 @@ __global__ __launch_bounds__(256) void k_divergence(plm_grid_info g, const float
     div[nc.at(i, j, k)] = flux / g.cell_m;
 }
 
+// zero_closed_faces_kernel then divergence_kernel (sim/pressure.py:21-36, 146-149) in one pass: each cell reads
+// a closed boundary face as the 0 the zeroing writes, and the cell beside that face writes the 0, so the
+// divergence and the faces end as the two kernels leave them. Every load comes before the stores.
+__global__ __launch_bounds__(256) void k_close_divergence(plm_grid_info g, float *__restrict__ u,
+                                                          float *__restrict__ v, float *__restrict__ w,
+                                                          float *__restrict__ div) {
+    const Dims nc = cell_dims(g), nu = face_dims(g, 0), nv = face_dims(g, 1), nw = face_dims(g, 2);
+    int i, j, k;
+    if (!thread_cell(nc, i, j, k)) return;
+    const bool zx_lo = i == 0 && g.open_lo[0] == 0, zx_hi = i == g.dims[0] - 1 && g.open_hi[0] == 0;
+    const bool zy_lo = j == 0 && g.open_lo[1] == 0, zy_hi = j == g.dims[1] - 1 && g.open_hi[1] == 0;
+    const bool zz_lo = k == 0 && g.open_lo[2] == 0, zz_hi = k == g.dims[2] - 1 && g.open_hi[2] == 0;
+    const int ux_lo = nu.at(i, j, k), ux_hi = nu.at(i + 1, j, k), vy_lo = nv.at(i, j, k), vy_hi = nv.at(i, j + 1, k);
+    const int wz_lo = nw.at(i, j, k), wz_hi = nw.at(i, j, k + 1);
+    const float u_lo = zx_lo ? 0.0f : u[ux_lo], u_hi = zx_hi ? 0.0f : u[ux_hi];
+    const float v_lo = zy_lo ? 0.0f : v[vy_lo], v_hi = zy_hi ? 0.0f : v[vy_hi];
+    const float w_lo = zz_lo ? 0.0f : w[wz_lo], w_hi = zz_hi ? 0.0f : w[wz_hi];
+    const float flux = (u_hi - u_lo) + (v_hi - v_lo) + (w_hi - w_lo);
+    div[nc.at(i, j, k)] = flux / g.cell_m;
+    if (zx_lo) u[ux_lo] = 0.0f;
+    if (zx_hi) u[ux_hi] = 0.0f;
+    if (zy_lo) v[vy_lo] = 0.0f;
+    if (zy_hi) v[vy_hi] = 0.0f;
+    if (zz_lo) w[wz_lo] = 0.0f;
+    if (zz_hi) w[wz_hi] = 0.0f;
+}
+
 // The integer content of neighbor_term's branches (sim/pressure.py:52-69), per cell.
 __device__ __forceinline__ int neighbor_kind(const plm_grid_info &g, int axis, int index, int step) {
     const int n = index + step;
@@ -3574,6 +3601,13 @@ int plm_divergence(const plm_grid_info *info, const float *u, const float *v, co
     PLM_REQUIRE(u && v && w && divergence, "plm_divergence: NULL array");
     k_divergence<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, u, v, w, divergence);
     return check_launch("plm_divergence");
+}
+
+int plm_close_divergence(const plm_grid_info *info, float *u, float *v, float *w, float *divergence, void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(u && v && w && divergence, "plm_close_divergence: NULL array");
+    k_close_divergence<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, u, v, w, divergence);
+    return check_launch("plm_close_divergence");
 }
 
 int plm_build_flags(const plm_grid_info *info, uint16_t *flags, void *stream) {

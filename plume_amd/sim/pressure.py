@@ -58,6 +58,14 @@ def compute_divergence(grid) -> torch.Tensor:
     return grid.divergence
 
 
+def close_faces_and_divergence(grid) -> torch.Tensor:
+    """zero_closed_faces then compute_divergence in one launch (sim/pressure.py:146-149): the same faces and
+    divergence. Fills and returns `grid.divergence`."""
+    u, v, w = (f.data for f in grid.velocity_fields())
+    call("plm_close_divergence", ctypes.byref(grid.info), ptr(u), ptr(v), ptr(w), ptr(grid.divergence), stream())
+    return grid.divergence
+
+
 def subtract_gradient(grid) -> None:
     """subtract_gradient_kernel on u, v and w (sim/pressure.py:95-116, 152-154)."""
     u, v, w = (f.data for f in grid.velocity_fields())
@@ -67,8 +75,7 @@ def subtract_gradient(grid) -> None:
 def project(grid, solve: PressureSolve) -> None:
     """Makes the face velocities divergence-free (to the solver's accuracy) and walls impermeable
     (sim/pressure.py:144-154)."""
-    zero_closed_faces(grid)
-    compute_divergence(grid)
+    close_faces_and_divergence(grid)
     solve(grid)
     subtract_gradient(grid)
 

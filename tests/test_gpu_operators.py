@@ -281,6 +281,22 @@ def test_divergence_sweeps_and_gradient_are_bit_exact(device, oracle32, cells, f
         assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
 
 
+@pytest.mark.parametrize("cells, faces", GRIDS)
+def test_closing_the_faces_with_the_divergence_is_bit_exact(device, oracle32, cells, faces):
+    """plm_close_divergence (project's zeroing of the closed faces and its divergence in one launch) leaves the
+    faces and the divergence as the oracle's two steps do."""
+    from plume_amd.sim.pressure import close_faces_and_divergence
+
+    gpu, orc = pair(device, oracle32, cells, faces)
+    close_faces_and_divergence(gpu)
+    for axis, f in enumerate(orc.velocity_fields()):
+        oracle32.zero_closed_faces(orc.info, f.data, axis)
+    oracle32.divergence(orc.info, *(f.data for f in orc.velocity_fields()), orc.divergence)
+    assert bit_equal(to_np(gpu.divergence), orc.divergence)
+    for name in ("u", "v", "w"):
+        assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
+
+
 @pytest.mark.parametrize("cells", [(64, 96, 64), (65, 47, 33)], ids=["64x96x64", "odd-large"])
 def test_fused_sweep_on_large_grids_is_bit_exact(device, oracle32, cells):
     from plume_amd.sim.pressure import red_black_sweeps
