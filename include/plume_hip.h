@@ -178,6 +178,14 @@ int plm_confinement(const plm_grid_info *info, float *u, float *v, float *w, flo
    three face arrays. `emitters` is a device array of n_emitters structs. */
 int plm_emit(const plm_grid_info *info, float *smoke, float *temperature, float *u, float *v, float *w,
              const plm_emitter *emitters, int32_t n_emitters, int32_t seed, float time_s, float dt, void *stream);
+/* The same two emit kernels (sim/emitters.py:90-121, 147-153) launched only over the cells [lo, hi) and their
+   faces, in one launch. Every cell and face outside that box must lie at least an emitter's extent (radius or
+   half size) from each active emitter's centre, where coverage is 0 and the kernels change nothing: the
+   caller passes a box that holds every active emitter's support (plume_amd.sim.emitters.emit_box). The result
+   is then bit for bit plm_emit's. 0 <= lo <= hi <= dims per axis. */
+int plm_emit_box(const plm_grid_info *info, float *smoke, float *temperature, float *u, float *v, float *w,
+                 const plm_emitter *emitters, int32_t n_emitters, int32_t seed, float time_s, float dt,
+                 const int32_t lo[3], const int32_t hi[3], void *stream);
 
 /* value_noise, sim/noise.py:40-57, at `count` points (xyz AoS) -> out; lattice_value (:24-32) when
    points is NULL and `lattice` holds (i,j,k) int triples. For parity tests of the integer hash. */

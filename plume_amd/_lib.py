@@ -85,6 +85,7 @@ SIGNATURES = {
     "plm_add_face_push": [_INFO, _P, _P, _P, _P, _F, _P],
     "plm_confinement": [_INFO, _P, _P, _P, _P, _P, _P, _F, _F, _I, _P],
     "plm_emit": [_INFO, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P],
+    "plm_emit_box": [_INFO, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P],
     "plm_noise_probe": [_I, _I, _P, _P, _I, _P, _P],
     "plm_turbulence": [_INFO, _P, _P, _P, _I, _F, _F, _F, _F, _P],
     "plm_dissipate": [_L, _P, _P, _F, _F, _F, _P],

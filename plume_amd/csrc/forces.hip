@@ -282,14 +282,11 @@ __device__ __forceinline__ bool is_active(const plm_emitter &e, float time_s) {
     return time_s >= e.start_s && time_s < e.end_s;
 }
 
-// emit_scalars_kernel, sim/emitters.py:90-102
-__global__ __launch_bounds__(256) void k_emit_scalars(plm_grid_info g, float *__restrict__ smoke,
-                                                      float *__restrict__ temperature,
-                                                      const plm_emitter *__restrict__ emitters, int n_emitters,
-                                                      int32_t seed, float time_s, float dt) {
+// emit_scalars_kernel, sim/emitters.py:90-102, at cell (i, j, k)
+__device__ __forceinline__ void emit_cell(const plm_grid_info &g, float *__restrict__ smoke,
+                                          float *__restrict__ temperature, const plm_emitter *__restrict__ emitters,
+                                          int n_emitters, int32_t seed, float time_s, float dt, int i, int j, int k) {
     const Dims nc = cell_dims(g);
-    int i, j, k;
-    if (!thread_cell(nc, i, j, k)) return;
     const float x = g.origin_m[0] + ((float)i + 0.5f) * g.cell_m;
     const float y = g.origin_m[1] + ((float)j + 0.5f) * g.cell_m;
     const float z = g.origin_m[2] + ((float)k + 0.5f) * g.cell_m;
@@ -305,17 +302,24 @@ __global__ __launch_bounds__(256) void k_emit_scalars(plm_grid_info g, float *__
     }
 }
 
-// emit_velocity_kernel, sim/emitters.py:105-121, on all three face arrays
-__global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, FaceLaunch fl, Tri<float> f,
-                                                       const plm_emitter *__restrict__ emitters, int n_emitters,
-                                                       int32_t seed, float time_s, float dt) {
-    int axis, i, j, k;
-    if (!packed_face(g, fl, axis, i, j, k)) return;
+__global__ __launch_bounds__(256) void k_emit_scalars(plm_grid_info g, float *__restrict__ smoke,
+                                                      float *__restrict__ temperature,
+                                                      const plm_emitter *__restrict__ emitters, int n_emitters,
+                                                      int32_t seed, float time_s, float dt) {
+    int i, j, k;
+    if (!thread_cell(cell_dims(g), i, j, k)) return;
+    emit_cell(g, smoke, temperature, emitters, n_emitters, seed, time_s, dt, i, j, k);
+}
+
+// emit_velocity_kernel, sim/emitters.py:105-121, at face (i, j, k) of face array `axis`
+__device__ __forceinline__ void emit_face(const plm_grid_info &g, float *__restrict__ field, int axis,
+                                          const plm_emitter *__restrict__ emitters, int n_emitters, int32_t seed,
+                                          float time_s, float dt, int i, int j, int k) {
     const Dims d = face_dims(g, axis);
     const float x = g.origin_m[0] + ((float)i + (axis == 0 ? 0.0f : 0.5f)) * g.cell_m;
     const float y = g.origin_m[1] + ((float)j + (axis == 1 ? 0.0f : 0.5f)) * g.cell_m;
     const float z = g.origin_m[2] + ((float)k + (axis == 2 ? 0.0f : 0.5f)) * g.cell_m;
-    float *p = f.p[axis] + d.at(i, j, k);
+    float *p = field + d.at(i, j, k);
     for (int n = 0; n < n_emitters; n++) {
         const plm_emitter e = emitters[n];
         if (!is_active(e, time_s)) continue;
@@ -335,6 +339,36 @@ __global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, FaceLaun
             *p = lerp_(*p, target[axis], blend_factor(amount, dt));
         }
     }
+}
+
+// emit_velocity_kernel on all three face arrays (the packed face launch)
+__global__ __launch_bounds__(256) void k_emit_velocity(plm_grid_info g, FaceLaunch fl, Tri<float> f,
+                                                       const plm_emitter *__restrict__ emitters, int n_emitters,
+                                                       int32_t seed, float time_s, float dt) {
+    int axis, i, j, k;
+    if (!packed_face(g, fl, axis, i, j, k)) return;
+    emit_face(g, f.p[axis], axis, emitters, n_emitters, seed, time_s, dt, i, j, k);
+}
+
+// Both emit kernels over a box of cells [lo, hi) and the faces of those cells (a face array's index runs to hi on
+// its own axis): one thread per point of the box (hi - lo + 1 per axis), the cell and up to three faces.
+struct EmitBox {
+    int lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(256) void k_emit_box(plm_grid_info g, EmitBox box, float *__restrict__ smoke,
+                                                  float *__restrict__ temperature, Tri<float> f,
+                                                  const plm_emitter *__restrict__ emitters, int n_emitters,
+                                                  int32_t seed, float time_s, float dt) {
+    const Dims span{box.hi[0] - box.lo[0] + 1, box.hi[1] - box.lo[1] + 1, box.hi[2] - box.lo[2] + 1};
+    int a, b, c;
+    if (!thread_cell(span, a, b, c)) return;
+    const int i = box.lo[0] + a, j = box.lo[1] + b, k = box.lo[2] + c;
+    const bool in_i = i < box.hi[0], in_j = j < box.hi[1], in_k = k < box.hi[2];
+    if (in_i && in_j && in_k) emit_cell(g, smoke, temperature, emitters, n_emitters, seed, time_s, dt, i, j, k);
+    if (in_j && in_k) emit_face(g, f.p[0], 0, emitters, n_emitters, seed, time_s, dt, i, j, k);
+    if (in_i && in_k) emit_face(g, f.p[1], 1, emitters, n_emitters, seed, time_s, dt, i, j, k);
+    if (in_i && in_j) emit_face(g, f.p[2], 2, emitters, n_emitters, seed, time_s, dt, i, j, k);
 }
 
 // ------------------------------------------------------------------ turbulence, sim/turbulence.py
@@ -595,6 +629,28 @@ int plm_emit(const plm_grid_info *info, float *smoke, float *temperature, float 
     k_emit_velocity<<<fl.block_end[2], FACE_BLOCK, 0, s>>>(*info, fl, Tri<float>{{u, v, w}}, emitters, n_emitters, seed,
                                                           time_s, dt);
     return check_launch("plm_emit velocity");
+}
+
+int plm_emit_box(const plm_grid_info *info, float *smoke, float *temperature, float *u, float *v, float *w,
+                 const plm_emitter *emitters, int32_t n_emitters, int32_t seed, float time_s, float dt,
+                 const int32_t lo[3], const int32_t hi[3], void *stream) {
+    PLM_TRY(validate_info(info));
+    PLM_REQUIRE(smoke && temperature && u && v && w && lo && hi, "plm_emit_box: NULL array");
+    PLM_REQUIRE(n_emitters >= 0 && (n_emitters == 0 || emitters), "plm_emit_box: bad emitter array");
+    EmitBox box{};
+    for (int a = 0; a < 3; a++) {
+        PLM_REQUIRE(0 <= lo[a] && lo[a] <= hi[a] && hi[a] <= info->dims[a],
+                    "plm_emit_box: box [%d, %d) outside the grid's %d cells on axis %d", lo[a], hi[a], info->dims[a], a);
+        box.lo[a] = lo[a];
+        box.hi[a] = hi[a];
+    }
+    if (n_emitters == 0) return 0;
+    hipStream_t s = as_stream(stream);
+    TimingScope t("emit", s);
+    const Dims span{hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    k_emit_box<<<grid3(span), block3(), 0, s>>>(*info, box, smoke, temperature, Tri<float>{{u, v, w}}, emitters,
+                                                 n_emitters, seed, time_s, dt);
+    return check_launch("plm_emit_box");
 }
 
 int plm_noise_probe(int32_t seed, int32_t stream_id, const float *points, const int32_t *lattice, int32_t count,

@@ -219,6 +219,50 @@ def test_emit_matches_oracle(device, oracle32, cells, faces, variant):
         compare(to_np(getattr(gpu, name).data), getattr(orc, name).data, what=name)
 
 
+@pytest.mark.parametrize("case", ["sphere-noise", "box-outward", "corner-and-inactive", "outside-the-grid"])
+def test_emit_over_the_emitters_box_equals_the_whole_grid_launch(device, oracle32, case):
+    """plm_emit_box over the active emitters' box (plume_amd.sim.emitters.emit) leaves every field bit for bit as
+    plm_emit over the whole grid: an emitter in the middle, a box emitter with outward velocity, an emitter
+    overhanging a corner beside one not yet active, and an emitter wholly outside the grid."""
+    from support import tiny_recipe
+
+    from plume_amd.sim.emitters import emit, emit_box, emit_whole_grid, emitter_params
+
+    base = {"name": "e", "center_m": [0.0, 0.12, 0.0], "radius_m": 0.06, "end_s": 1.0, "smoke_per_s": 2.0,
+            "temperature_k": 900.0, "velocity_mps": [0.1, 0.5, -0.2], "edge_softness": 0.6,
+            "noise": {"scale_m": 0.03, "strength": 0.7}}
+    if case == "sphere-noise":
+        emitters = [base]
+    elif case == "box-outward":
+        emitters = [dict(base, shape="box", size_m=[0.1, 0.08, 0.12], velocity_mode="outward")]
+    elif case == "corner-and-inactive":
+        emitters = [dict(base, center_m=[0.15, 0.02, -0.14], radius_m=0.05),
+                    dict(base, name="later", center_m=[-0.1, 0.2, 0.1], start_s=0.5)]
+    else:
+        emitters = [dict(base, center_m=[1.0, 0.1, 0.0], radius_m=0.05)]
+    settings = tiny_recipe(emitter=emitters).emitter
+    params = emitter_params(settings, device)
+    gpu_box, _ = pair(device, oracle32, (31, 33, 29), DEFAULT_FACES, k=3)
+    gpu_all, _ = pair(device, oracle32, (31, 33, 29), DEFAULT_FACES, k=3)
+    box = emit_box(gpu_box.info, params.params, 0.1)
+    if case == "outside-the-grid":
+        assert box is None
+    else:
+        lo, hi = box
+        assert all(0 <= lo[a] < hi[a] <= gpu_box.info.dims[a] for a in range(3))
+        assert any(hi[a] - lo[a] < gpu_box.info.dims[a] for a in range(3)), "the box should not be the whole grid"
+    names = ("smoke", "temperature", "u", "v", "w")
+    before = {name: to_np(getattr(gpu_all, name).data) for name in names}
+    emit(gpu_box, params, 7, 0.1, DT)
+    emit_whole_grid(gpu_all, params, 7, 0.1, DT)
+    changed = False
+    for name in names:
+        a, b = to_np(getattr(gpu_box, name).data), to_np(getattr(gpu_all, name).data)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+        changed = changed or not np.array_equal(b.view(np.uint32), before[name].view(np.uint32))
+    assert changed == (case != "outside-the-grid")
+
+
 @pytest.mark.parametrize("cells, faces", GRIDS[:2])
 def test_turbulence_matches_oracle(device, oracle32, cells, faces):
     from oracle.sim import apply_turbulence as oracle_turbulence
