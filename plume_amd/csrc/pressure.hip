@@ -1298,11 +1298,39 @@ __device__ __forceinline__ void prolong_row_cell(const T *__restrict__ coarse, D
     *cell = from_f32<T>(to_f32(*cell) + p);
 }
 
+
+// prolong_add_kernel by rows (prolong_row_cell's values) with four rows per thread (rows j, j + BY, j + 2 BY, j + 3 BY of the block's 4 BY rows, each
+// uniform over a wavefront): a quarter of the wavefronts, each with the four cells' loads in flight together,
+// and every load issued before the gate is read (the loads are in range whatever the gate says), so the gate's
+// latency overlaps theirs. The values are prolong_row_cell's.
+constexpr int PROLONG_ROWS_PER_THREAD = 4;
 template <typename T = float>
-__global__ __launch_bounds__(256) void k_prolong_rows(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine,
-                                                      Dims nf, const double *gate) {
+__global__ __launch_bounds__(256) void k_prolong_rows4(const T *__restrict__ coarse, Dims nc, T *__restrict__ fine,
+                                                       Dims nf, const double *gate) {
+    constexpr int R = PROLONG_ROWS_PER_THREAD;
+    const int k = blockIdx.x * BX + threadIdx.x, i = blockIdx.z;
+    const int j0 = uniform((int)(blockIdx.y * BY * R + threadIdx.y));
+    const int kk = min(k, nf.n2 - 1);
+    uint32_t pk0, pk1;
+    float tz;
+    prolong_k<T>(nc, kk, pk0, pk1, tz);
+    float p[R], old[R];
+    T *cell[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = min(j0 + r * BY, nf.n1 - 1);
+        p[r] = prolong_at_row(coarse, nc, i, j, pk0, pk1, tz);
+        cell[r] = reinterpret_cast<T *>(reinterpret_cast<char *>(fine) + offset_of(nf, i, j, kk) * (uint32_t)sizeof(T));
+        old[r] = to_f32(*cell[r]);
+    }
     if (gated_off(gate)) return;
-    prolong_row_cell<T>(coarse, nc, fine, nf, blockIdx);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (k < nf.n2 && j0 + r * BY < nf.n1) *cell[r] = from_f32<T>(old[r] + p[r]);
+}
+
+inline dim3 prolong_rows4_grid(Dims n) {
+    return dim3((n.n2 + BX - 1) / BX, (n.n1 + BY * PROLONG_ROWS_PER_THREAD - 1) / (BY * PROLONG_ROWS_PER_THREAD), n.n0);
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t count, float *__restrict__ f, const double *gate) {
@@ -2020,7 +2048,7 @@ int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch
     }
     // On large levels the prolongation is cheaper as its own row kernel than sampled inside the tile load.
     if (start == FROM_PROLONG && n.count() >= PROLONG_ROWS_CELLS) {
-        k_prolong_rows<float><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
+        k_prolong_rows4<float><<<prolong_rows4_grid(n), block3(), 0, s>>>(coarse, nc, q, n, gate);
         PLM_TRY(check_launch("prolong rows"));
         start = FROM_Q;
     }
@@ -2643,7 +2671,7 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_f16x2(LevelArgs L, const _
         sweep_tile_f16x2<TI, TJ, THREADS, START, true>(L, s, q_in, q_out, rhs, i0, j0, k0);
 }
 
-// The prolongation of a packed level in FP16 storage (the values of k_prolong_rows<__half>, bit for bit):
+// The prolongation of a packed level in FP16 storage (the values of k_prolong_rows4<__half>, bit for bit):
 // q = half(float(q) + p) with p the trilinear interpolation of the coarse correction
 // (prolongation_kernel, sim/multigrid.py:63-80). A thread takes four fine cells along k, k = 4t .. 4t + 3,
 // whose interpolation reads coarse k 2t - 1 .. 2t + 2: three dwords of each of the four coarse rows around
@@ -2831,7 +2859,7 @@ int lp_prolong(const LevelArgs &L, T *q, const T *coarse, Dims nc, hipStream_t s
             return check_launch("packed prolongation");
         }
     }
-    k_prolong_rows<T><<<grid3(n), block3(), 0, s>>>(coarse, nc, q, n, nullptr);
+    k_prolong_rows4<T><<<prolong_rows4_grid(n), block3(), 0, s>>>(coarse, nc, q, n, nullptr);
     return check_launch("prolong rows");
 }
 
