@@ -525,7 +525,7 @@ template <int TI, int TJ, int THREADS, int START, bool EDGE, typename T, typenam
 __device__ __forceinline__ void sweep_tile_flat(const LevelArgs &L, float *__restrict__ s, const T *__restrict__ q_in,
                                                 T *__restrict__ q_out, const T *__restrict__ rhs,
                                                 const T *__restrict__ coarse, Dims nc, float relaxation, int i0,
-                                                int j0, int k0, EdgeUpdate edge) {
+                                                int j0, int k0, EdgeUpdate edge, const double *gate) {
     constexpr int WAVES = THREADS / 64;
     constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
     constexpr uint32_t B = sizeof(T);
@@ -665,6 +665,9 @@ __device__ __forceinline__ void sweep_tile_flat(const LevelArgs &L, float *__res
             ii += wrap ? 1 : 0;
             g += wrap ? n12 + (WAVES - LJ) * n.n2 : WAVES * n.n2;
         }
+        // the device-side gate is read once the first loads are in flight (they are in range whatever it says),
+        // so its latency overlaps theirs; a gated-off block leaves before any write or barrier
+        if (b == 0 && gated_off(gate)) return;
 #pragma unroll
         for (int u = 0; u < BATCH; u++) s[(wave + (b + u) * WAVES) * LK + own_slot] = v[u];
     }
@@ -789,7 +792,7 @@ template <int TI, int TJ, int THREADS, bool UNIT, int START, typename T>
 __device__ __forceinline__ void sweep_tile(const LevelArgs &L, const T *__restrict__ q_in, T *__restrict__ q_out,
                                            const T *__restrict__ rhs, const T *__restrict__ coarse, Dims nc,
                                            float relaxation, int tile, FastDiv tiles_k, FastDiv tiles_j, float *s,
-                                           float *cps) {
+                                           float *cps, const double *gate = nullptr) {
     constexpr int WAVES = THREADS / 64;
     constexpr int LJ = TJ + 2 * TH, LI = TI + 2 * TH, HALF = LK / 2;
     constexpr int RI = TI + 2, RJ = TJ + 2;
@@ -859,12 +862,13 @@ __device__ __forceinline__ void sweep_tile(const LevelArgs &L, const T *__restri
         };
         if (i0 >= TH && i0 + TI + TH <= n.n0 && j0 >= TH && j0 + TJ + TH <= n.n1 && k0 + TK <= n.n2)
             sweep_tile_flat<TI, TJ, THREADS, START, false, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0,
-                                                              k0, edge_cells);
+                                                              k0, edge_cells, gate);
         else
             sweep_tile_flat<TI, TJ, THREADS, START, true, T>(L, s, q_in, q_out, rhs, coarse, nc, relaxation, i0, j0,
-                                                             k0, edge_cells);
+                                                             k0, edge_cells, gate);
         return;
     }
+    if (gated_off(gate)) return;
     float rhs_red[RED_ITERS], rhs_black[BLACK_ITERS], rhs_edge = 0.0f;
 #pragma unroll
     for (int u = 0; u < RED_ITERS; u++) {
@@ -969,11 +973,10 @@ __global__ __launch_bounds__(THREADS) void k_rb_sweep_tile(LevelArgs L, const T 
                                                            const T *__restrict__ coarse, Dims nc,
                                                            float relaxation, const double *gate, FastDiv tiles_k,
                                                            FastDiv tiles_j) {
-    if (gated_off(gate)) return;
     __shared__ float s[(TI + 2 * TH) * (TJ + 2 * TH) * LK];
     __shared__ float cps[UNIT ? 1 : CP_WORDS];
     sweep_tile<TI, TJ, THREADS, UNIT, START, T>(L, q_in, q_out, rhs, coarse, nc, relaxation,
-                                                xcd_contiguous(blockIdx.x, gridDim.x), tiles_k, tiles_j, s, cps);
+                                                xcd_contiguous(blockIdx.x, gridDim.x), tiles_k, tiles_j, s, cps, gate);
 }
 
 // The tile shapes the autotuner chooses among: TI, TJ (k is 64 wide), threads per block.
@@ -1111,7 +1114,8 @@ __global__ __launch_bounds__(256) void k_restrict(const float *__restrict__ fine
 template <bool UNIT, typename T>
 __device__ __forceinline__ void residual_restrict_cell(const LevelArgs &L, const T *__restrict__ q,
                                                        const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
-                                                       T *__restrict__ coarse_q, Dims nc, dim3 block) {
+                                                       T *__restrict__ coarse_q, Dims nc, dim3 block,
+                                                       const double *gate = nullptr) {
     const Dims nf = cell_dims(L.g);
     const int k = block.x * BX + threadIdx.x, j = block.y * BY + threadIdx.y, i = block.z;
     if (!(k < nc.n2 && j < nc.n1 && i < nc.n0)) return;
@@ -1198,6 +1202,9 @@ __device__ __forceinline__ void residual_restrict_cell(const LevelArgs &L, const
                     }
         }
     }
+    // the device-side gate is read only before the stores, so its latency hides behind the loads (a gated-off
+    // launch, rare, then does the reads and arithmetic for nothing)
+    if (gated_off(gate)) return;
     const int c = nc.at(i, j, k);
     coarse_rhs[c] = from_f32<T>(total / (float)count);
     if (coarse_q != nullptr) coarse_q[c] = from_f32<T>(0.0f);
@@ -1208,8 +1215,7 @@ template <bool UNIT, typename T = float>
 __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T *__restrict__ q,
                                                            const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
                                                            T *__restrict__ coarse_q, Dims nc, const double *gate) {
-    if (gated_off(gate)) return;
-    residual_restrict_cell<UNIT, T>(L, q, rhs, coarse_rhs, coarse_q, nc, blockIdx);
+    residual_restrict_cell<UNIT, T>(L, q, rhs, coarse_rhs, coarse_q, nc, blockIdx, gate);
 }
 
 // k_residual_restrict for levels with couplings other than 1, with one thread per child: lanes 8 m + 4 di +
@@ -1222,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_residual_restrict(LevelArgs L, const T 
 template <typename T>
 __device__ __forceinline__ void residual_restrict_children(const LevelArgs &L, const T *__restrict__ q,
                                                            const T *__restrict__ rhs, T *__restrict__ coarse_rhs,
-                                                           Dims nc, uint32_t block) {
+                                                           Dims nc, uint32_t block, const double *gate = nullptr) {
     const Dims nf = cell_dims(L.g);
     const int child = threadIdx.x & 7;
     const int cell = (int)(block * 32u + (threadIdx.x >> 3));
@@ -1257,6 +1263,7 @@ __device__ __forceinline__ void residual_restrict_children(const LevelArgs &L, c
         total = total + __shfl(r, lead + c, 64);
         count += __shfl(exists ? 1 : 0, lead + c, 64);
     }
+    if (gated_off(gate)) return;  // read only before the store, as in residual_restrict_cell
     if (child == 0 && valid) coarse_rhs[cell] = from_f32<T>(total / (float)count);
 }
 
@@ -1265,8 +1272,7 @@ __global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L,
                                                                     const T *__restrict__ rhs,
                                                                     T *__restrict__ coarse_rhs, Dims nc,
                                                                     const double *gate) {
-    if (gated_off(gate)) return;
-    residual_restrict_children<T>(L, q, rhs, coarse_rhs, nc, blockIdx.x);
+    residual_restrict_children<T>(L, q, rhs, coarse_rhs, nc, blockIdx.x, gate);
 }
 
 inline dim3 children_grid(const Dims &nc) { return dim3((unsigned)((nc.count() + 31) / 32)); }
@@ -1710,7 +1716,7 @@ __device__ __forceinline__ int coarsest_sweeps(const TailLevel &L) {
 // KEEP block levels build their plans once, on the way down, and keep them in registers for the way up.
 template <bool ONES>
 __device__ __forceinline__ void mg_tail(const TailLevels &args, float *__restrict__ q_top,
-                                        const float *__restrict__ rhs_top, int sweeps) {
+                                        const float *__restrict__ rhs_top, int sweeps, const double *gate = nullptr) {
     constexpr int K = ONES ? BLOCK_PLAN_ONES : BLOCK_PLAN;
     constexpr int KEEP = ONES ? 2 : 1;
     extern __shared__ float lds[];
@@ -1727,6 +1733,7 @@ __device__ __forceinline__ void mg_tail(const TailLevels &args, float *__restric
         top_rhs[u] = c < top_cells ? rhs_top[c] : 0.0f;
         top_qv[u] = c < top_cells && top_q ? q_top[c] : 0.0f;
     }
+    if (gated_off(gate)) return;  // read once the prefetch is in flight (uniform, before any barrier)
     // The level table is indexed by runtime level numbers; read from the kernel argument that way it would
     // be copied to per-thread scratch, so it is staged in LDS once.
     __shared__ TailLevels t;
@@ -1827,8 +1834,7 @@ template <bool ONES>
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailLevels args, float *__restrict__ q_top,
                                                           const float *__restrict__ rhs_top, int sweeps,
                                                           const double *gate) {
-    if (gated_off(gate)) return;
-    mg_tail<ONES>(args, q_top, rhs_top, sweeps);
+    mg_tail<ONES>(args, q_top, rhs_top, sweeps, gate);
 }
 
 // ------------------------------------------------------------------ FP64 norms (tolerance mode)
@@ -1860,7 +1866,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool UNIT>
 __device__ __forceinline__ void norm_partials(const LevelArgs &L, const float *__restrict__ q,
                                               const float *__restrict__ rhs, double *__restrict__ partials,
-                                              dim3 block, dim3 grid) {
+                                              dim3 block, dim3 grid, const double *gate = nullptr) {
     const Dims n = cell_dims(L.g);
     const int k = block.x * BX + threadIdx.x, j = block.y * BY + threadIdx.y;
     double rr = 0.0, bb = 0.0;
@@ -1908,6 +1914,9 @@ __device__ __forceinline__ void norm_partials(const LevelArgs &L, const float *_
     }
     rr = wave_sum(rr);
     bb = wave_sum(bb);
+    // the device-side gate is read only now, its latency hidden behind the loads (uniform: the whole block
+    // leaves before the barrier)
+    if (gated_off(gate)) return;
     __shared__ double srr[NORM_THREADS / 64], sbb[NORM_THREADS / 64];
     const int wave = threadIdx.y;  // blocks are 64 x 4: one wavefront per row of j
     if (threadIdx.x == 0) {
@@ -1926,8 +1935,7 @@ template <bool UNIT>
 __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, const float *__restrict__ q,
                                                                 const float *__restrict__ rhs,
                                                                 double *__restrict__ partials, const double *gate) {
-    if (gated_off(gate)) return;
-    norm_partials<UNIT>(L, q, rhs, partials, blockIdx, gridDim);
+    norm_partials<UNIT>(L, q, rhs, partials, blockIdx, gridDim, gate);
 }
 
 // scratch layout: [0] relative residual, [1] gate (1 active / 0 converged), [2] |r|^2, [3] |rhs|^2,
