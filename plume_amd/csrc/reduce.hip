@@ -52,9 +52,22 @@ __global__ __launch_bounds__(BUDGET_THREADS) void k_budget_totals(int plane, con
                                                                   double *__restrict__ totals,
                                                                   double *__restrict__ copy) {
     __shared__ double s[3][BUDGET_THREADS];
+    // each thread's lines in increasing order, BURST of them loaded at once (one memory round trip per burst,
+    // where a load per step made every step wait for the last)
+    constexpr int BURST = 8;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < plane; t += BUDGET_THREADS)
-        for (int q = 0; q < 3; q++) acc[q] += (double)lines[q * plane + t];
+    for (int t0 = threadIdx.x; t0 < plane; t0 += BURST * BUDGET_THREADS) {
+        float v[3][BURST];
+#pragma unroll
+        for (int u = 0; u < BURST; u++) {
+            const int t = min(t0 + u * BUDGET_THREADS, plane - 1);
+            for (int q = 0; q < 3; q++) v[q][u] = lines[q * plane + t];
+        }
+#pragma unroll
+        for (int u = 0; u < BURST; u++)
+            if (t0 + u * BUDGET_THREADS < plane)
+                for (int q = 0; q < 3; q++) acc[q] += (double)v[q][u];
+    }
     for (int q = 0; q < 3; q++) s[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int half = BUDGET_THREADS / 2; half > 0; half /= 2) {
@@ -77,15 +90,17 @@ __global__ __launch_bounds__(BUDGET_THREADS) void k_budget_totals(int plane, con
 // scale_kernel, sim/conserve.py:27-30, applied only when the factor is not 1 (sim/conserve.py:51-52)
 __global__ __launch_bounds__(256) void k_scale_by_total(int64_t count, float *__restrict__ f,
                                                         const double *__restrict__ totals) {
-    const double factor = totals[3];
-    if (factor == 1.0) return;
-    const float scale = (float)factor;
-    // four values per thread (one 16-byte access) over the aligned bulk, then the tail one by one
+    // four values per thread (one 16-byte access) over the aligned bulk, then the tail one by one; the first
+    // values are read before the factor, so the two loads' latencies overlap
     const int64_t quads = (reinterpret_cast<uintptr_t>(f) & 15u) == 0 ? count / 4 : 0;
     const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
     float4 *f4 = reinterpret_cast<float4 *>(f);
+    const float4 head = first < quads ? f4[first] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    const double factor = totals[3];
+    if (factor == 1.0) return;
+    const float scale = (float)factor;
     for (int64_t c = first; c < quads; c += step) {
-        float4 x = f4[c];
+        float4 x = c == first ? head : f4[c];
         x.x = x.x * scale;
         x.y = x.y * scale;
         x.z = x.z * scale;
