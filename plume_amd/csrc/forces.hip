@@ -7,21 +7,44 @@ namespace {
 
 // ------------------------------------------------------------------ buoyancy, drag
 
-// buoyancy_kernel, sim/forces.py:13-23
+// buoyancy_kernel, sim/forces.py:13-23, BUOYANCY_ROWS consecutive v rows (j) per thread: the rows' temperature
+// and smoke (one more row than faces, the rows below and above shared between neighbouring faces) and their v
+// are loaded first, then each face updated with the reference's operations. A quarter of the wavefronts of a
+// face per thread, each with its loads in flight together.
+constexpr int BUOYANCY_ROWS = 4;
 __global__ __launch_bounds__(256) void k_buoyancy(plm_grid_info g, float *__restrict__ v,
                                                   const float *__restrict__ smoke,
                                                   const float *__restrict__ temperature, float gravity,
                                                   float heat_buoyancy, float smoke_weight, float ambient_k,
                                                   float dt) {
+    constexpr int R = BUOYANCY_ROWS;
     const Dims nv = face_dims(g, 1), nc = cell_dims(g);
-    int i, j, k;
-    if (!thread_cell(nv, i, j, k)) return;
-    const int below = max(j - 1, 0), above = min(j, g.dims[1] - 1);
-    const float heat = 0.5f * (temperature[nc.at(i, below, k)] + temperature[nc.at(i, above, k)]);
-    const float density = 0.5f * (smoke[nc.at(i, below, k)] + smoke[nc.at(i, above, k)]);
-    const float lift = heat_buoyancy * (heat - ambient_k) / ambient_k - smoke_weight * density;
-    const int c = nv.at(i, j, k);
-    v[c] = v[c] + gravity * lift * dt;
+    const int k = blockIdx.x * BX + threadIdx.x, i = blockIdx.z;
+    const int j0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * BY + threadIdx.y) * R));
+    if (k >= nv.n2 || j0 >= nv.n1) return;
+    // row m holds cell row clamp(j0 - 1 + m): face j0 + r averages rows r (below) and r + 1 (above)
+    float heat_row[R + 1], smoke_row[R + 1], v_old[R];
+#pragma unroll
+    for (int m = 0; m <= R; m++) {
+        const int row = min(max(j0 - 1 + m, 0), g.dims[1] - 1);
+        heat_row[m] = temperature[nc.at(i, row, k)];
+        smoke_row[m] = smoke[nc.at(i, row, k)];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) v_old[r] = v[nv.at(i, min(j0 + r, nv.n1 - 1), k)];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (j0 + r >= nv.n1) break;
+        const float heat = 0.5f * (heat_row[r] + heat_row[r + 1]);
+        const float density = 0.5f * (smoke_row[r] + smoke_row[r + 1]);
+        const float lift = heat_buoyancy * (heat - ambient_k) / ambient_k - smoke_weight * density;
+        v[nv.at(i, j0 + r, k)] = v_old[r] + gravity * lift * dt;
+    }
+}
+
+inline dim3 buoyancy_grid(const plm_grid_info &g) {
+    const Dims nv = face_dims(g, 1);
+    return dim3((nv.n2 + BX - 1) / BX, (nv.n1 + BY * BUOYANCY_ROWS - 1) / (BY * BUOYANCY_ROWS), nv.n0);
 }
 
 // drag_kernel, sim/forces.py:34-37, on all three face arrays
@@ -544,7 +567,7 @@ int plm_buoyancy(const plm_grid_info *info, float *v, const float *smoke, const 
     PLM_REQUIRE(v && smoke && temperature, "plm_buoyancy: NULL array");
     hipStream_t s = as_stream(stream);
     TimingScope t("forces", s);
-    k_buoyancy<<<grid3(face_dims(*info, 1)), block3(), 0, s>>>(*info, v, smoke, temperature, gravity, heat_buoyancy,
+    k_buoyancy<<<buoyancy_grid(*info), block3(), 0, s>>>(*info, v, smoke, temperature, gravity, heat_buoyancy,
                                                               smoke_weight, ambient_k, dt);
     return check_launch("plm_buoyancy");
 }

@@ -65,29 +65,53 @@ __global__ __launch_bounds__(256) void k_divergence(plm_grid_info g, const float
 
 // zero_closed_faces_kernel then divergence_kernel (sim/pressure.py:21-36, 146-149) in one pass: each cell reads
 // a closed boundary face as the 0 the zeroing writes, and the cell beside that face writes the 0, so the
-// divergence and the faces end as the two kernels leave them. Every load comes before the stores.
+// divergence and the faces end as the two kernels leave them. DIVERGENCE_ROWS consecutive rows (j) per thread,
+// their v faces shared between neighbouring rows; every load comes before the stores.
+constexpr int DIVERGENCE_ROWS = 4;
 __global__ __launch_bounds__(256) void k_close_divergence(plm_grid_info g, float *__restrict__ u,
                                                           float *__restrict__ v, float *__restrict__ w,
                                                           float *__restrict__ div) {
+    constexpr int R = DIVERGENCE_ROWS;
     const Dims nc = cell_dims(g), nu = face_dims(g, 0), nv = face_dims(g, 1), nw = face_dims(g, 2);
-    int i, j, k;
-    if (!thread_cell(nc, i, j, k)) return;
+    const int k = blockIdx.x * BX + threadIdx.x, i = blockIdx.z;
+    const int j0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * BY + threadIdx.y) * R));
+    if (k >= nc.n2 || j0 >= nc.n1) return;
     const bool zx_lo = i == 0 && g.open_lo[0] == 0, zx_hi = i == g.dims[0] - 1 && g.open_hi[0] == 0;
-    const bool zy_lo = j == 0 && g.open_lo[1] == 0, zy_hi = j == g.dims[1] - 1 && g.open_hi[1] == 0;
     const bool zz_lo = k == 0 && g.open_lo[2] == 0, zz_hi = k == g.dims[2] - 1 && g.open_hi[2] == 0;
-    const int ux_lo = nu.at(i, j, k), ux_hi = nu.at(i + 1, j, k), vy_lo = nv.at(i, j, k), vy_hi = nv.at(i, j + 1, k);
-    const int wz_lo = nw.at(i, j, k), wz_hi = nw.at(i, j, k + 1);
-    const float u_lo = zx_lo ? 0.0f : u[ux_lo], u_hi = zx_hi ? 0.0f : u[ux_hi];
-    const float v_lo = zy_lo ? 0.0f : v[vy_lo], v_hi = zy_hi ? 0.0f : v[vy_hi];
-    const float w_lo = zz_lo ? 0.0f : w[wz_lo], w_hi = zz_hi ? 0.0f : w[wz_hi];
-    const float flux = (u_hi - u_lo) + (v_hi - v_lo) + (w_hi - w_lo);
-    div[nc.at(i, j, k)] = flux / g.cell_m;
-    if (zx_lo) u[ux_lo] = 0.0f;
-    if (zx_hi) u[ux_hi] = 0.0f;
-    if (zy_lo) v[vy_lo] = 0.0f;
-    if (zy_hi) v[vy_hi] = 0.0f;
-    if (zz_lo) w[wz_lo] = 0.0f;
-    if (zz_hi) w[wz_hi] = 0.0f;
+    // v faces j0 .. j0 + R (face m below row m, above row m - 1); rows past the grid read its last row
+    float u_lo[R], u_hi[R], w_lo[R], w_hi[R], v_face[R + 1];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = min(j0 + r, nc.n1 - 1);
+        u_lo[r] = u[nu.at(i, j, k)];
+        u_hi[r] = u[nu.at(i + 1, j, k)];
+        w_lo[r] = w[nw.at(i, j, k)];
+        w_hi[r] = w[nw.at(i, j, k + 1)];
+    }
+#pragma unroll
+    for (int m = 0; m <= R; m++) v_face[m] = v[nv.at(i, min(j0 + m, nv.n1 - 1), k)];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = j0 + r;
+        if (j >= nc.n1) break;
+        const bool zy_lo = j == 0 && g.open_lo[1] == 0, zy_hi = j == g.dims[1] - 1 && g.open_hi[1] == 0;
+        const float a_lo = zx_lo ? 0.0f : u_lo[r], a_hi = zx_hi ? 0.0f : u_hi[r];
+        const float b_lo = zy_lo ? 0.0f : v_face[r], b_hi = zy_hi ? 0.0f : v_face[r + 1];
+        const float c_lo = zz_lo ? 0.0f : w_lo[r], c_hi = zz_hi ? 0.0f : w_hi[r];
+        const float flux = (a_hi - a_lo) + (b_hi - b_lo) + (c_hi - c_lo);
+        div[nc.at(i, j, k)] = flux / g.cell_m;
+        if (zx_lo) u[nu.at(i, j, k)] = 0.0f;
+        if (zx_hi) u[nu.at(i + 1, j, k)] = 0.0f;
+        if (zy_lo) v[nv.at(i, j, k)] = 0.0f;
+        if (zy_hi) v[nv.at(i, j + 1, k)] = 0.0f;
+        if (zz_lo) w[nw.at(i, j, k)] = 0.0f;
+        if (zz_hi) w[nw.at(i, j, k + 1)] = 0.0f;
+    }
+}
+
+inline dim3 divergence_grid(const plm_grid_info &g) {
+    const Dims nc = cell_dims(g);
+    return dim3((nc.n2 + BX - 1) / BX, (nc.n1 + BY * DIVERGENCE_ROWS - 1) / (BY * DIVERGENCE_ROWS), nc.n0);
 }
 
 // The integer content of neighbor_term's branches (sim/pressure.py:52-69), per cell.
@@ -3642,7 +3666,7 @@ int plm_divergence(const plm_grid_info *info, const float *u, const float *v, co
 int plm_close_divergence(const plm_grid_info *info, float *u, float *v, float *w, float *divergence, void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && divergence, "plm_close_divergence: NULL array");
-    k_close_divergence<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, u, v, w, divergence);
+    k_close_divergence<<<divergence_grid(*info), block3(), 0, as_stream(stream)>>>(*info, u, v, w, divergence);
     return check_launch("plm_close_divergence");
 }
 
