@@ -167,41 +167,61 @@ __global__ __launch_bounds__(256) void k_confinement_push(plm_grid_info g, const
     push[3 * c + 2] = scale * (nx * wy - ny * wx);
 }
 
-// add_face_push_kernel, sim/vorticity.py:74-84, one thread per cell: the thread adds to the three faces on its
-// cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, to that axis's face on the
-// high boundary. Each face's value is the face kernel's, 0.5 (push[below] + push[above]) dt with the clamped
-// neighbours; the cell's own push vector is read once, contiguously, and the lower neighbours' from L2. Every
-// load is issued before the first store (loads at clamped addresses whose values a lane may not use), so they
-// are in flight together.
+// add_face_push_kernel, sim/vorticity.py:74-84, one thread per FACE_PUSH_ROWS consecutive cells along j: the
+// thread adds to the three faces on each cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell
+// along an axis, to that axis's face on the high boundary. Each face's value is the face kernel's,
+// 0.5 (push[below] + push[above]) dt with the clamped neighbours; a cell's push vector is read once,
+// contiguously (the j neighbour is the thread's own row below), the i and k neighbours' from L2. Every load is
+// issued before the first store (clamped addresses; a high face read only by the boundary cell it belongs to).
+constexpr int FACE_PUSH_ROWS = 4;
 __global__ __launch_bounds__(256) void k_add_face_push(plm_grid_info g, Tri<float> f,
                                                        const float *__restrict__ push, float dt) {
+    constexpr int R = FACE_PUSH_ROWS;
+    const Dims nc = cell_dims(g), nu = face_dims(g, 0), nv = face_dims(g, 1), nw = face_dims(g, 2);
+    const int k = blockIdx.x * BX + threadIdx.x, i = blockIdx.z;
+    const int j0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * BY + threadIdx.y) * R));
+    if (k >= nc.n2 || j0 >= nc.n1) return;
+    // push y of rows j0 - 1 .. j0 + R - 1 (clamped); each row's own x and z and its i and k neighbours'
+    float py_row[R + 1], px[R], pz[R], px_im[R], pz_km[R], u_lo[R], u_hi[R], v_lo[R], v_hi[R], w_lo[R], w_hi[R];
+#pragma unroll
+    for (int m = 0; m <= R; m++) py_row[m] = push[3 * nc.at(i, min(max(j0 - 1 + m, 0), nc.n1 - 1), k) + 1];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = min(j0 + r, nc.n1 - 1), c = nc.at(i, j, k);
+        const bool hx = i == nc.n0 - 1, hy = j == nc.n1 - 1, hz = k == nc.n2 - 1;
+        px[r] = push[3 * c];
+        pz[r] = push[3 * c + 2];
+        px_im[r] = push[3 * nc.at(max(i - 1, 0), j, k)];
+        pz_km[r] = push[3 * nc.at(i, j, max(k - 1, 0)) + 2];
+        u_lo[r] = f.p[0][nu.at(i, j, k)];
+        u_hi[r] = f.p[0][nu.at(hx ? i + 1 : i, j, k)];
+        v_lo[r] = f.p[1][nv.at(i, j, k)];
+        v_hi[r] = f.p[1][nv.at(i, hy ? j + 1 : j, k)];
+        w_lo[r] = f.p[2][nw.at(i, j, k)];
+        w_hi[r] = f.p[2][nw.at(i, j, hz ? k + 1 : k)];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = j0 + r;
+        if (j >= nc.n1) break;
+        const int cell[3] = {i, j, k};
+        // below = push at the clamped lower neighbour (the cell itself on the low boundary)
+        const float own[3] = {px[r], py_row[r + 1], pz[r]};
+        const float below[3] = {i > 0 ? px_im[r] : px[r], j > 0 ? py_row[r] : py_row[r + 1], k > 0 ? pz_km[r] : pz[r]};
+        const float low_face[3] = {u_lo[r], v_lo[r], w_lo[r]}, high_face[3] = {u_hi[r], v_hi[r], w_hi[r]};
+        float *low_at[3] = {f.p[0] + nu.at(i, j, k), f.p[1] + nv.at(i, j, k), f.p[2] + nw.at(i, j, k)};
+        float *high_at[3] = {f.p[0] + nu.at(i + 1, j, k), f.p[1] + nv.at(i, j + 1, k), f.p[2] + nw.at(i, j, k + 1)};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            *low_at[a] = low_face[a] + 0.5f * (below[a] + own[a]) * dt;
+            if (cell[a] == g.dims[a] - 1) *high_at[a] = high_face[a] + 0.5f * (own[a] + own[a]) * dt;
+        }
+    }
+}
+
+inline dim3 face_push_grid(const plm_grid_info &g) {
     const Dims nc = cell_dims(g);
-    int i, j, k;
-    if (!thread_cell(nc, i, j, k)) return;
-    const int cell[3] = {i, j, k};
-    const int stride[3] = {nc.n1 * nc.n2, nc.n2, 1};
-    const int c = nc.at(i, j, k);
-    float own[3], below[3], low_face[3], high_face[3];
-    float *low_at[3], *high_at[3];
-    bool high[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const Dims d = face_dims(g, a);
-        int hi[3] = {i, j, k};
-        hi[a] += 1;
-        high[a] = cell[a] == g.dims[a] - 1;
-        own[a] = push[3 * c + a];
-        below[a] = push[3 * (cell[a] > 0 ? c - stride[a] : c) + a];
-        low_at[a] = f.p[a] + d.at(i, j, k);
-        high_at[a] = high[a] ? f.p[a] + d.at(hi[0], hi[1], hi[2]) : low_at[a];
-        low_face[a] = *low_at[a];
-        high_face[a] = *high_at[a];
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        *low_at[a] = low_face[a] + 0.5f * (below[a] + own[a]) * dt;
-        if (high[a]) *high_at[a] = high_face[a] + 0.5f * (own[a] + own[a]) * dt;
-    }
+    return dim3((nc.n2 + BX - 1) / BX, (nc.n1 + BY * FACE_PUSH_ROWS - 1) / (BY * FACE_PUSH_ROWS), nc.n0);
 }
 
 // ------------------------------------------------------------------ noise, sim/noise.py
@@ -610,7 +630,7 @@ int plm_add_face_push(const plm_grid_info *info, float *u, float *v, float *w, c
                       void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && push, "plm_add_face_push: NULL array");
-    k_add_face_push<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    k_add_face_push<<<face_push_grid(*info), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, push, dt);
     return check_launch("plm_add_face_push");
 }
 
@@ -633,7 +653,7 @@ int plm_confinement(const plm_grid_info *info, float *u, float *v, float *w, flo
     PLM_TRY(check_launch("plm_confinement curl"));
     k_confinement_push<<<grid3(nc), block3(), 0, s>>>(*info, curl, push, strength);
     PLM_TRY(check_launch("plm_confinement push"));
-    k_add_face_push<<<grid3(nc), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, push, dt);
+    k_add_face_push<<<face_push_grid(*info), block3(), 0, s>>>(*info, Tri<float>{{u, v, w}}, push, dt);
     return check_launch("plm_confinement faces");
 }
 

@@ -1062,43 +1062,64 @@ int tile_shape_for(const Dims &n) {
 
 // ------------------------------------------------------------------ gradient subtraction
 
-// subtract_gradient_kernel, sim/pressure.py:95-116, one thread per cell: the thread updates the three faces on
-// its cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell along an axis, that axis's face on the
-// high boundary, so q is read once, coalesced (the lower neighbours from L1/L2). Each face's update is the face
-// kernel's: f -= (q_above - q_below) / cell_m, a ghost q of 0 outside an open face, closed boundary faces
-// untouched. Every load is issued before the first store (at clamped addresses whose values a lane may not
-// use), so they are in flight together.
+// subtract_gradient_kernel, sim/pressure.py:95-116, one thread per GRADIENT_ROWS consecutive cells along j: the
+// thread updates the three faces on each cell's low side (u[i,j,k], v[i,j,k], w[i,j,k]) and, on the last cell
+// along an axis, that axis's face on the high boundary, so q is read once, coalesced (the lower neighbours from
+// L1/L2, the j neighbour from the thread's own row below). Each face's update is the face kernel's:
+// f -= (q_above - q_below) / cell_m, a ghost q of 0 outside an open face, closed boundary faces untouched.
+// Every load is issued before the first store (at clamped addresses whose values a lane may not use).
+constexpr int GRADIENT_ROWS = 4;
 __global__ __launch_bounds__(256) void k_subtract_gradient(plm_grid_info g, Tri<float> f,
                                                            const float *__restrict__ q) {
+    constexpr int R = GRADIENT_ROWS;
     const Dims n = cell_dims(g);
-    int i, j, k;
-    if (!thread_cell(n, i, j, k)) return;
-    const int cell[3] = {i, j, k};
-    const int stride[3] = {n.n1 * n.n2, n.n2, 1};
-    const int c = n.at(i, j, k);
-    const float q_cell = q[c];
-    float q_lower[3], low_face[3], high_face[3];
-    float *low_at[3], *high_at[3];
-    bool low[3], high[3];
+    const int k = blockIdx.x * BX + threadIdx.x, i = blockIdx.z;
+    const int j0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.y * BY + threadIdx.y) * R));
+    if (k >= n.n2 || j0 >= n.n1) return;
+    const Dims nu = face_dims(g, 0), nv = face_dims(g, 1), nw = face_dims(g, 2);
+    // q rows j0 - 1 .. j0 + R - 1 (clamped), and the i and k neighbours of each row
+    float q_row[R + 1], q_im[R], q_km[R], u_lo[R], u_hi[R], v_lo[R], v_hi[R], w_lo[R], w_hi[R];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const Dims d = face_dims(g, a);
-        int hi[3] = {i, j, k};
-        hi[a] += 1;
-        low[a] = cell[a] > 0 || g.open_lo[a] == 1;
-        high[a] = cell[a] == g.dims[a] - 1 && g.open_hi[a] == 1;
-        q_lower[a] = q[cell[a] > 0 ? c - stride[a] : c];
-        low_at[a] = f.p[a] + d.at(i, j, k);
-        high_at[a] = high[a] ? f.p[a] + d.at(hi[0], hi[1], hi[2]) : low_at[a];
-        low_face[a] = *low_at[a];
-        high_face[a] = *high_at[a];
-    }
+    for (int m = 0; m <= R; m++) q_row[m] = q[n.at(i, min(max(j0 - 1 + m, 0), n.n1 - 1), k)];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const float q_below = cell[a] > 0 ? q_lower[a] : 0.0f;
-        if (low[a]) *low_at[a] = low_face[a] - (q_cell - q_below) / g.cell_m;
-        if (high[a]) *high_at[a] = high_face[a] - (0.0f - q_cell) / g.cell_m;
+    for (int r = 0; r < R; r++) {
+        const int j = min(j0 + r, n.n1 - 1);
+        q_im[r] = q[n.at(max(i - 1, 0), j, k)];
+        q_km[r] = q[n.at(i, j, max(k - 1, 0))];
+        // a high face is read only by the cell it belongs to (a boundary cell); others re-read their low face
+        const bool hx = i == n.n0 - 1, hy = j == n.n1 - 1, hz = k == n.n2 - 1;
+        u_lo[r] = f.p[0][nu.at(i, j, k)];
+        u_hi[r] = f.p[0][nu.at(hx ? i + 1 : i, j, k)];
+        v_lo[r] = f.p[1][nv.at(i, j, k)];
+        v_hi[r] = f.p[1][nv.at(i, hy ? j + 1 : j, k)];
+        w_lo[r] = f.p[2][nw.at(i, j, k)];
+        w_hi[r] = f.p[2][nw.at(i, j, hz ? k + 1 : k)];
     }
+    const float cell_m = g.cell_m;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = j0 + r;
+        if (j >= n.n1) break;
+        const float q_cell = q_row[r + 1];
+        const int cell[3] = {i, j, k};
+        const float lower[3] = {q_im[r], q_row[r], q_km[r]};
+        const float low_face[3] = {u_lo[r], v_lo[r], w_lo[r]}, high_face[3] = {u_hi[r], v_hi[r], w_hi[r]};
+        float *low_at[3] = {f.p[0] + nu.at(i, j, k), f.p[1] + nv.at(i, j, k), f.p[2] + nw.at(i, j, k)};
+        float *high_at[3] = {f.p[0] + nu.at(i + 1, j, k), f.p[1] + nv.at(i, j + 1, k), f.p[2] + nw.at(i, j, k + 1)};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (cell[a] > 0 || g.open_lo[a] == 1) {
+                const float q_below = cell[a] > 0 ? lower[a] : 0.0f;
+                *low_at[a] = low_face[a] - (q_cell - q_below) / cell_m;
+            }
+            if (cell[a] == g.dims[a] - 1 && g.open_hi[a] == 1) *high_at[a] = high_face[a] - (0.0f - q_cell) / cell_m;
+        }
+    }
+}
+
+inline dim3 gradient_grid(const plm_grid_info &g) {
+    const Dims n = cell_dims(g);
+    return dim3((n.n2 + BX - 1) / BX, (n.n1 + BY * GRADIENT_ROWS - 1) / (BY * GRADIENT_ROWS), n.n0);
 }
 
 // ------------------------------------------------------------------ multigrid transfers
@@ -3703,7 +3724,7 @@ int plm_rb_sweeps(const plm_grid_info *info, float *q, const float *rhs, float *
 int plm_subtract_gradient(const plm_grid_info *info, float *u, float *v, float *w, const float *q, void *stream) {
     PLM_TRY(validate_info(info));
     PLM_REQUIRE(u && v && w && q, "plm_subtract_gradient: NULL array");
-    k_subtract_gradient<<<grid3(cell_dims(*info)), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, q);
+    k_subtract_gradient<<<gradient_grid(*info), block3(), 0, as_stream(stream)>>>(*info, Tri<float>{{u, v, w}}, q);
     return check_launch("plm_subtract_gradient");
 }
 
