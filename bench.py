@@ -414,6 +414,10 @@ def main() -> int:
     parser.add_argument("--tolerance", type=float, default=1e-5)
     parser.add_argument("--top-tile", type=int, default=-1, metavar="SHAPE",
                         help="pin the sweep tile shape of the 128x192x128 level instead of autotuning it")
+    parser.add_argument("--tiles", default="", metavar="S0,S1,...",
+                        help="pin the sweep tile shapes of the levels from the top down (128x192x128, 64x96x64, ...) "
+                             "after autotuning, e.g. for a profiled run, whose per-dispatch timings the profiler "
+                             "inflates")
     parser.add_argument("--ensemble", action="store_true",
                         help="run the 32-member batched ensemble benchmark instead and exit")
     parser.add_argument("--ensemble-members", type=int, default=32)
@@ -484,6 +488,13 @@ def main() -> int:
 
         call("plm_set_sweep_tile", (ctypes.c_int32 * 3)(*GRID), args.top_tile)
         tuned[GRID] = (args.top_tile, tuned.get(GRID, (None, []))[1])
+    if args.tiles:
+        from plume_amd._lib import call
+
+        for level, shape in zip(solve.levels, (int(s) for s in args.tiles.split(","))):
+            dims = tuple(int(d) for d in level.info.dims)
+            call("plm_set_sweep_tile", (ctypes.c_int32 * 3)(*dims), shape)
+            tuned[dims] = (shape, tuned.get(dims, tuned.get(level.cells, (None, [])))[1])
     print(f"[bench] sweep tiles: {tuned}", file=sys.stderr, flush=True)
     step_index = 0
 
