@@ -40,7 +40,7 @@ from plume_amd.recipe.derived import derive
 from plume_amd.recipe.load import fill_dependent_defaults, parse_recipe
 from plume_amd.sim.grid import make_grid
 from plume_amd.sim.multigrid import BatchedMultigrid, Multigrid
-from plume_amd.sim.pressure import compute_divergence, subtract_gradient, zero_closed_faces
+from plume_amd.sim.pressure import close_faces_and_divergence, subtract_gradient
 from plume_amd.sim.scenes import puff_table
 from plume_amd.sim.step_cap import StepCap
 from plume_amd.sim.stepper import Stepper
@@ -242,8 +242,7 @@ class Ensemble:
                         with torch.cuda.stream(self.streams[self.stream_of[index]]):
                             if index in self.batched:
                                 m.stepper.before_projection(t, dt)
-                                zero_closed_faces(m.grid)
-                                compute_divergence(m.grid)
+                                close_faces_and_divergence(m.grid)
                             else:
                                 m.stepper.step(t, dt)
                                 note(index)
