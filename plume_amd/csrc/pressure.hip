@@ -2124,9 +2124,9 @@ int launch_sweeps(const LevelArgs &L, float *q, const float *rhs, float *scratch
 
 // Times every tile shape of the plain two-colour sweep on one level and pins the fastest. Each launch carries
 // its own begin and end timestamps (events bound to the dispatch through hipExtLaunchKernelGGL), so a shape is
-// judged by its kernel time, not by the gaps between back-to-back launches; the shapes take turns round by
-// round (drift in clocks or caches touches all of them alike), and each one's figure is the median of its
-// 2 x repeats launches after one untimed round.
+// judged by its kernel time, not by the gaps between back-to-back launches; the shapes take turns, one launch
+// each per pass (drift in clocks or caches touches all of them alike), and each one's figure is the median of
+// its 2 x repeats launches after one untimed pass.
 template <bool UNIT>
 int autotune_sweep(const LevelArgs &L, float *q, const float *rhs, float *scratch, int repeats, float *times_us,
                    int32_t *chosen, hipStream_t s) {
