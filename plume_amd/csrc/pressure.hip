@@ -1322,6 +1322,167 @@ __global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L,
 
 inline dim3 children_grid(const Dims &nc) { return dim3((unsigned)((nc.count() + 31) / 32)); }
 
+// ------------------------------------------------------------------ last pre-smoothing sweep + residual + restrict
+
+// The last pre-smoothing sweep of a flat level fused with its residual and restriction (red_black_sweep_kernel
+// for both colours, residual_kernel and restrict_kernel; sim/pressure.py:80-92, sim/multigrid.py:36-60, 137-143)
+// for the levels between the simulation grid and the LDS tail, whose launches cost their latency more than their
+// bytes: one launch where a sweep and a restriction took two. A block holds a TI x TJ x TK tile (even
+// extents at even origins, so the tile's cells are whole 2 x 2 x 2 children blocks) with a 3-cell halo of q and
+// rhs in LDS, +0.0 outside the level. Red runs on the tile and two rings, black on the tile and one ring, so the
+// tile and its first ring end with their swept values; the tile's residuals go to LDS and each coarse cell sums
+// its children in the reference's order (x major). On a flat level every coupling between two cells and every
+// volume is 1 and an open face couples by the level's ghost value: the update and the residual are the tile
+// sweep's and k_residual_restrict_children's operations (an outside neighbour adds the +0.0 the reference adds
+// for a ghost or a closed face; the division by the sum of couplings is IEEE, which divide_flat equals), so the
+// swept q and the coarse rhs are the same bits as the two launches'.
+constexpr int SR_H = 3;
+
+template <int TI, int TJ, int TK, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sweep_restrict(LevelArgs L, const float *__restrict__ q_in,
+                                                               float *__restrict__ q_out,
+                                                               const float *__restrict__ rhs,
+                                                               float *__restrict__ coarse_rhs, Dims nc,
+                                                               float relaxation, const double *gate, FastDiv tiles_k,
+                                                               FastDiv tiles_j) {
+    static_assert(TI % 2 == 0 && TJ % 2 == 0 && TK % 2 == 0, "tiles hold whole 2 x 2 x 2 children blocks");
+    constexpr int SR_QI = TI + 2 * SR_H, SR_QJ = TJ + 2 * SR_H, SR_QK = TK + 2 * SR_H;
+    constexpr int SR_CELLS = SR_QI * SR_QJ * SR_QK, SR_LOADS = (SR_CELLS + THREADS - 1) / THREADS;
+    __shared__ float sq[SR_CELLS], sr[SR_CELLS], sres[TI * TJ * TK];
+    const Dims n = cell_dims(L.g);
+    const int tile = (int)blockIdx.x, tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d;
+    const int ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
+    const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * TK;
+    const int tid = (int)threadIdx.x;
+    // the tile and its 3-cell halo, q and rhs, +0.0 outside the level; the gate is read once the loads are in
+    // flight (uniform: the block leaves before any barrier)
+    float vq[SR_LOADS], vr[SR_LOADS];
+#pragma unroll
+    for (int u = 0; u < SR_LOADS; u++) {
+        const int idx = min(tid + u * THREADS, SR_CELLS - 1);
+        const int a = idx / (SR_QJ * SR_QK), b = (idx / SR_QK) % SR_QJ, c = idx % SR_QK;
+        const int i = i0 - SR_H + a, j = j0 - SR_H + b, k = k0 - SR_H + c;
+        const bool inside = (unsigned)i < (unsigned)n.n0 && (unsigned)j < (unsigned)n.n1 && (unsigned)k < (unsigned)n.n2;
+        const int at = inside ? n.at(i, j, k) : 0;
+        vq[u] = inside ? q_in[at] : 0.0f;
+        vr[u] = inside ? rhs[at] : 0.0f;
+    }
+    if (gated_off(gate)) return;
+#pragma unroll
+    for (int u = 0; u < SR_LOADS; u++) {
+        const int idx = tid + u * THREADS;
+        if (idx < SR_CELLS) {
+            sq[idx] = vq[u];
+            sr[idx] = vr[u];
+        }
+    }
+    __syncthreads();
+    const float cm2 = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m: every volume is 1
+    const float ghost = uniform(L.ghost);
+    bool open_lo[3], open_hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        open_lo[a] = uniform(L.g.open_lo[a]) == 1;
+        open_hi[a] = uniform(L.g.open_hi[a]) == 1;
+    }
+    constexpr int SI = SR_QJ * SR_QK, SJ = SR_QK;
+    // neighbor_sum's t0 (the six neighbours in the reference's order) and t1 (their couplings) at LDS index p
+    auto sums = [&](int i, int j, int k, int p, float &t0, float &t1) {
+        t0 = 0.0f + sq[p - SI];
+        t0 = t0 + sq[p + SI];
+        t0 = t0 + sq[p - SJ];
+        t0 = t0 + sq[p + SJ];
+        t0 = t0 + sq[p - 1];
+        t0 = t0 + sq[p + 1];
+        const int cell[3] = {i, j, k}, extent[3] = {n.n0, n.n1, n.n2};
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            sum = sum + (cell[a] > 0 ? 1.0f : open_lo[a] ? ghost : 0.0f);
+            sum = sum + (cell[a] < extent[a] - 1 ? 1.0f : open_hi[a] ? ghost : 0.0f);
+        }
+        t1 = sum;
+    };
+    // one colour over the tile and `ring` rings: (TI + 2 ring) (TJ + 2 ring) rows of (TK + 2 ring) / 2 cells
+    auto pass = [&](int color, int ring) {
+        const int ri = TI + 2 * ring, rj = TJ + 2 * ring, half = (TK + 2 * ring) / 2;
+        const int total = ri * rj * half;
+        for (int x = tid; x < total; x += THREADS) {
+            const int row = x / half, slot = x - row * half;
+            const int a = row / rj, b = row - a * rj;
+            const int i = i0 - ring + a, j = j0 - ring + b, kb = k0 - ring;
+            const int k = kb + 2 * slot + ((color ^ (i + j + kb)) & 1);
+            if ((unsigned)i >= (unsigned)n.n0 || (unsigned)j >= (unsigned)n.n1 || (unsigned)k >= (unsigned)n.n2) continue;
+            const int p = ((i - i0 + SR_H) * SR_QJ + (j - j0 + SR_H)) * SR_QK + (k - k0 + SR_H);
+            float t0, t1;
+            sums(i, j, k, p, t0, t1);
+            if (t1 == 0.0f) continue;
+            const float q_c = sq[p];
+            const float target = (t0 - cm2 * sr[p]) / t1;
+            sq[p] = q_c + relaxation * (target - q_c);
+        }
+    };
+    pass(0, 2);
+    __syncthreads();
+    pass(1, 1);
+    __syncthreads();
+    // the tile's residuals (residual_value with volume 1) and its swept q
+    for (int x = tid; x < TI * TJ * TK; x += THREADS) {
+        const int a = x / (TJ * TK), b = (x / TK) % TJ, c = x % TK;
+        const int i = i0 + a, j = j0 + b, k = k0 + c;
+        if (i >= n.n0 || j >= n.n1 || k >= n.n2) continue;
+        const int p = ((a + SR_H) * SR_QJ + (b + SR_H)) * SR_QK + (c + SR_H);
+        float t0, t1;
+        sums(i, j, k, p, t0, t1);
+        const float q_c = sq[p];
+        sres[x] = sr[p] - (t0 - t1 * q_c) / cm2;
+        q_out[n.at(i, j, k)] = q_c;
+    }
+    __syncthreads();
+    // restrict_kernel: each coarse cell of the tile, its children added in x-major order, overhang skipped
+    constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
+    for (int x = tid; x < CI * CJ * CK; x += THREADS) {
+        const int a = x / (CJ * CK), b = (x / CK) % CJ, c = x % CK;
+        const int ci = i0 / 2 + a, cj = j0 / 2 + b, ck = k0 / 2 + c;
+        if (ci >= nc.n0 || cj >= nc.n1 || ck >= nc.n2) continue;
+        float total = 0.0f;
+        int count = 0;
+#pragma unroll
+        for (int ch = 0; ch < 8; ch++) {
+            const int di = ch >> 2, dj = (ch >> 1) & 1, dk = ch & 1;
+            if (2 * ci + di < n.n0 && 2 * cj + dj < n.n1 && 2 * ck + dk < n.n2) {
+                total = total + sres[((2 * a + di) * TJ + (2 * b + dj)) * TK + (2 * c + dk)];
+                count += 1;
+            }
+        }
+        coarse_rhs[nc.at(ci, cj, ck)] = total / (float)count;
+    }
+}
+
+template <int TI, int TJ, int TK, int THREADS>
+void launch_sweep_restrict_shape(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs,
+                                 float *coarse_rhs, Dims nc, float relaxation, const double *gate, hipStream_t s) {
+    const Dims n = cell_dims(L.g);
+    const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + TJ - 1) / TJ, tiles_i = (n.n0 + TI - 1) / TI;
+    k_sweep_restrict<TI, TJ, TK, THREADS><<<dim3((unsigned)(tiles_k * tiles_j * tiles_i)), THREADS, 0, s>>>(
+        L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, fast_div(tiles_k), fast_div(tiles_j));
+}
+
+// 1024 threads a tile: each pass is one or two updates a thread, so a block's chain of passes stays short. Big
+// levels take 4 x 8 x 32 tiles (fewer halo loads and ring updates per cell), small ones 2 x 4 x 32 (enough
+// blocks); measured per launch: 8.1 us vs 13.6 us at 64 x 96 x 64, 4.7 us at 32 x 48 x 32 (256 threads: 11.1
+// and 7.4 us).
+int launch_sweep_restrict(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs, float *coarse_rhs,
+                          Dims nc, float relaxation, const double *gate, hipStream_t s) {
+    const Dims n = cell_dims(L.g);
+    const int big_tiles = ((n.n0 + 3) / 4) * ((n.n1 + 7) / 8) * ((n.n2 + 31) / 32);
+    if (big_tiles >= 128)
+        launch_sweep_restrict_shape<4, 8, 32, 1024>(L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, s);
+    else
+        launch_sweep_restrict_shape<2, 4, 32, 1024>(L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, s);
+    return check_launch("sweep restrict");
+}
+
 // prolong_add_kernel, sim/multigrid.py:63-68
 __global__ __launch_bounds__(256) void k_prolong_add(const float *__restrict__ coarse, Dims nc, float *__restrict__ fine,
                                                      Dims nf, const double *gate) {
@@ -2203,6 +2364,14 @@ int allow_tail_lds(size_t bytes) {
     return 0;
 }
 
+// A flat level below the simulation grid with an even number (at least 2) of pre-smoothing sweeps and a level
+// of its own below it fuses its last pre-smoothing sweep with the residual and restriction
+bool fuses_restrict(const Hierarchy &h, int index) {
+    const LevelArgs &L = h.level[index];
+    return !L.unit && L.flat == 1 && h.sweeps >= 2 && h.sweeps % 2 == 0 && h.residual[index] != nullptr &&
+           index + 1 < h.n_levels;
+}
+
 // _v_cycle, sim/multigrid.py:130-147, from level `index`; `cleared`: this level's q is the reference's
 // freshly zeroed coarse q (the caller did not write it)
 int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
@@ -2222,11 +2391,27 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
         return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], max(20, 4 * longest), 1.5f, start,
                              nullptr, none, h.gate, s);
     }
-    // residual[index] doubles as the ping-pong buffer of this level's sweeps
+    const Dims nc = cell_dims(h.level[index + 1].g);
+    // residual[index] doubles as the ping-pong buffer of this level's sweeps; the coarse q is cleared by the
+    // next level's first sweep (or the tail), so it is not written here
+    if (fuses_restrict(h, index)) {
+        // an even count: the sweeps before the last ping-pong q -> residual -> ... and end in residual, from
+        // which k_sweep_restrict's sweep writes q and the coarse rhs
+        float *q = h.q[index], *scratch = h.residual[index];
+        for (int sw = 0; sw + 1 < h.sweeps; sw++) {
+            const bool to_scratch = sw % 2 == 0;
+            const float *src = to_scratch ? q : scratch, *no_coarse = nullptr;
+            float *dst = to_scratch ? scratch : q;
+            PLM_TRY(launch_tile<false>(L, sw == 0 ? start : FROM_Q, src, dst, h.rhs[index], no_coarse, none, 1.0f,
+                                       h.gate, s));
+        }
+        PLM_TRY(launch_sweep_restrict(L, scratch, q, h.rhs[index], h.rhs[index + 1], nc, 1.0f, h.gate, s));
+        PLM_TRY(v_cycle(h, index + 1, true, s));
+        return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, FROM_PROLONG,
+                             h.q[index + 1], nc, h.gate, s);
+    }
     PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
                           h.gate, s));
-    const Dims nc = cell_dims(h.level[index + 1].g);
-    // the coarse q is cleared by the next level's first sweep (or the tail), so it is not written here
     if (L.unit)
         k_residual_restrict<true, float><<<grid3(nc), block3(), 0, s>>>(L, h.q[index], h.rhs[index], h.rhs[index + 1],
                                                                 nullptr, nc, h.gate);

@@ -550,6 +550,36 @@ def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
     assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
+@pytest.mark.parametrize("cells, faces", [((96, 128, 96), ALL_OPEN), ((96, 128, 96), ALL_CLOSED),
+                                          ((100, 72, 88), DEFAULT_FACES), ((100, 72, 88), ALL_OPEN)],
+                         ids=["96x128x96-open", "96x128x96-closed", "overhang-default", "overhang-open"])
+def test_fused_last_sweep_and_restriction_is_bit_exact(device, oracle32, cells, faces):
+    """Flat levels between the simulation grid and the LDS tail run their last pre-smoothing sweep, residual and
+    restriction as one launch, in 4 x 8 x 32 tiles on a level with at least 128 of them and 2 x 4 x 32 tiles
+    below that. 96 x 128 x 96 has one level of each (48 x 64 x 48, 24 x 32 x 24); 100 x 72 x 88 has a big-tile
+    level whose tiles overhang its end on every axis (50 x 36 x 44) and a level of odd extent (25 x 18 x 22)
+    whose last coarse cells have fewer than eight children."""
+    from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
+    from plume_amd.sim.multigrid import Multigrid, PLM_LDS_CELLS, build_levels
+    from plume_amd.sim.pressure import project
+
+    gpu, orc = pair(device, oracle32, cells, faces)
+    levels = build_levels(gpu)
+    tail, fused = 0, []
+    for index in reversed(range(len(levels))):
+        tail += int(np.prod(levels[index].cells))
+        flat = all(f == 1.0 for f in levels[index].info.fill_hi)
+        if tail > PLM_LDS_CELLS and 0 < index < len(levels) - 1 and flat:
+            n = levels[index].cells
+            fused.append(((n[0] + 3) // 4) * ((n[1] + 7) // 8) * ((n[2] + 31) // 32) >= 128)
+    assert sorted(fused) == [False, True], fused
+    project(gpu, Multigrid(gpu, 2, 2))
+    oracle_project(orc, OracleMultigrid(orc, 2, 2))
+    assert bit_equal(to_np(gpu.pressure), orc.pressure)
+    for name in ("u", "v", "w"):
+        assert bit_equal(to_np(getattr(gpu, name).data), getattr(orc, name).data), name
+
+
 def test_tolerance_mode_reaches_the_target_and_matches_fixed_cycles(device, oracle32):
     from oracle.sim import Multigrid as OracleMultigrid, compute_divergence, poisson_residual
     from plume_amd.sim.multigrid import Multigrid, relative_residual
