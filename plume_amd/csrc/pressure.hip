@@ -1322,61 +1322,119 @@ __global__ __launch_bounds__(256) void k_residual_restrict_children(LevelArgs L,
 
 inline dim3 children_grid(const Dims &nc) { return dim3((unsigned)((nc.count() + 31) / 32)); }
 
-// ------------------------------------------------------------------ last pre-smoothing sweep + residual + restrict
+// ------------------------------------------------------------------ smoothing of a flat mid level in LDS
 
-// The last pre-smoothing sweep of a flat level fused with its residual and restriction (red_black_sweep_kernel
-// for both colours, residual_kernel and restrict_kernel; sim/pressure.py:80-92, sim/multigrid.py:36-60, 137-143)
-// for the levels between the simulation grid and the LDS tail, whose launches cost their latency more than their
-// bytes: one launch where a sweep and a restriction took two. A block holds a TI x TJ x TK tile (even
-// extents at even origins, so the tile's cells are whole 2 x 2 x 2 children blocks) with a 3-cell halo of q and
-// rhs in LDS, +0.0 outside the level. Red runs on the tile and two rings, black on the tile and one ring, so the
-// tile and its first ring end with their swept values; the tile's residuals go to LDS and each coarse cell sums
-// its children in the reference's order (x major). On a flat level every coupling between two cells and every
-// volume is 1 and an open face couples by the level's ghost value: the update and the residual are the tile
-// sweep's and k_residual_restrict_children's operations (an outside neighbour adds the +0.0 the reference adds
-// for a ghost or a closed face; the division by the sum of couplings is IEEE, which divide_flat equals), so the
-// swept q and the coarse rhs are the same bits as the two launches'.
-constexpr int SR_H = 3;
-
-template <int TI, int TJ, int TK, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_sweep_restrict(LevelArgs L, const float *__restrict__ q_in,
-                                                               float *__restrict__ q_out,
-                                                               const float *__restrict__ rhs,
-                                                               float *__restrict__ coarse_rhs, Dims nc,
-                                                               float relaxation, const double *gate, FastDiv tiles_k,
-                                                               FastDiv tiles_j) {
+// Sweeps of a flat level between the simulation grid and the LDS tail (red_black_sweep_kernel for both colours,
+// sim/pressure.py:80-92), whose launches cost their latency more than their bytes, run in LDS: a block holds a
+// TI x TJ x TK tile (even extents at even origins, so the tile's cells are whole 2 x 2 x 2 children blocks) with
+// an H-cell halo of q and rhs, +0.0 outside the level, H = 2 SWEEPS (+ 1 with RESTRICT). Sweep s (from 0) runs
+// red on the tile and its rings up to H - 1 - 2s, then black up to H - 2 - 2s, so each pass reads only values
+// that are already those of the whole-level sweeps, and the tile (with RESTRICT also its first ring) ends with
+// the values of SWEEPS sweeps. Before the sweeps: FROM_Q loads q_in; FROM_ZERO starts from q = 0 and loads no q;
+// FROM_PROLONG adds prolong_add_kernel's correction (sim/multigrid.py:63-68) to q_in, from a box of coarse cells
+// staged in LDS with prolong_at_row's taps, weights and order. RESTRICT: the tile's residuals (residual_kernel,
+// sim/multigrid.py:36-43) and their restriction (restrict_kernel, sim/multigrid.py:46-60: each coarse cell's
+// children summed in x-major order, the overhang skipped) to the coarse rhs. On a flat level every coupling
+// between two cells and every volume is 1 and an open face couples by the level's ghost value: the update and
+// the residual are the tile sweep's and k_residual_restrict_children's operations (an outside neighbour adds
+// the +0.0 the reference adds for a ghost or a closed face; the division by the sum of couplings is IEEE, which
+// divide_flat equals), so q and the coarse rhs are the same bits as the separate launches'.
+template <int TI, int TJ, int TK, int SWEEPS, int START, bool RESTRICT>
+struct SmoothBox {
     static_assert(TI % 2 == 0 && TJ % 2 == 0 && TK % 2 == 0, "tiles hold whole 2 x 2 x 2 children blocks");
-    constexpr int SR_QI = TI + 2 * SR_H, SR_QJ = TJ + 2 * SR_H, SR_QK = TK + 2 * SR_H;
-    constexpr int SR_CELLS = SR_QI * SR_QJ * SR_QK, SR_LOADS = (SR_CELLS + THREADS - 1) / THREADS;
-    __shared__ float sq[SR_CELLS], sr[SR_CELLS], sres[TI * TJ * TK];
+    static constexpr int H = 2 * SWEEPS + (RESTRICT ? 1 : 0);
+    static constexpr int QI = TI + 2 * H, QJ = TJ + 2 * H, QK = TK + 2 * H, CELLS = QI * QJ * QK;
+    // the coarse taps of the box's fine cells lie in QI / 2 + 2 coarse cells from the box's lowest one, and so on
+    static constexpr int CI = QI / 2 + 2, CJ = QJ / 2 + 2, CK = QK / 2 + 2;
+    static constexpr int COARSE = START == FROM_PROLONG ? CI * CJ * CK : 0;
+    static constexpr int RES = RESTRICT ? TI * TJ * TK : 0;
+    static constexpr size_t BYTES = sizeof(float) * (size_t)(2 * CELLS + COARSE + RES);
+};
+
+// the lowest coarse tap of a fine index f >= 1 is (f - 1) / 2, of f = 0 it is 0; the box's lowest tap
+__device__ __forceinline__ int coarse_origin(int f0) { return max(0, (f0 - 1) >> 1); }
+
+template <int TI, int TJ, int TK, int THREADS, int SWEEPS, int START, bool RESTRICT>
+__global__ __launch_bounds__(THREADS) void k_level_smooth(LevelArgs L, const float *__restrict__ q_in,
+                                                          float *__restrict__ q_out, const float *__restrict__ rhs,
+                                                          const float *__restrict__ coarse, Dims nq,
+                                                          float *__restrict__ coarse_rhs, Dims nc, float relaxation,
+                                                          const double *gate, FastDiv tiles_k, FastDiv tiles_j) {
+    using Box = SmoothBox<TI, TJ, TK, SWEEPS, START, RESTRICT>;
+    constexpr int H = Box::H, QJ = Box::QJ, QK = Box::QK, CELLS = Box::CELLS;
+    constexpr int CJ = Box::CJ, CK = Box::CK, COARSE = Box::COARSE;
+    constexpr int LOADS = (CELLS + THREADS - 1) / THREADS, COARSE_LOADS = (COARSE + THREADS - 1) / THREADS;
+    extern __shared__ float lds[];
+    float *sq = lds, *sr = lds + CELLS, *sc = sr + CELLS, *sres = sc + COARSE;
     const Dims n = cell_dims(L.g);
     const int tile = (int)blockIdx.x, tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d;
     const int ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
     const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * TK;
     const int tid = (int)threadIdx.x;
-    // the tile and its 3-cell halo, q and rhs, +0.0 outside the level; the gate is read once the loads are in
-    // flight (uniform: the block leaves before any barrier)
-    float vq[SR_LOADS], vr[SR_LOADS];
+    auto inside = [&](int i, int j, int k) {
+        return (unsigned)i < (unsigned)n.n0 && (unsigned)j < (unsigned)n.n1 && (unsigned)k < (unsigned)n.n2;
+    };
+    // the box's q and rhs (the outermost ring's rhs is never used) and the coarse box, all in flight before the
+    // gate is read (uniform: the block leaves before any barrier)
+    float vq[LOADS], vr[LOADS];
 #pragma unroll
-    for (int u = 0; u < SR_LOADS; u++) {
-        const int idx = min(tid + u * THREADS, SR_CELLS - 1);
-        const int a = idx / (SR_QJ * SR_QK), b = (idx / SR_QK) % SR_QJ, c = idx % SR_QK;
-        const int i = i0 - SR_H + a, j = j0 - SR_H + b, k = k0 - SR_H + c;
-        const bool inside = (unsigned)i < (unsigned)n.n0 && (unsigned)j < (unsigned)n.n1 && (unsigned)k < (unsigned)n.n2;
-        const int at = inside ? n.at(i, j, k) : 0;
-        vq[u] = inside ? q_in[at] : 0.0f;
-        vr[u] = inside ? rhs[at] : 0.0f;
+    for (int u = 0; u < LOADS; u++) {
+        const int idx = min(tid + u * THREADS, CELLS - 1);
+        const int a = idx / (QJ * QK), b = (idx / QK) % QJ, c = idx % QK;
+        const int i = i0 - H + a, j = j0 - H + b, k = k0 - H + c;
+        const bool in = inside(i, j, k);
+        const bool ring = a > 0 && a < Box::QI - 1 && b > 0 && b < QJ - 1 && c > 0 && c < QK - 1;
+        const int at = in ? n.at(i, j, k) : 0;
+        vq[u] = START != FROM_ZERO && in ? q_in[at] : 0.0f;
+        vr[u] = in && ring ? rhs[at] : 0.0f;
+    }
+    const int ci0 = coarse_origin(i0 - H), cj0 = coarse_origin(j0 - H), ck0 = coarse_origin(k0 - H);
+    float vc[COARSE_LOADS > 0 ? COARSE_LOADS : 1];
+#pragma unroll
+    for (int u = 0; u < COARSE_LOADS; u++) {
+        const int idx = min(tid + u * THREADS, COARSE - 1);
+        const int ci = ci0 + idx / (CJ * CK), cj = cj0 + (idx / CK) % CJ, ck = ck0 + idx % CK;
+        const bool in = ci < nq.n0 && cj < nq.n1 && ck < nq.n2;
+        vc[u] = in ? coarse[nq.at(ci, cj, ck)] : 0.0f;
     }
     if (gated_off(gate)) return;
 #pragma unroll
-    for (int u = 0; u < SR_LOADS; u++) {
+    for (int u = 0; u < LOADS; u++) {
         const int idx = tid + u * THREADS;
-        if (idx < SR_CELLS) {
+        if (idx < CELLS) {
             sq[idx] = vq[u];
             sr[idx] = vr[u];
         }
     }
+#pragma unroll
+    for (int u = 0; u < COARSE_LOADS; u++) {
+        const int idx = tid + u * THREADS;
+        if (idx < COARSE) sc[idx] = vc[u];
+    }
     __syncthreads();
+    if constexpr (START == FROM_PROLONG) {
+        // q + the coarse correction at each box cell inside the level (prolong_at_row's taps and order)
+        for (int idx = tid; idx < CELLS; idx += THREADS) {
+            const int a = idx / (QJ * QK), b = (idx / QK) % QJ, c = idx % QK;
+            const int i = i0 - H + a, j = j0 - H + b, k = k0 - H + c;
+            if (!inside(i, j, k)) continue;
+            const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nq.n0 - 1));
+            const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nq.n1 - 1));
+            const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(nq.n2 - 1));
+            const int ic0 = (int)x, jc0 = (int)y, kc0 = (int)z;
+            const int ic1 = min(ic0 + 1, nq.n0 - 1), jc1 = min(jc0 + 1, nq.n1 - 1), kc1 = min(kc0 + 1, nq.n2 - 1);
+            const float tx = x - (float)ic0, ty = y - (float)jc0, tz = z - (float)kc0;
+            const int r00 = ((ic0 - ci0) * CJ + (jc0 - cj0)) * CK, r10 = ((ic1 - ci0) * CJ + (jc0 - cj0)) * CK;
+            const int r01 = ((ic0 - ci0) * CJ + (jc1 - cj0)) * CK, r11 = ((ic1 - ci0) * CJ + (jc1 - cj0)) * CK;
+            const int p0 = kc0 - ck0, p1 = kc1 - ck0;
+            const float c00 = lerp_(sc[r00 + p0], sc[r10 + p0], tx);
+            const float c10 = lerp_(sc[r01 + p0], sc[r11 + p0], tx);
+            const float c01 = lerp_(sc[r00 + p1], sc[r10 + p1], tx);
+            const float c11 = lerp_(sc[r01 + p1], sc[r11 + p1], tx);
+            sq[idx] = sq[idx] + lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+        }
+        __syncthreads();
+    }
     const float cm2 = uniform(L.g.cell_m) * uniform(L.g.cell_m);  // (1 cell_m) cell_m: every volume is 1
     const float ghost = uniform(L.ghost);
     bool open_lo[3], open_hi[3];
@@ -1385,7 +1443,7 @@ __global__ __launch_bounds__(THREADS) void k_sweep_restrict(LevelArgs L, const f
         open_lo[a] = uniform(L.g.open_lo[a]) == 1;
         open_hi[a] = uniform(L.g.open_hi[a]) == 1;
     }
-    constexpr int SI = SR_QJ * SR_QK, SJ = SR_QK;
+    constexpr int SI = QJ * QK, SJ = QK;
     // neighbor_sum's t0 (the six neighbours in the reference's order) and t1 (their couplings) at LDS index p
     auto sums = [&](int i, int j, int k, int p, float &t0, float &t1) {
         t0 = 0.0f + sq[p - SI];
@@ -1412,8 +1470,8 @@ __global__ __launch_bounds__(THREADS) void k_sweep_restrict(LevelArgs L, const f
             const int a = row / rj, b = row - a * rj;
             const int i = i0 - ring + a, j = j0 - ring + b, kb = k0 - ring;
             const int k = kb + 2 * slot + ((color ^ (i + j + kb)) & 1);
-            if ((unsigned)i >= (unsigned)n.n0 || (unsigned)j >= (unsigned)n.n1 || (unsigned)k >= (unsigned)n.n2) continue;
-            const int p = ((i - i0 + SR_H) * SR_QJ + (j - j0 + SR_H)) * SR_QK + (k - k0 + SR_H);
+            if (!inside(i, j, k)) continue;
+            const int p = ((i - i0 + H) * QJ + (j - j0 + H)) * QK + (k - k0 + H);
             float t0, t1;
             sums(i, j, k, p, t0, t1);
             if (t1 == 0.0f) continue;
@@ -1422,65 +1480,85 @@ __global__ __launch_bounds__(THREADS) void k_sweep_restrict(LevelArgs L, const f
             sq[p] = q_c + relaxation * (target - q_c);
         }
     };
-    pass(0, 2);
-    __syncthreads();
-    pass(1, 1);
-    __syncthreads();
-    // the tile's residuals (residual_value with volume 1) and its swept q
+#pragma unroll
+    for (int sweep = 0; sweep < SWEEPS; sweep++) {
+        pass(0, H - 1 - 2 * sweep);
+        __syncthreads();
+        pass(1, H - 2 - 2 * sweep);
+        __syncthreads();
+    }
+    // the tile's q, and with RESTRICT its residuals (residual_value with volume 1)
     for (int x = tid; x < TI * TJ * TK; x += THREADS) {
         const int a = x / (TJ * TK), b = (x / TK) % TJ, c = x % TK;
         const int i = i0 + a, j = j0 + b, k = k0 + c;
-        if (i >= n.n0 || j >= n.n1 || k >= n.n2) continue;
-        const int p = ((a + SR_H) * SR_QJ + (b + SR_H)) * SR_QK + (c + SR_H);
-        float t0, t1;
-        sums(i, j, k, p, t0, t1);
+        if (!inside(i, j, k)) continue;
+        const int p = ((a + H) * QJ + (b + H)) * QK + (c + H);
         const float q_c = sq[p];
-        sres[x] = sr[p] - (t0 - t1 * q_c) / cm2;
+        if constexpr (RESTRICT) {
+            float t0, t1;
+            sums(i, j, k, p, t0, t1);
+            sres[x] = sr[p] - (t0 - t1 * q_c) / cm2;
+        }
         q_out[n.at(i, j, k)] = q_c;
     }
-    __syncthreads();
-    // restrict_kernel: each coarse cell of the tile, its children added in x-major order, overhang skipped
-    constexpr int CI = TI / 2, CJ = TJ / 2, CK = TK / 2;
-    for (int x = tid; x < CI * CJ * CK; x += THREADS) {
-        const int a = x / (CJ * CK), b = (x / CK) % CJ, c = x % CK;
-        const int ci = i0 / 2 + a, cj = j0 / 2 + b, ck = k0 / 2 + c;
-        if (ci >= nc.n0 || cj >= nc.n1 || ck >= nc.n2) continue;
-        float total = 0.0f;
-        int count = 0;
+    if constexpr (RESTRICT) {
+        __syncthreads();
+        constexpr int RI = TI / 2, RJ = TJ / 2, RK = TK / 2;
+        for (int x = tid; x < RI * RJ * RK; x += THREADS) {
+            const int a = x / (RJ * RK), b = (x / RK) % RJ, c = x % RK;
+            const int ci = i0 / 2 + a, cj = j0 / 2 + b, ck = k0 / 2 + c;
+            if (ci >= nc.n0 || cj >= nc.n1 || ck >= nc.n2) continue;
+            float total = 0.0f;
+            int count = 0;
 #pragma unroll
-        for (int ch = 0; ch < 8; ch++) {
-            const int di = ch >> 2, dj = (ch >> 1) & 1, dk = ch & 1;
-            if (2 * ci + di < n.n0 && 2 * cj + dj < n.n1 && 2 * ck + dk < n.n2) {
-                total = total + sres[((2 * a + di) * TJ + (2 * b + dj)) * TK + (2 * c + dk)];
-                count += 1;
+            for (int ch = 0; ch < 8; ch++) {
+                const int di = ch >> 2, dj = (ch >> 1) & 1, dk = ch & 1;
+                if (2 * ci + di < n.n0 && 2 * cj + dj < n.n1 && 2 * ck + dk < n.n2) {
+                    total = total + sres[((2 * a + di) * TJ + (2 * b + dj)) * TK + (2 * c + dk)];
+                    count += 1;
+                }
             }
+            coarse_rhs[nc.at(ci, cj, ck)] = total / (float)count;
         }
-        coarse_rhs[nc.at(ci, cj, ck)] = total / (float)count;
     }
 }
 
-template <int TI, int TJ, int TK, int THREADS>
-void launch_sweep_restrict_shape(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs,
-                                 float *coarse_rhs, Dims nc, float relaxation, const double *gate, hipStream_t s) {
+template <int TI, int TJ, int TK, int THREADS, int SWEEPS, int START, bool RESTRICT>
+int launch_level_smooth_shape(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs,
+                              const float *coarse, Dims nq, float *coarse_rhs, Dims nc, float relaxation,
+                              const double *gate, hipStream_t s) {
+    constexpr size_t bytes = SmoothBox<TI, TJ, TK, SWEEPS, START, RESTRICT>::BYTES;
+    static_assert(bytes <= 160 * 1024, "a block's LDS fits one CU");
+    const auto kernel = &k_level_smooth<TI, TJ, TK, THREADS, SWEEPS, START, RESTRICT>;
+    if (bytes > 65536) {
+        static bool allowed = false;  // more than the default 64 KB of dynamic LDS, granted once per kernel
+        if (!allowed) {
+            if (hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+                return fail(PLM_EINVAL, "level smoothing: %zu bytes of LDS refused", bytes);
+            allowed = true;
+        }
+    }
     const Dims n = cell_dims(L.g);
     const int tiles_k = (n.n2 + TK - 1) / TK, tiles_j = (n.n1 + TJ - 1) / TJ, tiles_i = (n.n0 + TI - 1) / TI;
-    k_sweep_restrict<TI, TJ, TK, THREADS><<<dim3((unsigned)(tiles_k * tiles_j * tiles_i)), THREADS, 0, s>>>(
-        L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, fast_div(tiles_k), fast_div(tiles_j));
+    kernel<<<dim3((unsigned)(tiles_k * tiles_j * tiles_i)), THREADS, bytes, s>>>(
+        L, q_in, q_out, rhs, coarse, nq, coarse_rhs, nc, relaxation, gate, fast_div(tiles_k), fast_div(tiles_j));
+    return check_launch("level smoothing");
 }
 
 // 1024 threads a tile: each pass is one or two updates a thread, so a block's chain of passes stays short. Big
 // levels take 4 x 8 x 32 tiles (fewer halo loads and ring updates per cell), small ones 2 x 4 x 32 (enough
-// blocks); measured per launch: 8.1 us vs 13.6 us at 64 x 96 x 64, 4.7 us at 32 x 48 x 32 (256 threads: 11.1
-// and 7.4 us).
-int launch_sweep_restrict(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs, float *coarse_rhs,
-                          Dims nc, float relaxation, const double *gate, hipStream_t s) {
-    const Dims n = cell_dims(L.g);
-    const int big_tiles = ((n.n0 + 3) / 4) * ((n.n1 + 7) / 8) * ((n.n2 + 31) / 32);
-    if (big_tiles >= 128)
-        launch_sweep_restrict_shape<4, 8, 32, 1024>(L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, s);
-    else
-        launch_sweep_restrict_shape<2, 4, 32, 1024>(L, q_in, q_out, rhs, coarse_rhs, nc, relaxation, gate, s);
-    return check_launch("sweep restrict");
+// blocks).
+inline bool big_smooth_tiles(const Dims &n) { return ((n.n0 + 3) / 4) * ((n.n1 + 7) / 8) * ((n.n2 + 31) / 32) >= 128; }
+
+template <int SWEEPS, int START, bool RESTRICT>
+int launch_level_smooth(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs, const float *coarse,
+                        Dims nq, float *coarse_rhs, Dims nc, float relaxation, const double *gate, hipStream_t s) {
+    if (big_smooth_tiles(cell_dims(L.g)))
+        return launch_level_smooth_shape<4, 8, 32, 1024, SWEEPS, START, RESTRICT>(
+            L, q_in, q_out, rhs, coarse, nq, coarse_rhs, nc, relaxation, gate, s);
+    return launch_level_smooth_shape<2, 4, 32, 1024, SWEEPS, START, RESTRICT>(L, q_in, q_out, rhs, coarse, nq,
+                                                                             coarse_rhs, nc, relaxation, gate, s);
 }
 
 // prolong_add_kernel, sim/multigrid.py:63-68
@@ -2364,12 +2442,11 @@ int allow_tail_lds(size_t bytes) {
     return 0;
 }
 
-// A flat level below the simulation grid with an even number (at least 2) of pre-smoothing sweeps and a level
-// of its own below it fuses its last pre-smoothing sweep with the residual and restriction
-bool fuses_restrict(const Hierarchy &h, int index) {
+// A flat level below the simulation grid with two sweeps a side and a level of its own below it smooths in LDS
+// (k_level_smooth)
+bool smooths_in_lds(const Hierarchy &h, int index) {
     const LevelArgs &L = h.level[index];
-    return !L.unit && L.flat == 1 && h.sweeps >= 2 && h.sweeps % 2 == 0 && h.residual[index] != nullptr &&
-           index + 1 < h.n_levels;
+    return !L.unit && L.flat == 1 && h.sweeps == 2 && h.residual[index] != nullptr && index + 1 < h.n_levels;
 }
 
 // _v_cycle, sim/multigrid.py:130-147, from level `index`; `cleared`: this level's q is the reference's
@@ -2394,21 +2471,19 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     const Dims nc = cell_dims(h.level[index + 1].g);
     // residual[index] doubles as the ping-pong buffer of this level's sweeps; the coarse q is cleared by the
     // next level's first sweep (or the tail), so it is not written here
-    if (fuses_restrict(h, index)) {
-        // an even count: the sweeps before the last ping-pong q -> residual -> ... and end in residual, from
-        // which k_sweep_restrict's sweep writes q and the coarse rhs
+    if (cleared && smooths_in_lds(h, index)) {
+        // pre-smoothing: a sweep from zero into q, then the second sweep with the residual and restriction from q
+        // into residual (the ping-pong buffer); post-smoothing: the prolongation and both sweeps from residual
+        // into q
         float *q = h.q[index], *scratch = h.residual[index];
-        for (int sw = 0; sw + 1 < h.sweeps; sw++) {
-            const bool to_scratch = sw % 2 == 0;
-            const float *src = to_scratch ? q : scratch, *no_coarse = nullptr;
-            float *dst = to_scratch ? scratch : q;
-            PLM_TRY(launch_tile<false>(L, sw == 0 ? start : FROM_Q, src, dst, h.rhs[index], no_coarse, none, 1.0f,
-                                       h.gate, s));
-        }
-        PLM_TRY(launch_sweep_restrict(L, scratch, q, h.rhs[index], h.rhs[index + 1], nc, 1.0f, h.gate, s));
+        const float *unread = scratch;
+        PLM_TRY(launch_tile<false>(L, FROM_ZERO, unread, q, h.rhs[index], unread, none, 1.0f, h.gate, s));
+        const int pre = launch_level_smooth<1, FROM_Q, true>(L, q, scratch, h.rhs[index], nullptr, none,
+                                                             h.rhs[index + 1], nc, 1.0f, h.gate, s);
+        PLM_TRY(pre);
         PLM_TRY(v_cycle(h, index + 1, true, s));
-        return launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, FROM_PROLONG,
-                             h.q[index + 1], nc, h.gate, s);
+        return launch_level_smooth<2, FROM_PROLONG, false>(L, scratch, q, h.rhs[index], h.q[index + 1], nc, nullptr,
+                                                           none, 1.0f, h.gate, s);
     }
     PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
                           h.gate, s));
