@@ -1548,7 +1548,10 @@ int launch_level_smooth_shape(const LevelArgs &L, const float *q_in, float *q_ou
 
 // 1024 threads a tile: each pass is one or two updates a thread, so a block's chain of passes stays short. Big
 // levels take 4 x 8 x 32 tiles (fewer halo loads and ring updates per cell), small ones 2 x 4 x 32 (enough
-// blocks).
+// blocks). Measured per launch at 64 x 96 x 64 (big) and 32 x 48 x 32 (small): the last pre-smoothing sweep with
+// the restriction 8.2 and 4.7 us (256 threads: 11.1 and 7.4; 2 x 4 x 32 tiles on the big level: 13.6); both
+// pre-smoothing sweeps with the restriction 17.4 and 7.0 us (a tile sweep and the one-sweep form: 13.4 and 9.6);
+// the prolongation with both post-smoothing sweeps 18.4 and 9.0 us (the tile sweeps: 17.4 and 14.4).
 inline bool big_smooth_tiles(const Dims &n) { return ((n.n0 + 3) / 4) * ((n.n1 + 7) / 8) * ((n.n2 + 31) / 32) >= 128; }
 
 template <int SWEEPS, int START, bool RESTRICT>
@@ -2472,18 +2475,29 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     // residual[index] doubles as the ping-pong buffer of this level's sweeps; the coarse q is cleared by the
     // next level's first sweep (or the tail), so it is not written here
     if (cleared && smooths_in_lds(h, index)) {
-        // pre-smoothing: a sweep from zero into q, then the second sweep with the residual and restriction from q
-        // into residual (the ping-pong buffer); post-smoothing: the prolongation and both sweeps from residual
-        // into q
+        // A small level: both pre-smoothing sweeps from zero with the residual and restriction into residual (the
+        // ping-pong buffer), then the prolongation and both post-smoothing sweeps from residual into q. A big
+        // level: a tile sweep from zero into residual and the second sweep with the residual and restriction into
+        // q, then the tile sweeps' post-smoothing (their box would repeat too much of the halo work).
         float *q = h.q[index], *scratch = h.residual[index];
-        const float *unread = scratch;
-        PLM_TRY(launch_tile<false>(L, FROM_ZERO, unread, q, h.rhs[index], unread, none, 1.0f, h.gate, s));
-        const int pre = launch_level_smooth<1, FROM_Q, true>(L, q, scratch, h.rhs[index], nullptr, none,
-                                                             h.rhs[index + 1], nc, 1.0f, h.gate, s);
-        PLM_TRY(pre);
+        const bool small = !big_smooth_tiles(cell_dims(L.g));
+        if (small) {
+            const int pre = launch_level_smooth<2, FROM_ZERO, true>(L, nullptr, scratch, h.rhs[index], nullptr, none,
+                                                                    h.rhs[index + 1], nc, 1.0f, h.gate, s);
+            PLM_TRY(pre);
+        } else {
+            const float *unread = q;
+            PLM_TRY(launch_tile<false>(L, FROM_ZERO, unread, scratch, h.rhs[index], unread, none, 1.0f, h.gate, s));
+            const int pre = launch_level_smooth<1, FROM_Q, true>(L, scratch, q, h.rhs[index], nullptr, none,
+                                                                 h.rhs[index + 1], nc, 1.0f, h.gate, s);
+            PLM_TRY(pre);
+        }
         PLM_TRY(v_cycle(h, index + 1, true, s));
-        return launch_level_smooth<2, FROM_PROLONG, false>(L, scratch, q, h.rhs[index], h.q[index + 1], nc, nullptr,
-                                                           none, 1.0f, h.gate, s);
+        if (small)
+            return launch_level_smooth<2, FROM_PROLONG, false>(L, scratch, q, h.rhs[index], h.q[index + 1], nc,
+                                                               nullptr, none, 1.0f, h.gate, s);
+        return launch_sweeps(L, q, h.rhs[index], scratch, h.sweeps, 1.0f, FROM_PROLONG, h.q[index + 1], nc, h.gate,
+                             s);
     }
     PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
                           h.gate, s));

@@ -554,11 +554,13 @@ def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
                                           ((100, 72, 88), DEFAULT_FACES), ((100, 72, 88), ALL_OPEN)],
                          ids=["96x128x96-open", "96x128x96-closed", "overhang-default", "overhang-open"])
 def test_fused_last_sweep_and_restriction_is_bit_exact(device, oracle32, cells, faces):
-    """Flat levels between the simulation grid and the LDS tail run their last pre-smoothing sweep, residual and
-    restriction as one launch, in 4 x 8 x 32 tiles on a level with at least 128 of them and 2 x 4 x 32 tiles
-    below that. 96 x 128 x 96 has one level of each (48 x 64 x 48, 24 x 32 x 24); 100 x 72 x 88 has a big-tile
-    level whose tiles overhang its end on every axis (50 x 36 x 44) and a level of odd extent (25 x 18 x 22)
-    whose last coarse cells have fewer than eight children."""
+    """Flat levels between the simulation grid and the LDS tail smooth in LDS tiles. A big level (at least 128
+    tiles of 4 x 8 x 32) runs its last pre-smoothing sweep with the residual and restriction as one launch; a
+    small one (2 x 4 x 32 tiles) runs both pre-smoothing sweeps with the restriction as one launch, and the
+    prolongation with both post-smoothing sweeps as another. 96 x 128 x 96 has one level of each (48 x 64 x 48,
+    24 x 32 x 24); 100 x 72 x 88 has a big level whose tiles overhang its end on every axis (50 x 36 x 44) and a
+    small level of odd extent (25 x 18 x 22) whose last coarse cells have fewer than eight children and whose
+    prolongation clamps at the coarse level's end."""
     from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
     from plume_amd.sim.multigrid import Multigrid, PLM_LDS_CELLS, build_levels
     from plume_amd.sim.pressure import project
