@@ -1551,7 +1551,8 @@ int launch_level_smooth_shape(const LevelArgs &L, const float *q_in, float *q_ou
 // blocks). Measured per launch at 64 x 96 x 64 (big) and 32 x 48 x 32 (small): the last pre-smoothing sweep with
 // the restriction 8.2 and 4.7 us (256 threads: 11.1 and 7.4; 2 x 4 x 32 tiles on the big level: 13.6); both
 // pre-smoothing sweeps with the restriction 17.4 and 7.0 us (a tile sweep and the one-sweep form: 13.4 and 9.6);
-// the prolongation with both post-smoothing sweeps 18.4 and 9.0 us (the tile sweeps: 17.4 and 14.4).
+// the prolongation with both post-smoothing sweeps 18.4 and 9.0 us (the tile sweeps: 17.4 and 14.4); on the big
+// level one sweep from zero 4.9 us (a tile sweep 5.5), the prolongation with one sweep 9.9 us (10.7).
 inline bool big_smooth_tiles(const Dims &n) { return ((n.n0 + 3) / 4) * ((n.n1 + 7) / 8) * ((n.n2 + 31) / 32) >= 128; }
 
 template <int SWEEPS, int START, bool RESTRICT>
@@ -2477,8 +2478,9 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
     if (cleared && smooths_in_lds(h, index)) {
         // A small level: both pre-smoothing sweeps from zero with the residual and restriction into residual (the
         // ping-pong buffer), then the prolongation and both post-smoothing sweeps from residual into q. A big
-        // level: a tile sweep from zero into residual and the second sweep with the residual and restriction into
-        // q, then the tile sweeps' post-smoothing (their box would repeat too much of the halo work).
+        // level (where a two-sweep box would repeat too much of the halo work) one sweep a launch: from zero
+        // into residual, the second with the residual and restriction into q; the prolongation with the first
+        // post-smoothing sweep into residual, the second (a tile sweep) into q.
         float *q = h.q[index], *scratch = h.residual[index];
         const bool small = !big_smooth_tiles(cell_dims(L.g));
         if (small) {
@@ -2486,8 +2488,9 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
                                                                     h.rhs[index + 1], nc, 1.0f, h.gate, s);
             PLM_TRY(pre);
         } else {
-            const float *unread = q;
-            PLM_TRY(launch_tile<false>(L, FROM_ZERO, unread, scratch, h.rhs[index], unread, none, 1.0f, h.gate, s));
+            const int first = launch_level_smooth<1, FROM_ZERO, false>(L, nullptr, scratch, h.rhs[index], nullptr,
+                                                                       none, nullptr, none, 1.0f, h.gate, s);
+            PLM_TRY(first);
             const int pre = launch_level_smooth<1, FROM_Q, true>(L, scratch, q, h.rhs[index], nullptr, none,
                                                                  h.rhs[index + 1], nc, 1.0f, h.gate, s);
             PLM_TRY(pre);
@@ -2496,8 +2499,11 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
         if (small)
             return launch_level_smooth<2, FROM_PROLONG, false>(L, scratch, q, h.rhs[index], h.q[index + 1], nc,
                                                                nullptr, none, 1.0f, h.gate, s);
-        return launch_sweeps(L, q, h.rhs[index], scratch, h.sweeps, 1.0f, FROM_PROLONG, h.q[index + 1], nc, h.gate,
-                             s);
+        const int post = launch_level_smooth<1, FROM_PROLONG, false>(L, q, scratch, h.rhs[index], h.q[index + 1], nc,
+                                                                     nullptr, none, 1.0f, h.gate, s);
+        PLM_TRY(post);
+        const float *no_coarse = nullptr;
+        return launch_tile<false>(L, FROM_Q, scratch, q, h.rhs[index], no_coarse, none, 1.0f, h.gate, s);
     }
     PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
                           h.gate, s));

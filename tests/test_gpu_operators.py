@@ -555,7 +555,8 @@ def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
                          ids=["96x128x96-open", "96x128x96-closed", "overhang-default", "overhang-open"])
 def test_fused_last_sweep_and_restriction_is_bit_exact(device, oracle32, cells, faces):
     """Flat levels between the simulation grid and the LDS tail smooth in LDS tiles. A big level (at least 128
-    tiles of 4 x 8 x 32) runs its last pre-smoothing sweep with the residual and restriction as one launch; a
+    tiles of 4 x 8 x 32) runs one sweep a launch: from zero, then the last pre-smoothing sweep with the residual
+    and restriction, then the prolongation with the first post-smoothing sweep (the second is a tile sweep); a
     small one (2 x 4 x 32 tiles) runs both pre-smoothing sweeps with the restriction as one launch, and the
     prolongation with both post-smoothing sweeps as another. 96 x 128 x 96 has one level of each (48 x 64 x 48,
     24 x 32 x 24); 100 x 72 x 88 has a big level whose tiles overhang its end on every axis (50 x 36 x 44) and a
