@@ -2165,21 +2165,33 @@ __device__ __forceinline__ void norm_partials(const LevelArgs &L, const float *_
         const bool jm = j >= 1, jp = j + 1 < n.n1, km = k >= 1, kp = k + 1 < n.n2;
         const int couples_jk = unit_couples(L.g, 1, jm, jp) + unit_couples(L.g, 2, km, kp);
         const uint32_t dj_lo = jm ? sj : 0u, dj_hi = jp ? sj : 0u, dk_lo = km ? 4u : 0u, dk_hi = kp ? 4u : 0u;
-        // rows past the level clamped to its last row, not counted
+        // rows past the level clamped to its last row, not counted. On unit levels the column of q from row
+        // ib - 1 to ib + NORM_I (clamped into the level) gives each row its value and its i neighbours, so a
+        // row's i neighbours are not loaded again: where the clamp applies the neighbour is masked.
         float qc[NORM_I], bc[NORM_I], qn[NORM_I][6];
+        float column[NORM_I + 2];
+        if (UNIT) {
+#pragma unroll
+            for (int m = 0; m < NORM_I + 2; m++) {
+                const int i = max(0, min(ib - 1 + m, n.n0 - 1));
+                column[m] = load_bytes(q, offset_of(n, i, j, k) * 4u);
+            }
+        }
 #pragma unroll
         for (int t = 0; t < NORM_I; t++) {
             const int i = min(ib + t, n.n0 - 1);
             const uint32_t o = offset_of(n, i, j, k) * 4u;
-            qc[t] = load_bytes(q, o);
             bc[t] = load_bytes(rhs, o);
             if (UNIT) {
-                qn[t][0] = load_bytes(q, i >= 1 ? o - si : o);
-                qn[t][1] = load_bytes(q, i + 1 < n.n0 ? o + si : o);
+                qc[t] = column[t + 1];
+                qn[t][0] = column[t];
+                qn[t][1] = column[t + 2];
                 qn[t][2] = load_bytes(q, o - dj_lo);
                 qn[t][3] = load_bytes(q, o + dj_hi);
                 qn[t][4] = load_bytes(q, o - dk_lo);
                 qn[t][5] = load_bytes(q, o + dk_hi);
+            } else {
+                qc[t] = load_bytes(q, o);
             }
         }
 #pragma unroll
@@ -2230,7 +2242,6 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_partials(LevelArgs L, con
 // [4 ...] 2 per block partials. `mode`: 0 plain norms (no gate change), 1 initial check, 2 after a cycle.
 __device__ __forceinline__ void norm_finish(double *scratch, int64_t blocks, double tolerance, int mode,
                                             int32_t *cycles_run) {
-    if (mode == 2 && scratch[1] == 0.0) return;
     const double *partials = scratch + 4;
     // fixed order: each thread's blocks in turn, a butterfly per wavefront, then the wavefronts in order (two
     // block barriers where an LDS tree over every thread took ten)
@@ -2241,6 +2252,9 @@ __device__ __forceinline__ void norm_finish(double *scratch, int64_t blocks, dou
         rr += partials[2 * b];
         bb += partials[2 * b + 1];
     }
+    // a closed gate (read once the partials are in flight; every thread reads the same word, which only thread
+    // 0 writes, after the barrier below) leaves the norms as they are
+    if (mode == 2 && scratch[1] == 0.0) return;
     rr = wave_sum(rr);
     bb = wave_sum(bb);
     if ((threadIdx.x & 63) == 0) {
