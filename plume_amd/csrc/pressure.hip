@@ -1548,7 +1548,8 @@ int launch_level_smooth_shape(const LevelArgs &L, const float *q_in, float *q_ou
 
 // 1024 threads a tile: each pass is one or two updates a thread, so a block's chain of passes stays short. Big
 // levels take 4 x 8 x 32 tiles (fewer halo loads and ring updates per cell), small ones 2 x 4 x 32 (enough
-// blocks). Measured per launch at 64 x 96 x 64 (big) and 32 x 48 x 32 (small): the last pre-smoothing sweep with
+// blocks), or 2 x 4 x 16 when the level has at most 16 cells along k (5.7 and 7.1 us at 16 x 24 x 16 against 7.0
+// and 8.7 with 32). Measured per launch at 64 x 96 x 64 (big) and 32 x 48 x 32 (small): the last pre-smoothing sweep with
 // the restriction 8.2 and 4.7 us (256 threads: 11.1 and 7.4; 2 x 4 x 32 tiles on the big level: 13.6); both
 // pre-smoothing sweeps with the restriction 17.4 and 7.0 us (a tile sweep and the one-sweep form: 13.4 and 9.6);
 // the prolongation with both post-smoothing sweeps 18.4 and 9.0 us (the tile sweeps: 17.4 and 14.4); on the big
@@ -1558,9 +1559,13 @@ inline bool big_smooth_tiles(const Dims &n) { return ((n.n0 + 3) / 4) * ((n.n1 +
 template <int SWEEPS, int START, bool RESTRICT>
 int launch_level_smooth(const LevelArgs &L, const float *q_in, float *q_out, const float *rhs, const float *coarse,
                         Dims nq, float *coarse_rhs, Dims nc, float relaxation, const double *gate, hipStream_t s) {
-    if (big_smooth_tiles(cell_dims(L.g)))
+    const Dims n = cell_dims(L.g);
+    if (big_smooth_tiles(n))
         return launch_level_smooth_shape<4, 8, 32, 1024, SWEEPS, START, RESTRICT>(
             L, q_in, q_out, rhs, coarse, nq, coarse_rhs, nc, relaxation, gate, s);
+    if (n.n2 <= 16)  // thin levels: a 32-cell k tile would be half empty
+        return launch_level_smooth_shape<2, 4, 16, 1024, SWEEPS, START, RESTRICT>(L, q_in, q_out, rhs, coarse, nq,
+                                                                                 coarse_rhs, nc, relaxation, gate, s);
     return launch_level_smooth_shape<2, 4, 32, 1024, SWEEPS, START, RESTRICT>(L, q_in, q_out, rhs, coarse, nq,
                                                                              coarse_rhs, nc, relaxation, gate, s);
 }
@@ -3321,8 +3326,14 @@ int vcycle_lp(int32_t n_levels, const plm_grid_info *infos, void *const *q, void
 
 // The level table of one solve: LevelArgs per level, the arrays, and the LDS tail (the first level from which
 // every coarser level fits in LDS together, with its staged table and size). Host only; allocates nothing.
+// A flat level below the simulation grid with more cells than this smooths faster in LDS tiles (k_level_smooth,
+// two launches a cycle) than inside the tail's one workgroup: at 128x192x128, 16x24x16 out of the tail takes the
+// tail from 38.5 to 21.4 us a cycle for 12.8 us of its own.
+constexpr int TAIL_SMOOTH_CELLS = 2048;
+
+// `lds_smoothing`: the single-grid V-cycle, which smooths flat levels with two sweeps a side in LDS tiles
 int build_hierarchy(Hierarchy &h, int32_t n_levels, const plm_grid_info *infos, float *const *q, float *const *rhs,
-                    float *const *residual, int32_t sweeps, const double *gate) {
+                    float *const *residual, int32_t sweeps, const double *gate, bool lds_smoothing) {
     h = Hierarchy{};
     h.n_levels = n_levels;
     for (int l = 0; l < n_levels; l++) h.level[l] = level_args(infos[l]);
@@ -3343,6 +3354,9 @@ int build_hierarchy(Hierarchy &h, int32_t n_levels, const plm_grid_info *infos, 
         const bool with = ones && g.fill_hi[0] == 1.0f && g.fill_hi[1] == 1.0f && g.fill_hi[2] == 1.0f;
         const int plan = with ? BLOCK_PLAN_ONES : BLOCK_PLAN;
         if (total + cells > PLM_LDS_CELLS || n_levels - l > MAX_TAIL || colour_slots(g) > plan * TAIL_THREADS) break;
+        if (lds_smoothing && l + 1 < n_levels && cells > TAIL_SMOOTH_CELLS && sweeps == 2 && !h.level[l].unit &&
+            h.level[l].flat == 1)
+            break;
         total += cells;
         ones = with;
         h.tail_start = l;
@@ -4129,7 +4143,8 @@ int plm_vcycle(int32_t n_levels, const plm_grid_info *infos, float *const *q, fl
     hipStream_t s = as_stream(stream);
 
     Hierarchy h;
-    PLM_TRY(build_hierarchy(h, n_levels, infos, q, rhs, residual, sweeps, tolerance_mode ? norm_scratch + 1 : nullptr));
+    PLM_TRY(build_hierarchy(h, n_levels, infos, q, rhs, residual, sweeps, tolerance_mode ? norm_scratch + 1 : nullptr,
+                            true));
 
     TimingScope t("multigrid", s);
     if (tolerance_mode && !resume)
@@ -4320,7 +4335,7 @@ int plm_mg_batch_create(int32_t members, const int32_t *n_levels, const plm_grid
                                 "plm_mg_batch_create: member %d level %d does not halve level %d", m, l, l - 1);
         }
         Hierarchy &h = bp.h[m];
-        PLM_TRY(build_hierarchy(h, n, infos[m], q[m], rhs[m], residual[m], sweeps, norm_scratch[m] + 1));
+        PLM_TRY(build_hierarchy(h, n, infos[m], q[m], rhs[m], residual[m], sweeps, norm_scratch[m] + 1, false));
         PLM_REQUIRE(h.tail_start < n, "plm_mg_batch_create: member %d has no LDS tail", m);
         PLM_TRY(h.tail_ones ? allow_batch_tail_lds<true>(h.tail_bytes) : allow_batch_tail_lds<false>(h.tail_bytes));
         // the reference's freshly zeroed coarse q, unless the whole hierarchy is the tail (warm start)

@@ -550,32 +550,54 @@ def test_multigrid_on_a_larger_grid_is_bit_exact(device, oracle32):
     assert bit_equal(to_np(gpu.pressure), orc.pressure)
 
 
-@pytest.mark.parametrize("cells, faces", [((96, 128, 96), ALL_OPEN), ((96, 128, 96), ALL_CLOSED),
-                                          ((100, 72, 88), DEFAULT_FACES), ((100, 72, 88), ALL_OPEN)],
-                         ids=["96x128x96-open", "96x128x96-closed", "overhang-default", "overhang-open"])
-def test_fused_last_sweep_and_restriction_is_bit_exact(device, oracle32, cells, faces):
+TAIL_SMOOTH_CELLS = 2048  # a flat level below the simulation grid with more cells stays out of the LDS tail
+
+
+def lds_smoothed_levels(levels, lds_cells):
+    """The kinds ("big", "small", "thin") of the levels the single-grid V-cycle smooths in LDS tiles (two sweeps a
+    side): flat levels below the simulation grid and above the tail. The tail starts at the first level, counted
+    from the coarsest, at which the levels so far would exceed lds_cells, or which is flat with more than
+    TAIL_SMOOTH_CELLS cells and a level below it."""
+    total, start = 0, len(levels)
+    for index in reversed(range(len(levels))):
+        cells = int(np.prod(levels[index].cells))
+        flat = all(f == 1.0 for f in levels[index].info.fill_hi)
+        if total + cells > lds_cells or (0 < index < len(levels) - 1 and flat and cells > TAIL_SMOOTH_CELLS):
+            break
+        total += cells
+        start = index
+    kinds = []
+    for index in range(1, start):
+        if all(f == 1.0 for f in levels[index].info.fill_hi):
+            n = levels[index].cells
+            big = ((n[0] + 3) // 4) * ((n[1] + 7) // 8) * ((n[2] + 31) // 32) >= 128
+            kinds.append("big" if big else "thin" if n[2] <= 16 else "small")
+    return kinds
+
+
+@pytest.mark.parametrize("cells, faces, kinds", [((96, 128, 96), ALL_OPEN, ["big", "small", "thin"]),
+                                                 ((96, 128, 96), ALL_CLOSED, ["big", "small", "thin"]),
+                                                 ((100, 72, 88), DEFAULT_FACES, ["big", "small"]),
+                                                 ((100, 72, 88), ALL_OPEN, ["big", "small"]),
+                                                 ((64, 96, 64), DEFAULT_FACES, ["small", "thin"])],
+                         ids=["96x128x96-open", "96x128x96-closed", "overhang-default", "overhang-open", "64x96x64"])
+def test_fused_last_sweep_and_restriction_is_bit_exact(device, oracle32, cells, faces, kinds):
     """Flat levels between the simulation grid and the LDS tail smooth in LDS tiles. A big level (at least 128
     tiles of 4 x 8 x 32) runs one sweep a launch: from zero, then the last pre-smoothing sweep with the residual
     and restriction, then the prolongation with the first post-smoothing sweep (the second is a tile sweep); a
-    small one (2 x 4 x 32 tiles) runs both pre-smoothing sweeps with the restriction as one launch, and the
-    prolongation with both post-smoothing sweeps as another. 96 x 128 x 96 has one level of each (48 x 64 x 48,
-    24 x 32 x 24); 100 x 72 x 88 has a big level whose tiles overhang its end on every axis (50 x 36 x 44) and a
-    small level of odd extent (25 x 18 x 22) whose last coarse cells have fewer than eight children and whose
-    prolongation clamps at the coarse level's end."""
+    small one (2 x 4 x 32 tiles, 2 x 4 x 16 on a thin level of at most 16 cells along k) runs both pre-smoothing
+    sweeps with the restriction as one launch, and the prolongation with both post-smoothing sweeps as another;
+    flat levels of more than 2048 cells stay out of the LDS tail. 96 x 128 x 96 has one level of each kind
+    (48 x 64 x 48, 24 x 32 x 24, 12 x 16 x 12); 100 x 72 x 88 has a big level whose tiles overhang its end on every
+    axis (50 x 36 x 44) and a small level of odd extent (25 x 18 x 22) whose last coarse cells have fewer than
+    eight children and whose prolongation clamps at the coarse level's end; 64 x 96 x 64 has 32 x 48 x 32 and the
+    thin 16 x 24 x 16, whose tail is then 8 x 12 x 8 and below."""
     from oracle.sim import Multigrid as OracleMultigrid, project as oracle_project
     from plume_amd.sim.multigrid import Multigrid, PLM_LDS_CELLS, build_levels
     from plume_amd.sim.pressure import project
 
     gpu, orc = pair(device, oracle32, cells, faces)
-    levels = build_levels(gpu)
-    tail, fused = 0, []
-    for index in reversed(range(len(levels))):
-        tail += int(np.prod(levels[index].cells))
-        flat = all(f == 1.0 for f in levels[index].info.fill_hi)
-        if tail > PLM_LDS_CELLS and 0 < index < len(levels) - 1 and flat:
-            n = levels[index].cells
-            fused.append(((n[0] + 3) // 4) * ((n[1] + 7) // 8) * ((n[2] + 31) // 32) >= 128)
-    assert sorted(fused) == [False, True], fused
+    assert lds_smoothed_levels(build_levels(gpu), PLM_LDS_CELLS) == kinds
     project(gpu, Multigrid(gpu, 2, 2))
     oracle_project(orc, OracleMultigrid(orc, 2, 2))
     assert bit_equal(to_np(gpu.pressure), orc.pressure)
