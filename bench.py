@@ -288,6 +288,7 @@ def ensemble_bench(args) -> dict:
     device = init_device("cuda")
     ens = Ensemble(ensemble_tables(args.ensemble_members, seed=20261204, frames=48), device,
                    batch_cells=args.ensemble_batch_cells or None)
+    ens.extra_cycles = args.ensemble_extra_cycles
     ens.autotune()
     cells = ens.cells
     print(f"[bench] ensemble of {len(ens.members)}: {cells} cells", file=sys.stderr, flush=True)
@@ -322,6 +323,7 @@ def ensemble_bench(args) -> dict:
             "batched": {"cell_substeps_per_s": batched, "seconds": batched_s, "substeps": batched_steps},
             "plain": {"cell_substeps_per_s": plain, "seconds": plain_s, "substeps": plain_steps},
             "speedup": batched / plain, "frames_timed": args.ensemble_frames,
+            "extra_cycles": ens.extra_cycles, "frames_redone_eagerly": list(ens.redone),
             "batched_multigrid": {"members": len(ens.batched), "max_cells": args.ensemble_batch_cells or None,
                                   "tiles_per_depth": [shape for shape, _ in ens.batch_tiles]},
             "max_final_relative_residual": max(residuals),
@@ -422,6 +424,9 @@ def main() -> int:
                         help="run the 32-member batched ensemble benchmark instead and exit")
     parser.add_argument("--ensemble-members", type=int, default=32)
     parser.add_argument("--ensemble-frames", type=int, default=4)
+    parser.add_argument("--ensemble-extra-cycles", type=int, default=1,
+                        help="device-gated V-cycles each replayed solve runs beyond the most its member needed "
+                             "lately (a frame whose solve stays open is re-run eagerly)")
     parser.add_argument("--ensemble-batch-cells", type=int, default=1_500_000,
                         help="members of at most this many cells share one batched multigrid (0: every member)")
     parser.add_argument("--mixed", action="store_true",
