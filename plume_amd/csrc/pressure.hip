@@ -1348,7 +1348,10 @@ struct SmoothBox {
     static constexpr int CI = QI / 2 + 2, CJ = QJ / 2 + 2, CK = QK / 2 + 2;
     static constexpr int COARSE = START == FROM_PROLONG ? CI * CJ * CK : 0;
     static constexpr int RES = RESTRICT ? TI * TJ * TK : 0;
-    static constexpr size_t BYTES = sizeof(float) * (size_t)(2 * CELLS + COARSE + RES);
+    // the prolongation's x and then y interpolations: per fine i over the coarse box's (j, k), then per fine
+    // (i, j) over its k
+    static constexpr int PX = START == FROM_PROLONG ? QI * CJ * CK : 0, PY = START == FROM_PROLONG ? QI * QJ * CK : 0;
+    static constexpr size_t BYTES = sizeof(float) * (size_t)(2 * CELLS + COARSE + RES + PX + PY);
 };
 
 // the lowest coarse tap of a fine index f >= 1 is (f - 1) / 2, of f = 0 it is 0; the box's lowest tap
@@ -1365,7 +1368,7 @@ __global__ __launch_bounds__(THREADS) void k_level_smooth(LevelArgs L, const flo
     constexpr int CJ = Box::CJ, CK = Box::CK, COARSE = Box::COARSE;
     constexpr int LOADS = (CELLS + THREADS - 1) / THREADS, COARSE_LOADS = (COARSE + THREADS - 1) / THREADS;
     extern __shared__ float lds[];
-    float *sq = lds, *sr = lds + CELLS, *sc = sr + CELLS, *sres = sc + COARSE;
+    float *sq = lds, *sr = lds + CELLS, *sc = sr + CELLS, *sres = sc + COARSE, *px = sres + Box::RES, *py = px + Box::PX;
     const Dims n = cell_dims(L.g);
     const int tile = (int)blockIdx.x, tr = tiles_k(tile), tk = tile - tr * (int)tiles_k.d;
     const int ti = tiles_j(tr), tj = tr - ti * (int)tiles_j.d;
@@ -1413,25 +1416,43 @@ __global__ __launch_bounds__(THREADS) void k_level_smooth(LevelArgs L, const flo
     }
     __syncthreads();
     if constexpr (START == FROM_PROLONG) {
-        // q + the coarse correction at each box cell inside the level (prolong_at_row's taps and order)
+        // q + the coarse correction at each box cell inside the level, separably: prolong_at_row's c00, c10, c01
+        // and c11 are the x interpolations at the cell's i for coarse (j, k) = (jc0, kc0), (jc1, kc0), (jc0, kc1)
+        // and (jc1, kc1); its two y interpolations are those at the cell's j for kc0 and kc1, and the z one joins
+        // them: the same taps, weights and order, each interpolation formed once for every box cell that uses it.
+        // Box rows outside the level take the clamped taps and are never used.
+        constexpr int CI = Box::CI, QI = Box::QI;
+        (void)CI;
+        auto taps = [](int f, int extent, int &c0, int &c1, float &w) {
+            const float x = clamp_((float)f * 0.5f - 0.25f, 0.0f, (float)(extent - 1));
+            c0 = (int)x;
+            c1 = min(c0 + 1, extent - 1);
+            w = x - (float)c0;
+        };
+        for (int idx = tid; idx < QI * CJ * CK; idx += THREADS) {
+            const int a = idx / (CJ * CK), rest = idx - a * (CJ * CK);
+            int c0, c1;
+            float w;
+            taps(i0 - H + a, nq.n0, c0, c1, w);
+            px[idx] = lerp_(sc[(c0 - ci0) * (CJ * CK) + rest], sc[(c1 - ci0) * (CJ * CK) + rest], w);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < QI * QJ * CK; idx += THREADS) {
+            const int row = idx / CK, kc = idx - row * CK, a = row / QJ, b = row - a * QJ;
+            int c0, c1;
+            float w;
+            taps(j0 - H + b, nq.n1, c0, c1, w);
+            py[idx] = lerp_(px[(a * CJ + (c0 - cj0)) * CK + kc], px[(a * CJ + (c1 - cj0)) * CK + kc], w);
+        }
+        __syncthreads();
         for (int idx = tid; idx < CELLS; idx += THREADS) {
-            const int a = idx / (QJ * QK), b = (idx / QK) % QJ, c = idx % QK;
+            const int row = idx / QK, c = idx - row * QK, a = row / QJ, b = row - a * QJ;
             const int i = i0 - H + a, j = j0 - H + b, k = k0 - H + c;
             if (!inside(i, j, k)) continue;
-            const float x = clamp_((float)i * 0.5f - 0.25f, 0.0f, (float)(nq.n0 - 1));
-            const float y = clamp_((float)j * 0.5f - 0.25f, 0.0f, (float)(nq.n1 - 1));
-            const float z = clamp_((float)k * 0.5f - 0.25f, 0.0f, (float)(nq.n2 - 1));
-            const int ic0 = (int)x, jc0 = (int)y, kc0 = (int)z;
-            const int ic1 = min(ic0 + 1, nq.n0 - 1), jc1 = min(jc0 + 1, nq.n1 - 1), kc1 = min(kc0 + 1, nq.n2 - 1);
-            const float tx = x - (float)ic0, ty = y - (float)jc0, tz = z - (float)kc0;
-            const int r00 = ((ic0 - ci0) * CJ + (jc0 - cj0)) * CK, r10 = ((ic1 - ci0) * CJ + (jc0 - cj0)) * CK;
-            const int r01 = ((ic0 - ci0) * CJ + (jc1 - cj0)) * CK, r11 = ((ic1 - ci0) * CJ + (jc1 - cj0)) * CK;
-            const int p0 = kc0 - ck0, p1 = kc1 - ck0;
-            const float c00 = lerp_(sc[r00 + p0], sc[r10 + p0], tx);
-            const float c10 = lerp_(sc[r01 + p0], sc[r11 + p0], tx);
-            const float c01 = lerp_(sc[r00 + p1], sc[r10 + p1], tx);
-            const float c11 = lerp_(sc[r01 + p1], sc[r11 + p1], tx);
-            sq[idx] = sq[idx] + lerp_(lerp_(c00, c10, ty), lerp_(c01, c11, ty), tz);
+            int c0, c1;
+            float w;
+            taps(k, nq.n2, c0, c1, w);
+            sq[idx] = sq[idx] + lerp_(py[row * CK + (c0 - ck0)], py[row * CK + (c1 - ck0)], w);
         }
         __syncthreads();
     }
