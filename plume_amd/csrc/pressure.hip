@@ -2520,7 +2520,7 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
         // ping-pong buffer), then the prolongation and both post-smoothing sweeps from residual into q. A big
         // level (where a two-sweep box would repeat too much of the halo work) one sweep a launch: from zero
         // into residual, the second with the residual and restriction into q; the prolongation with the first
-        // post-smoothing sweep into residual, the second (a tile sweep) into q.
+        // post-smoothing sweep into residual, the second into q.
         float *q = h.q[index], *scratch = h.residual[index];
         const bool small = !big_smooth_tiles(cell_dims(L.g));
         if (small) {
@@ -2542,8 +2542,8 @@ int v_cycle(Hierarchy &h, int index, bool cleared, hipStream_t s) {
         const int post = launch_level_smooth<1, FROM_PROLONG, false>(L, q, scratch, h.rhs[index], h.q[index + 1], nc,
                                                                      nullptr, none, 1.0f, h.gate, s);
         PLM_TRY(post);
-        const float *no_coarse = nullptr;
-        return launch_tile<false>(L, FROM_Q, scratch, q, h.rhs[index], no_coarse, none, 1.0f, h.gate, s);
+        return launch_level_smooth<1, FROM_Q, false>(L, scratch, q, h.rhs[index], nullptr, none, nullptr, none, 1.0f,
+                                                     h.gate, s);
     }
     PLM_TRY(launch_sweeps(L, h.q[index], h.rhs[index], h.residual[index], h.sweeps, 1.0f, start, nullptr, none,
                           h.gate, s));
