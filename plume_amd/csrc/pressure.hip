@@ -11,9 +11,13 @@
 //     see old values. Its first sweep of a level can start from q = 0 (the coarse q the reference zeroes)
 //     or from q + prolongation of the coarser correction (prolong_add fused into the load).
 //   - k_residual_restrict: the coarse right-hand side straight from q and rhs (the residual is never stored).
+//   - k_level_smooth: the flat levels between the simulation grid and the tail, one or two sweeps a launch in
+//     LDS tiles with a halo of two cells a sweep, fused with the prolongation before them or the residual and
+//     restriction after them.
 //   - k_mg_tail: every level from the first with few enough cells down to the coarsest, solved by one
-//     1024-thread workgroup in LDS, instead of ~90 launches per cycle at 128 x 192 x 128.
-// Multigrid kernels take an optional device gate (tolerance mode): when gate[0] == 0 they return at once.
+//     1024-thread workgroup in LDS, instead of ~60 launches per cycle at 128 x 192 x 128.
+// Multigrid kernels take an optional device gate (tolerance mode): when it reads 0 they do nothing (most read it
+// once their first loads are in flight).
 #include <algorithm>
 #include <array>
 #include <cstring>
